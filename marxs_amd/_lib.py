@@ -1,0 +1,76 @@
+"""ctypes binding of libmarxs_hip.so (declarations in include/marxs_hip.h).
+
+The library is loaded after torch so that it shares torch's HIP runtime
+(both carry the soname libamdhip64.so.7).  There is no fallback: if the
+library is missing the import fails with instructions to build it.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime first)
+
+__all__ = ['lib', 'MxTables', 'MxPhotons', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
+           'MX_NSTATUS', 'LIB_PATH', 'check']
+
+MX_MAX_FCOLS = 16
+MX_MAX_ICOLS = 8
+MX_NSTATUS = 8
+MX_ABI_VERSION = 4
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
+
+P = ctypes.c_void_p
+
+
+class MxTables(ctypes.Structure):
+    _fields_ = [('n_stages', ctypes.c_int32), ('n_elems', ctypes.c_int32),
+                ('stage_i', P), ('stage_d', P), ('elem_t', P), ('elem_p', P),
+                ('elem_i', P), ('elem_t32', P), ('elem_p32', P), ('sel_i', P),
+                ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P)]
+
+
+class MxPhotons(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int64), ('read_pos', ctypes.c_int32),
+                ('n_uniform_slots', ctypes.c_int32), ('uniform_stride', ctypes.c_int64),
+                ('pos', P * 3), ('dir', P * 3), ('pol', P * 3),
+                ('energy', P), ('prob', P), ('uniforms', P),
+                ('fcol', P * MX_MAX_FCOLS), ('icol', P * MX_MAX_ICOLS),
+                ('pos_in', P * 3), ('dir_in', P * 3), ('pol_in', P * 3), ('prob_in', P)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f'{LIB_PATH} not found: build it with `python -m marxs_amd.csrc.build` '
+            '(or __graft_entry__.build()).  The tracer has no CPU implementation.')
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.mx_abi_version.restype = ctypes.c_int
+    lib.mx_last_error.restype = ctypes.c_char_p
+    lib.mx_trace.restype = ctypes.c_int
+    lib.mx_trace.argtypes = [ctypes.POINTER(MxTables), ctypes.POINTER(MxPhotons),
+                             ctypes.c_int32, ctypes.c_int32, P, P]
+    lib.mx_fill_i32.restype = ctypes.c_int
+    lib.mx_fill_i32.argtypes = [P, ctypes.c_int64, ctypes.c_int32, P]
+    if hasattr(lib, 'mx_trace_batched'):
+        lib.mx_trace_batched.restype = ctypes.c_int
+        lib.mx_trace_batched.argtypes = [ctypes.POINTER(MxTables), ctypes.POINTER(MxPhotons),
+                                         ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32,
+                                         P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, P, P]
+    if hasattr(lib, 'mx_adjoint'):
+        lib.mx_adjoint.restype = ctypes.c_int
+        lib.mx_adjoint.argtypes = [P, P, P, P, P, P, P, ctypes.c_int64, P, P, ctypes.c_int32,
+                                   P, ctypes.c_int32, ctypes.c_int32, P, P, P,
+                                   ctypes.c_int32, P]
+    v = lib.mx_abi_version()
+    if v != MX_ABI_VERSION:
+        raise ImportError(f'libmarxs_hip.so ABI {v}, expected {MX_ABI_VERSION}: rebuild it')
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError('libmarxs_hip: ' + lib.mx_last_error().decode(errors='replace'))

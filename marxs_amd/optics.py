@@ -1,0 +1,292 @@
+"""Optical elements on the traced path.
+
+Each class keeps the constructor keywords and attributes of its reference
+counterpart so that an instrument description written for the reference
+builds unchanged.  Calling an element (or a container) traces the photons on
+the GPU: the element is compiled into a one-stage instrument and run through
+the HIP kernels (:mod:`marxs_amd.tracer`).  There is no per-photon host code.
+
+Reference counterparts:
+
+* ``OpticalElement``/``FlatOpticalElement`` -- ``marxs/optics/base.py:54-215``
+* ``OrderSelector``/``EfficiencyFile`` -- ``marxs/optics/grating.py:12-96``
+* ``FlatGrating``/``CATGrating`` -- ``marxs/optics/grating.py:99-301``
+* ``NonParallelCATGrating`` -- ``marxs/missions/mitsnl/catgrating.py:280-311``
+* ``CircleAperture``/``RectangleAperture`` -- ``marxs/optics/aperture.py:11-152``
+* ``PerfectLens``/``ThinLens`` -- ``marxs/optics/mirror.py:11-148``
+* ``FlatDetector`` -- ``marxs/optics/detector.py:18-75``
+"""
+import math
+import warnings
+
+import numpy as np
+
+from .base import (SimulationSequenceElement, GeometryError,
+                   SimulationSetupWarning)
+from .geometry import Geometry, FinitePlane, CircularHole, RectangleHole
+from .mathutils import decompose44
+
+__all__ = ['OpticalElement', 'FlatOpticalElement', 'OrderSelector',
+           'EfficiencyFile', 'FlatGrating', 'CATGrating',
+           'NonParallelCATGrating', 'FlatAperture', 'CircleAperture',
+           'RectangleAperture', 'PerfectLens', 'ThinLens', 'FlatDetector']
+
+
+class OpticalElement(SimulationSequenceElement):
+    """An element with a geometry; ``__call__`` traces photons on the GPU."""
+
+    default_geometry = FinitePlane
+
+    def __init__(self, **kwargs):
+        geometry = kwargs.pop('geometry', self.default_geometry)
+        if isinstance(geometry, Geometry):
+            self.geometry = geometry
+        elif isinstance(geometry, type) and issubclass(geometry, Geometry):
+            self.geometry = geometry(kwargs)
+        else:
+            raise TypeError('geometry must be a Geometry class or instance')
+        super().__init__(**kwargs)
+        if not hasattr(self, 'loc_coos_name'):
+            self.loc_coos_name = self.geometry.loc_coos_name
+
+    @property
+    def pos4d(self):
+        return self.geometry.pos4d
+
+    def __call__(self, photons, **kwargs):
+        from .tracer import trace
+        return trace(self, photons, **kwargs)
+
+    def process_photons(self, photons, **kwargs):
+        from .tracer import trace
+        return trace(self, photons, **kwargs)
+
+
+class FlatOpticalElement(OpticalElement):
+    pass
+
+
+# ---------------------------------------------------------------- selectors
+class OrderSelector:
+    """Energy-independent order distribution (``grating.py:12-57``).
+
+    A photon goes to ``orderlist[j]`` for the first ``j`` with
+    ``cdf[j] > u`` where ``cdf = cumsum(p / sum(p))`` renormalised to end at
+    1 (the inverse-CDF rule of numpy's legacy ``RandomState.choice``); its
+    probability is multiplied by ``sum(p)``.
+    """
+
+    def __init__(self, orderlist, p=None):
+        self.orderlist = orderlist
+        if p is None:
+            self.p = np.ones(len(orderlist)) / len(orderlist)
+        else:
+            p = np.asanyarray(p)
+            if len(p) != len(orderlist):
+                raise ValueError('Number of elements in orderlist and '
+                                 'probabilities does not match.')
+            if p.sum() > 1.:
+                raise ValueError('Sum of all probabilities must be <= 1.')
+            if np.any(p < 0):
+                raise ValueError('Probabilities cannot be negative')
+            self.p = p
+
+    def table(self):
+        """(energy grid or None, cdf rows (R, M), row totals (R,), orders)."""
+        p_sum = self.p.sum()
+        cdf = np.cumsum(self.p / p_sum)
+        cdf = cdf / cdf[-1]
+        return (None, cdf[None, :].astype(float), np.array([p_sum], dtype=float),
+                np.asarray(self.orderlist, dtype=np.int64))
+
+
+class EfficiencyFile:
+    """Order probabilities tabulated on an energy grid (``grating.py:60-96``).
+
+    Row = nearest grid energy (first on ties); order = first ``j`` with
+    ``cumprob[row, j] > u``; probability multiplied by the row total.  When
+    no entry exceeds ``u`` the trace raises ``ValueError`` like the reference.
+    """
+
+    def __init__(self, filename, orders):
+        dat = np.loadtxt(filename, ndmin=2)
+        self.energy = dat[:, 0]
+        if len(orders) != (dat.shape[1] - 1):
+            raise ValueError('orders has len={0}, but data files has {1} '
+                             'order columns.'.format(len(orders), dat.shape[1] - 1))
+        self.orders = np.array(orders)
+        self._set_prob(dat[:, 1:])
+
+    @classmethod
+    def from_arrays(cls, energy, prob, orders):
+        self = cls.__new__(cls)
+        self.energy = np.asarray(energy, dtype=float)
+        self.orders = np.asarray(orders)
+        prob = np.asarray(prob, dtype=float)
+        if prob.shape != (len(self.energy), len(self.orders)):
+            raise ValueError('prob must have shape (n_energy, n_orders)')
+        self._set_prob(prob)
+        return self
+
+    def _set_prob(self, prob):
+        self.prob = prob
+        self.totalprob = np.sum(self.prob, axis=1)
+        self.cumprob = np.cumsum(self.prob, axis=1) / self.totalprob[:, None]
+
+    def table(self):
+        return (self.energy.astype(float), self.cumprob.astype(float),
+                self.totalprob.astype(float), self.orders.astype(np.int64))
+
+
+# ----------------------------------------------------------------- gratings
+class FlatGrating(FlatOpticalElement):
+    """Thin flat grating (``grating.py:99-277``).
+
+    Diffraction follows the grating equation in the groove frame
+    ``(l, d, n)``: ``p_d' = p.d + s m lambda / D``, ``p_l' = p.l``,
+    ``p_n' = sqrt(1 - p_d'^2 - p_l'^2)`` with the sign of ``p.n`` kept
+    (flipped for reflection gratings); polarization is parallel-transported.
+    """
+    loc_coos_name = ['grat_y', 'grat_z']
+    order_name = 'order'
+    blaze_name = 'blaze'
+    sign_convention = 'flat'       # s = -1
+
+    def __init__(self, **kwargs):
+        self.order_selector = kwargs.pop('order_selector')
+        self.transmission = kwargs.pop('transmission', True)
+        if 'd' not in kwargs:
+            raise ValueError('Input parameter "d" (Grating constant) is required.')
+        self._d = kwargs.pop('d')
+        groove_angle = kwargs.pop('groove_angle', 0.)
+        super().__init__(**kwargs)
+        self.geometry._geometry['groove_angle'] = groove_angle
+
+    def d(self, intercoos=None):
+        if callable(self._d):
+            if intercoos is None:
+                raise ValueError('callable d needs intercoos')
+            return self._d(intercoos)
+        return self._d
+
+    def groove_frame(self):
+        """Per-facet constants ``(e_groove, e_perp_groove, n)`` as 3-vectors
+        (``grating.py:186-207``; the local bases are ``(e_y, e_z, e_x)``,
+        ``geometry.py:263-281``)."""
+        g = self.geometry
+        groove = g['groove_angle']
+        ex, ey, en = g['e_y'], g['e_z'], g['e_x']
+        e_groove = math.sin(-groove) * ex + math.cos(-groove) * ey
+        e_perp = math.cos(-groove) * ex - math.sin(-groove) * ey
+        return e_groove[:3], e_perp[:3], en[:3]
+
+    def blaze_modifier_params(self):
+        """``(blaze_center, d_blaze_mm)``: modifier = center + y_local * slope."""
+        return 0., 0.
+
+
+class CATGrating(FlatGrating):
+    """Critical-angle transmission grating: ``s = sign(p . d)``, 0 -> +1
+    (``grating.py:280-301``)."""
+    sign_convention = 'cat'
+
+
+class NonParallelCATGrating(CATGrating):
+    """CAT grating whose bar walls tilt linearly across the dispersion
+    direction (``catgrating.py:280-311``)."""
+
+    def __init__(self, **kwargs):
+        self.d_blaze_mm = kwargs.pop('d_blaze_mm', 0)
+        self.blaze_center = kwargs.pop('blaze_center', 0)
+        super().__init__(**kwargs)
+
+    def blaze_modifier_params(self):
+        return float(self.blaze_center), float(self.d_blaze_mm)
+
+
+# ---------------------------------------------------------------- apertures
+class FlatAperture(FlatOpticalElement):
+    """Start rays on the entrance plane; probability *= clip(dir . -e_x, 0, 1)
+    (``aperture.py:39-78``).  Each photon consumes two uniform draws."""
+    n_uniforms = 2
+
+
+class RectangleAperture(FlatAperture):
+    """Uniform on the rectangle: ``y = 2 u0 - 1``, ``z = 2 u1 - 1`` in
+    units of ``v_y``/``v_z`` (``aperture.py:81-100``)."""
+    default_geometry = RectangleHole
+
+
+class CircleAperture(FlatAperture):
+    """Uniform on an annulus / wedge: ``phi = phi0 + (phi1 - phi0) u0``,
+    ``r = sqrt(ri^2 + (1 - ri^2) u1)`` with ``ri = r_inner / |v_y|``
+    (``aperture.py:103-152``)."""
+    default_geometry = CircularHole
+
+    def __init__(self, **kwargs):
+        super().__init__(**kwargs)
+        if self.geometry['r_inner'] > np.linalg.norm(self.geometry['v_y']):
+            raise ValueError('r_inner must be less than size of full aperture.')
+        if not np.isclose(np.linalg.norm(self.geometry['v_y']),
+                          np.linalg.norm(self.geometry['v_z'])):
+            raise GeometryError('Aperture does not have the same size in y, z direction.')
+
+    @property
+    def area(self):
+        g = self.geometry
+        a = np.pi * (np.linalg.norm(g['v_y']) ** 2 - g['r_inner'] ** 2)
+        return (g.phi[1] - g.phi[0]) / (2 * np.pi) * a
+
+
+# ------------------------------------------------------------------- lenses
+class PerfectLens(FlatOpticalElement):
+    """Redirect every ray to ``center - d_off e_z + f * dir`` (``mirror.py:11-82``).
+
+    Only the default reflectivity (1) is compiled; a custom
+    ``reflectivity_interpolator`` is rejected at compile time.
+    """
+    loc_coos_name = ['mirror_x', 'mirror_y']
+
+    def __init__(self, **kwargs):
+        self.focallength = kwargs.pop('focallength')
+        self.d_center_optax = kwargs.pop('d_center_optical_axis', 0)
+        self.reflectivity_interpolator = kwargs.pop('reflectivity_interpolator', None)
+        super().__init__(**kwargs)
+
+
+class ThinLens(FlatOpticalElement):
+    """Thin-lens rotation of each ray by ``-|q - c| / f`` about
+    ``dir x (q - c)`` (``mirror.py:85-148``)."""
+
+    def __init__(self, **kwargs):
+        self.focallength = kwargs.pop('focallength')
+        super().__init__(**kwargs)
+
+
+# ---------------------------------------------------------------- detectors
+class FlatDetector(FlatOpticalElement):
+    """Flat pixelated detector (``detector.py:18-75``).
+
+    ``detpix = local / pixsize + centerpix`` with ``npix = round(2 zoom /
+    pixsize)`` and ``centerpix = (npix - 1) / 2``.  The trace also reports the
+    integer pixel index ``floor(detpix + 0.5)`` (-1 when outside
+    ``[0, npix)``).
+    """
+    loc_coos_name = ['det_x', 'det_y']
+    detpix_name = ['detpix_x', 'detpix_y']
+    pixel_name = ['pix_x', 'pix_y']
+
+    def __init__(self, pixsize=1, ignore_pixel_warning=False, **kwargs):
+        self.pixsize = pixsize
+        super().__init__(**kwargs)
+        t, r, zoom, s = decompose44(self.pos4d)
+        self.npix = [0, 0]
+        self.centerpix = [0, 0]
+        for i in (0, 1):
+            z = zoom[i + 1]
+            self.npix[i] = int(np.round(2. * z / self.pixsize))
+            if (np.abs(2. * z / self.pixsize - self.npix[i]) > 1e-2) and not ignore_pixel_warning:
+                warnings.warn('Detector size is not an integer multiple of pixel '
+                              'size in direction {0}. It will be rounded.'.format('xy'[i]),
+                              SimulationSetupWarning)
+            self.centerpix[i] = (self.npix[i] - 1) / 2

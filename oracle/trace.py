@@ -1,0 +1,419 @@
+"""numpy float64 restatement of the traced path (TEST INFRASTRUCTURE ONLY).
+
+Instrument description: a list of plain dicts, one per leaf element in
+execution order (see ``element dicts`` below).  Photons: dict of numpy
+arrays ``pos`` (N,3) (optional), ``dir`` (N,3), ``pol`` (N,3), ``energy``
+(N,), ``prob`` (N,).  Uniforms: (n_slots, N).
+
+Each function names the reference code it restates.  Arithmetic is written
+out component by component in a fixed left-to-right order (the reference's
+``np.dot``/``einsum`` leave the summation order to BLAS); numpy never fuses
+multiply-adds, so these are plain IEEE double operations.
+
+How each part is pinned (tests/test_oracle_*.py):
+
+* ``intersect`` -- known answers of ``marxs/math/tests/test_geometry.py:11-55``
+  and a brute-force check against the closed-form ray-plane solution.
+* ``parallel_transport`` -- ``marxs/math/tests/test_polarization.py:36-72``
+  invariants (identity for equal directions, s component kept, unit length,
+  perpendicular to the new direction) and a Rodrigues-rotation closed form.
+* ``diffract`` / ``select_order`` -- grating equation known answers of
+  ``marxs/optics/tests/test_grating.py`` (``sin a = sin b + lambda/d`` at
+  0/5/10/20 deg, order symmetry, CAT sign pattern [1,1,1,-1,-1],
+  EfficiencyFile totals [0.7, 0.8] and 0.6), and numpy's own
+  ``RandomState.choice`` run with a fixed seed for the inverse-CDF rule.
+* ``aperture_*`` -- ``test_apertures.py`` radius/quadrant/projected-area
+  answers ``[1, 1/sqrt2, 0, 0, 1/sqrt2]`` and the closed-form sampling map.
+* ``perfect_lens`` -- ``test_mirror.py`` focus answers (spread < 1e-4 at
+  ``d = f cos(ra)``; off-axis equivalence).
+* ``thin_lens`` -- focusing of paraxial rays (closed form) and unit
+  rotation invariants.
+* ``detector`` -- ``test_detector.py:16-35`` pixel answers.
+"""
+import math
+
+import numpy as np
+
+E2W = 1.2398419292004202e-06   # keV -> mm, marxs/__init__.py:14
+ORDER_NONE = -2 ** 31
+
+APERTURES = ('aperture_circle', 'aperture_rect')
+
+
+# ------------------------------------------------------------------ helpers
+def dot(a, b):
+    """Row-wise a . b for (N,3) a and (3,) or (N,3) b, fixed order."""
+    if b.ndim == 1:
+        return a[:, 0] * b[0] + a[:, 1] * b[1] + a[:, 2] * b[2]
+    return a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2]
+
+
+def unit(v):
+    """norm_vector, marxs/math/utils.py:150-164."""
+    n = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+    return v / n[:, None]
+
+
+def cross(a, b):
+    """np.cross component formulas."""
+    return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1],
+                     a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                     a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], axis=1)
+
+
+def frame(pos4d):
+    """Geometry.__getitem__ (marxs/math/geometry.py:108-160) for a flat
+    element: centre, unit axes and half sizes."""
+    pos4d = np.asarray(pos4d, dtype=float)
+    cols = [pos4d[:, i] for i in range(3)]
+    e = [c / np.linalg.norm(c) for c in cols]
+    return {'center': pos4d[:3, 3].copy(), 'e_x': e[0][:3], 'e_y': e[1][:3], 'e_z': e[2][:3],
+            'v_y': cols[1][:3].copy(), 'v_z': cols[2][:3].copy(),
+            'hy': float(np.linalg.norm(cols[1])), 'hz': float(np.linalg.norm(cols[2]))}
+
+
+def zooms(pos4d):
+    """Column lengths of the linear part after orthogonalisation (QR)."""
+    q, r = np.linalg.qr(np.asarray(pos4d, dtype=float)[:3, :3])
+    return np.abs(np.diag(r))
+
+
+# ----------------------------------------------------------- intersection
+def intersect(fr, pos, dir):
+    """FinitePlane.intersect, marxs/math/geometry.py:211-261."""
+    c = fr['center']
+    kn = dot(c[None, :] - pos, fr['e_x'])
+    kd = dot(dir, fr['e_x'])
+    with np.errstate(divide='ignore', invalid='ignore'):
+        k = kn / kd
+        q = pos + k[:, None] * dir
+        v = q - c[None, :]
+        ly = dot(v, fr['e_y'])
+        lz = dot(v, fr['e_z'])
+        hit = (kd != 0) & (k >= 0) & (np.abs(ly) <= fr['hy']) & (np.abs(lz) <= fr['hz'])
+    return hit, q, ly, lz
+
+
+# ------------------------------------------------------------ polarization
+def parallel_transport(d1, d2, pol):
+    """paralleltransport_matrix with J = I and parallel_transport,
+    marxs/math/polarization.py:90-170.  Rows with |d1 x d2| <= 1e-8
+    (np.isclose to 0) keep their polarization."""
+    a = unit(d1)
+    b = unit(d2)
+    s = cross(a, b)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        sn = np.sqrt(s[:, 0] * s[:, 0] + s[:, 1] * s[:, 1] + s[:, 2] * s[:, 2])
+        same = sn <= 1e-8
+        s = s / sn[:, None]
+        pin = cross(a, s)
+        pout = cross(b, s)
+        out = np.empty_like(pol)
+        # pmat[j][l] = s_j s_l + pout_j pin_l + b_j a_l
+        m = np.empty((len(pol), 3, 3))
+        for j in range(3):
+            for l in range(3):
+                m[:, j, l] = s[:, j] * s[:, l] + pout[:, j] * pin[:, l] + b[:, j] * a[:, l]
+        for j in range(3):
+            out[:, j] = m[:, j, 0] * pol[:, 0] + m[:, j, 1] * pol[:, 1] + m[:, j, 2] * pol[:, 2]
+    out[same] = pol[same]
+    return out
+
+
+# -------------------------------------------------------- order selection
+def selector_table(sel):
+    """CDF table of an order selector description.
+
+    ``{'type': 'orderselector', 'orders', 'p'}`` follows OrderSelector
+    (grating.py:39-57) and numpy's legacy ``choice`` (cdf = cumsum(p/sum p),
+    renormalised so that the last entry is 1).  ``{'type': 'efficiency',
+    'energy', 'prob', 'orders'}`` follows EfficiencyFile (grating.py:78-96).
+    """
+    if sel['type'] == 'orderselector':
+        p = np.asarray(sel['p'], dtype=float)
+        p_sum = p.sum()
+        cdf = np.cumsum(p / p_sum)
+        cdf = cdf / cdf[-1]
+        return {'energy': None, 'cdf': cdf[None, :], 'tot': np.array([p_sum]),
+                'orders': np.asarray(sel['orders'], dtype=np.int64)}
+    prob = np.asarray(sel['prob'], dtype=float)
+    tot = np.sum(prob, axis=1)
+    return {'energy': np.asarray(sel['energy'], dtype=float),
+            'cdf': np.cumsum(prob, axis=1) / tot[:, None], 'tot': tot,
+            'orders': np.asarray(sel['orders'], dtype=np.int64)}
+
+
+def select_order(tab, energy, u):
+    """Nearest energy row (first on ties), then the first order whose
+    cumulative probability exceeds u.  Returns (order, prob, ok)."""
+    if tab['energy'] is None or len(tab['energy']) == 1:
+        row = np.zeros(len(energy), dtype=int)
+    else:
+        row = np.argmin(np.abs(tab['energy'][None, :] - energy[:, None]), axis=1)
+    cdf = tab['cdf'][row]
+    above = cdf > u[:, None]
+    ok = above.any(axis=1)
+    j = np.argmax(above, axis=1)
+    order = np.where(ok, tab['orders'][j], ORDER_NONE)
+    return order, tab['tot'][row], ok
+
+
+# ----------------------------------------------------------------- elements
+def groove_frame(fr, groove_angle):
+    """FlatGrating.e_groove_coos, grating.py:186-207."""
+    ex, ey, en = fr['e_y'], fr['e_z'], fr['e_x']
+    e_groove = math.sin(-groove_angle) * ex + math.cos(-groove_angle) * ey
+    e_perp = math.cos(-groove_angle) * ex - math.sin(-groove_angle) * ey
+    return e_groove, e_perp, en
+
+
+def diffract(el, fr, dir, energy, ly, order, ptot):
+    """FlatGrating.diffract_photons, grating.py:233-266 (orders given).
+    Returns new direction and blaze angle."""
+    p = unit(dir)
+    l, e_perp, n = groove_frame(fr, el.get('groove_angle', 0.))
+    dg = -e_perp
+    wave = E2W / energy
+    with np.errstate(invalid='ignore', divide='ignore'):
+        pl = dot(p, l)
+        pperp = unit(p - pl[:, None] * l[None, :])
+        blaze = np.arccos(np.clip(np.abs(dot(pperp, n)), 0, 1))
+        blaze = blaze + (el.get('blaze_center', 0.) + ly * el.get('d_blaze_mm', 0.))
+        pd0 = dot(p, dg)
+        if el.get('sign', 'flat') == 'cat':
+            sign = np.sign(pd0)
+            sign[sign == 0] = 1
+        else:
+            sign = -np.ones(len(pd0))
+        p_d = pd0 + ((sign * order) * wave) / el['d']
+        p_l = pl
+        p_n = np.sqrt(1. - p_d * p_d - p_l * p_l)
+        direction = np.sign(dot(p, n))
+        if not el.get('transmission', True):
+            direction = direction * -1
+        w = direction * p_n
+        new = (p_d[:, None] * dg[None, :] + p_l[:, None] * l[None, :]) + w[:, None] * n[None, :]
+    return new, blaze
+
+
+def aperture_points(el, fr, u0, u1):
+    """CircleAperture / RectangleAperture generate_local_xy and
+    FlatAperture.specific_process_photons (aperture.py:68-78,92-95,138-146)."""
+    if el['kind'] == 'aperture_circle':
+        phi0, phi1 = el.get('phi', [0, 2 * np.pi])
+        phi = phi0 + (phi1 - phi0) * u0
+        rin = el.get('r_inner', 0.) / np.linalg.norm(np.asarray(el['pos4d'])[:, 1])
+        low = rin ** 2
+        r = np.sqrt(low + (1. - low) * u1)
+        x = r * np.cos(phi)
+        y = r * np.sin(phi)
+    else:
+        x = u0 * 2. - 1.
+        y = u1 * 2. - 1.
+    c, vy, vz = fr['center'], fr['v_y'], fr['v_z']
+    q = c[None, :] + x[:, None] * vy[None, :] + y[:, None] * vz[None, :]
+    return x, y, q
+
+
+def perfect_lens(el, fr, dir, q, pol):
+    """PerfectLens.specific_process_photons, mirror.py:53-82 (reflectivity 1)."""
+    pos4d = np.asarray(el['pos4d'], dtype=float)
+    e_z4 = pos4d[:, 2] / np.linalg.norm(pos4d[:, 2])
+    popt = (pos4d[:, 3] - el.get('d_center_optax', 0.) * e_z4)[:3]
+    dn = unit(dir)
+    fp = popt[None, :] + el['focallength'] * dn
+    new = unit(fp - q)
+    return new, parallel_transport(dir, new, pol)
+
+
+def thin_lens(el, fr, dir, q, pol):
+    """ThinLens.process_photon, mirror.py:130-148, with the axis-angle matrix
+    of transforms3d.axangles.axangle2mat in its operation order."""
+    c = fr['center']
+    r = q - c[None, :]
+    dist = np.sqrt(r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1] + r[:, 2] * r[:, 2])
+    new_d = dir.copy()
+    new_p = pol.copy()
+    mv = dist != 0
+    if mv.any():
+        ang = -dist[mv] / el['focallength']
+        ax = cross(dir[mv], r[mv])
+        an = np.sqrt(ax[:, 0] * ax[:, 0] + ax[:, 1] * ax[:, 1] + ax[:, 2] * ax[:, 2])
+        x, y, z = ax[:, 0] / an, ax[:, 1] / an, ax[:, 2] / an
+        cc, ss = np.cos(ang), np.sin(ang)
+        C = 1 - cc
+        xs, ys, zs = x * ss, y * ss, z * ss
+        xC, yC, zC = x * C, y * C, z * C
+        xyC, yzC, zxC = x * yC, y * zC, z * xC
+        m = [[x * xC + cc, xyC - zs, zxC + ys],
+             [xyC + zs, y * yC + cc, yzC - xs],
+             [zxC - ys, yzC + xs, z * zC + cc]]
+        dm, pm = dir[mv], pol[mv]
+        for i in range(3):
+            new_d[mv, i] = m[i][0] * dm[:, 0] + m[i][1] * dm[:, 1] + m[i][2] * dm[:, 2]
+            new_p[mv, i] = m[i][0] * pm[:, 0] + m[i][1] * pm[:, 1] + m[i][2] * pm[:, 2]
+    return new_d, new_p
+
+
+def detector_params(el):
+    """FlatDetector.__init__, detector.py:59-70."""
+    z = zooms(el['pos4d'])
+    npix = [int(np.round(2. * z[i + 1] / el['pixsize'])) for i in (0, 1)]
+    centerpix = [(n - 1) / 2 for n in npix]
+    return npix, centerpix
+
+
+def pixel_index(detpix, npix):
+    f = np.floor(detpix + 0.5)
+    return np.where((f >= 0) & (f < npix), f, -1).astype(np.int64)
+
+
+# ------------------------------------------------------------------ driver
+def columns_of(el):
+    kind = el['kind']
+    cols = {'loc0': None, 'loc1': None, 'id': el.get('id_col'), 'order': None,
+            'blaze': None, 'detpix0': None, 'detpix1': None, 'pix0': None, 'pix1': None}
+    default_loc = {'aperture_circle': ['y', 'z'], 'aperture_rect': ['y', 'z'],
+                   'lens_perfect': ['mirror_x', 'mirror_y'], 'lens_thin': ['y', 'z'],
+                   'grating': ['grat_y', 'grat_z'], 'detector': ['det_x', 'det_y']}[kind]
+    loc = el.get('loc', default_loc)
+    if loc is not None:
+        cols['loc0'], cols['loc1'] = loc
+    if kind == 'grating':
+        cols['order'] = el.get('order_name', 'order')
+        cols['blaze'] = el.get('blaze_name', 'blaze')
+    if kind == 'detector':
+        cols['detpix0'], cols['detpix1'] = el.get('detpix', ['detpix_x', 'detpix_y'])
+        pix = el.get('pix', ['pix_x', 'pix_y'])
+        if pix is not None:
+            cols['pix0'], cols['pix1'] = pix
+    return cols
+
+
+def uniform_slots(elements, n_draws=2):
+    """Slot base per element: consecutive elements of one kind writing the same
+    columns share a stage; an aperture stage takes 2 draws (phi or y, then r
+    or z), a grating stage ``n_draws`` (k-th interaction of a photon in the
+    stage uses draw k, draws reused cyclically beyond that)."""
+    out = []
+    slot = 0
+    prev = None
+    stage_id = -1
+    for el in elements:
+        key = (el['kind'], tuple(sorted((k, v) for k, v in columns_of(el).items() if v)))
+        new_stage = (prev is None or key != prev or el['kind'] in APERTURES)
+        if new_stage:
+            n = 2 if el['kind'] in APERTURES else (n_draws if el['kind'] == 'grating' else 0)
+            base = slot
+            slot += n
+            stage_id += 1
+        else:
+            base, n = out[-1][0], out[-1][1]
+        out.append((base, n, stage_id))
+        prev = key
+    return out, slot
+
+
+INT_COLS = ('id', 'order', 'pix0', 'pix1')
+
+
+def trace(elements, photons, uniforms, n_draws=2):
+    """Element-major trace (simulator.py:42-49 + optics/base.py:144-211).
+
+    Returns a new dict of columns.  Raises ValueError where the reference
+    raises (probability factor outside [0, 1]; EfficiencyFile without a
+    valid order).
+    """
+    ph = {k: np.array(v, dtype=float) for k, v in photons.items() if k in
+          ('pos', 'dir', 'pol', 'energy', 'prob')}
+    n = len(ph['dir'])
+    if 'pos' not in ph:
+        ph['pos'] = np.full((n, 3), np.nan)
+    slots, _ = uniform_slots(elements, n_draws)
+    cols = {}
+    int_names = {}
+    count = None
+    cur_stage = None
+    for el, (base, nslot, stage_id) in zip(elements, slots):
+        if stage_id != cur_stage:
+            count = np.zeros(n, dtype=np.int64)
+            cur_stage = stage_id
+        fr = frame(el['pos4d'])
+        kind = el['kind']
+        cnames = columns_of(el)
+        if kind in APERTURES:
+            hit = np.ones(n, dtype=bool)
+            x, y, q = aperture_points(el, fr, uniforms[base], uniforms[base + 1])
+            ly, lz = x, y
+            f = dot(ph['dir'], -fr['e_x'])
+            f = np.clip(f, 0, 1.)
+            out = {'probability': f}
+        else:
+            hit, q, ly, lz = intersect(fr, ph['pos'], ph['dir'])
+            if not hit.any():
+                continue
+            q, ly, lz = q[hit], ly[hit], lz[hit]
+            d_old, pol_old = ph['dir'][hit], ph['pol'][hit]
+            out = {}
+            if kind == 'grating':
+                k = count[hit]
+                slot = np.where(k < nslot, k, k % nslot)
+                u = uniforms[base + slot, np.nonzero(hit)[0]]
+                tab = selector_table(el['selector'])
+                order, ptot, ok = select_order(tab, ph['energy'][hit], u)
+                if not ok.all():
+                    raise ValueError('no order selected')
+                new, blaze = diffract(el, fr, d_old, ph['energy'][hit], ly, order, ptot)
+                out['dir'] = new
+                out['probability'] = ptot
+                out['polarization'] = parallel_transport(d_old, new, pol_old)
+                out['order'] = order
+                out['blaze'] = blaze
+            elif kind == 'lens_perfect':
+                new, pol = perfect_lens(el, fr, d_old, q, pol_old)
+                out['dir'] = new
+                out['polarization'] = pol
+            elif kind == 'lens_thin':
+                new, pol = thin_lens(el, fr, d_old, q, pol_old)
+                out['dir'] = new
+                out['polarization'] = pol
+            elif kind == 'detector':
+                npix, cpix = detector_params(el)
+                dp0 = ly / el['pixsize'] + cpix[0]
+                dp1 = lz / el['pixsize'] + cpix[1]
+                out['detpix0'] = dp0
+                out['detpix1'] = dp1
+                out['pix0'] = pixel_index(dp0, npix[0])
+                out['pix1'] = pixel_index(dp1, npix[1])
+            else:
+                raise ValueError(kind)
+            count[hit] += 1
+        # _assign_col_value, optics/base.py:12-51
+        if 'probability' in out:
+            v = out['probability']
+            if np.any(v < 0) or np.any(v > 1):
+                raise ValueError('Found probability outside of the 0..1 arange.')
+            ph['prob'][hit] = ph['prob'][hit] * v
+        if 'dir' in out:
+            ph['dir'][hit] = out['dir']
+        if 'polarization' in out:
+            ph['pol'][hit] = out['polarization']
+        extra = {'loc0': ly, 'loc1': lz, 'id': el.get('id_num', -9)}
+        for key in ('order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1'):
+            if key in out:
+                extra[key] = out[key]
+        for key, val in extra.items():
+            name = cnames.get(key)
+            if name is None:
+                continue
+            if name not in cols:
+                if key in INT_COLS:
+                    cols[name] = np.full(n, ORDER_NONE if key == 'order' else -1, dtype=np.int64)
+                    int_names[name] = key
+                else:
+                    cols[name] = np.full(n, np.nan)
+            cols[name][hit] = val
+        ph['pos'][hit] = q
+    res = dict(ph)
+    res.update(cols)
+    return res

@@ -1,0 +1,97 @@
+"""Shared test helpers: run the GPU trace and compare it with the oracle."""
+import numpy as np
+
+from oracle import trace as OT
+from oracle.describe import describe
+
+ORDER_NONE = -2 ** 31
+
+
+def gpu_trace(inst, ph, uniforms, precision=64, grid=True, n_draws=2, device=None):
+    """Trace host photon arrays (oracle layout) on the GPU; returns a dict in
+    the oracle's layout plus the status counters."""
+    import torch
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    ci = inst if hasattr(inst, 'trace_device') else CompiledInstrument(inst, n_draws=n_draws, grid=grid)
+    dp = DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'],
+                       pos=ph['pos'].T if 'pos' in ph else None)
+    u = torch.as_tensor(np.ascontiguousarray(uniforms), dtype=torch.float64, device=dp.device)
+    st = ci.trace_device(dp, uniforms=u, precision=precision, grid=grid)
+    out = {'pos': dp.pos.cpu().numpy().T, 'dir': dp.dir.cpu().numpy().T,
+           'pol': dp.pol.cpu().numpy().T, 'prob': dp.prob.cpu().numpy(),
+           'energy': dp.energy.cpu().numpy()}
+    for k, v in dp.fcols.items():
+        out[k] = v.cpu().numpy()
+    for k, v in dp.icols.items():
+        out[k] = v.cpu().numpy().astype(np.int64)
+    return out, st
+
+
+def oracle_trace(inst, ph, uniforms, n_draws=2):
+    return OT.trace(describe(inst), ph, uniforms, n_draws=n_draws)
+
+
+def near_pixel_boundary(ref, tol=1e-6):
+    """Photons whose oracle detector pixel coordinate lies within ``tol``
+    pixels of a pixel edge (k + 0.5): the integer pixel index there depends on
+    the last bits of the coordinate."""
+    out = {}
+    for a, b in (('pix_x', 'detpix_x'), ('pix_y', 'detpix_y')):
+        if b in ref:
+            v = np.asarray(ref[b])
+            with np.errstate(invalid='ignore'):
+                f = (v + 0.5) - np.floor(v + 0.5)
+                out[a] = np.nan_to_num(np.minimum(f, 1 - f), nan=1.) < tol
+    return out
+
+
+def compare(ref, got, rtol=1e-9, max_int_mismatch=0, near=None, verbose=True,
+            skip=()):
+    """Return a report dict; raise AssertionError when outside tolerance.
+
+    Floats: |a - b| <= rtol * max(1, |a|) (NaN must match NaN).  Vectors are
+    compared per component against the vector norm.  Integers: exact, with at
+    most ``max_int_mismatch`` differing photons outside the near-boundary masks
+    ``near[col]``.
+    """
+    near = near or {}
+    report = {}
+    names = [k for k in ref if k in got and k not in skip]
+    for k in names:
+        a, b = np.asarray(ref[k]), np.asarray(got[k])
+        if a.dtype.kind in 'iu':
+            diff = a != b
+            nb = int((diff & near[k]).sum()) if k in near else 0
+            report[k] = ('int', int(diff.sum()) - nb, nb)
+            continue
+        if a.ndim == 2:
+            scale = np.maximum(1., np.linalg.norm(a, axis=1))[:, None]
+        else:
+            scale = np.maximum(1., np.abs(a))
+        nan_a, nan_b = np.isnan(a), np.isnan(b)
+        nan_mism = int((nan_a != nan_b).sum())
+        with np.errstate(invalid='ignore'):
+            err = np.where(nan_a | nan_b, 0., np.abs(a - b) / scale)
+        report[k] = ('float', float(err.max()) if err.size else 0., nan_mism)
+    bad = []
+    for k, r in report.items():
+        if r[0] == 'int' and r[1] > max_int_mismatch:
+            bad.append((k, r))
+        if r[0] == 'float' and (r[1] > rtol or r[2] > max_int_mismatch):
+            bad.append((k, r))
+    if verbose:
+        print({k: r for k, r in report.items()})
+    assert not bad, f'parity failures: {bad}'
+    return report
+
+
+def make_inputs(n, seed, n_slots, energies='uniform'):
+    from marxs_amd import configs
+    rng = np.random.default_rng(seed)
+    u = rng.random((2 + n_slots + 1, n))
+    if energies == 'uniform':
+        e = configs.uniform_energies(u[0])
+    else:
+        e = configs.line_spectrum_energies(u[0], u[-1])
+    ph = configs.onaxis_photons(e, u[1])
+    return ph, np.ascontiguousarray(u[2:2 + n_slots])

@@ -1,0 +1,76 @@
+"""The C-ABI library: loads, exports every symbol of include/marxs_hip.h and
+matches the ctypes struct layout (CPU only; no compute calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'marxs_hip.h')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(mx_[a-z0-9_]+)\s*\(', text)))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    from marxs_amd import _lib
+    return _lib
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert 'mx_trace' in names
+    for n in names:
+        assert hasattr(lib.lib, n), n
+    out = subprocess.run(['nm', '-D', '--defined-only', lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r' T (mx_\w+)', out))
+    assert set(names) <= exported
+
+
+def test_abi_version(lib):
+    assert lib.lib.mx_abi_version() == lib.MX_ABI_VERSION
+
+
+def test_struct_layout_matches_header(lib, tmp_path):
+    """sizeof/offsetof from the C compiler against the ctypes mirror."""
+    fields = {'MxTables': [f[0] for f in lib.MxTables._fields_],
+              'MxPhotons': [f[0] for f in lib.MxPhotons._fields_]}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "marxs_hip.h"',
+           'int main(void) {']
+    for st, fs in fields.items():
+        src.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            src.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    src.append('return 0; }')
+    c = tmp_path / 'layout.c'
+    c.write_text('\n'.join(src))
+    exe = tmp_path / 'layout'
+    subprocess.run(['gcc', '-I', os.path.join(ROOT, 'include'), str(c), '-o', str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
+                                                       text=True, check=True).stdout.splitlines())
+    for st, fs in fields.items():
+        cls = getattr(lib, st)
+        assert int(got[st]) == ctypes.sizeof(cls)
+        for f in fs:
+            assert int(got[f'{st}.{f}']) == getattr(cls, f).offset, (st, f)
+
+
+def test_argument_errors_need_no_gpu(lib):
+    """Invalid arguments are rejected on the host before any launch."""
+    t = lib.MxTables()
+    p = lib.MxPhotons()
+    p.n = -1
+    rc = lib.lib.mx_trace(ctypes.byref(t), ctypes.byref(p), 64, 0, None, None)
+    assert rc != 0
+    assert b'negative' in lib.lib.mx_last_error()
+    p.n = 0
+    rc = lib.lib.mx_trace(ctypes.byref(t), ctypes.byref(p), 16, 0, None, None)
+    assert rc != 0 and b'precision' in lib.lib.mx_last_error()

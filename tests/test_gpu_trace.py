@@ -1,0 +1,239 @@
+"""GPU parity of the HIP trace against the oracle (needs an MI355X).
+
+FP64 path: integer outputs (facet, order, CCD, pixel) bit-exact except for
+photons the oracle places within 1e-6 px of a pixel edge (counted and
+reported); every float column within 1e-9 relative.
+FP32 path: detector coordinates within 0.5 um RMS of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from marxs_amd import configs
+import marxs_amd as mx
+from tests.helpers import (gpu_trace, oracle_trace, compare, make_inputs,
+                           near_pixel_boundary)
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+
+
+def _ci(inst, **kw):
+    from marxs_amd.tracer import CompiledInstrument
+    return CompiledInstrument(inst, **kw)
+
+
+def _parity(inst, n, seed, energies='uniform', grid=True, n_draws=2):
+    ci = _ci(inst, grid=grid, n_draws=n_draws)
+    ph, u = make_inputs(n, seed, ci.n_uniform_slots, energies)
+    ref = oracle_trace(inst, ph, u, n_draws=n_draws)
+    got, st = gpu_trace(ci, ph, u, precision=64, grid=grid)
+    rep = compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    return ref, got, st, rep
+
+
+@pytest.mark.parametrize('grid', [True, False])
+def test_flat_grid_fp64_parity(gpu, grid):
+    """121 coplanar FlatGrating facets (shared edges), one 8192^2 detector."""
+    ref, got, st, rep = _parity(configs.flat_grid_instrument(), 20000, 1, grid=grid)
+    assert np.isfinite(got['det_x']).sum() > 10000
+    # zeroth order focuses onto a pixel corner: pixel index is a near-boundary
+    # decision there; everything else is exact
+    assert rep['facet'][1:] == (0, 0) and rep['order'][1:] == (0, 0)
+
+
+@pytest.mark.parametrize('grid', [True, False])
+def test_cat_rowland_200_fp64_parity(gpu, grid):
+    inst, parts = configs.cat_rowland_instrument(200)
+    ref, got, st, rep = _parity(inst, 40000, 2, grid=grid)
+    assert (got['CCD'] >= 0).sum() > 1000
+    assert rep['CCD'][1] == 0
+
+
+def test_cat_rowland_600_fp64_parity_line_spectrum(gpu):
+    inst, parts = configs.cat_rowland_instrument(600)
+    ref, got, st, rep = _parity(inst, 40000, 3, energies='lines')
+    assert (got['facet'] >= 0).sum() > 10000
+
+
+def test_fp32_detector_error(gpu):
+    """FP32 local-frame tracer: <= 0.5 um RMS detector error vs the FP64
+    oracle; facet / CCD edge disagreements counted."""
+    inst, parts = configs.cat_rowland_instrument(600)
+    ci = _ci(inst)
+    ph, u = make_inputs(40000, 4, ci.n_uniform_slots, 'lines')
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(ci, ph, u, precision=32)
+    both = (ref['CCD'] >= 0) & (got['CCD'] == ref['CCD']) & (got['facet'] == ref['facet'])
+    err = np.hypot(got['det_x'] - ref['det_x'], got['det_y'] - ref['det_y'])[both] * 1e3
+    rms = float(np.sqrt(np.mean(err ** 2)))
+    edge = int((got['facet'] != ref['facet']).sum() + (got['CCD'] != ref['CCD']).sum())
+    print(f'FP32: rms {rms:.4f} um max {err.max():.4f} um edge mismatches {edge}')
+    assert both.sum() > 5000
+    assert rms <= 0.5
+    assert edge <= 0.001 * len(err)
+    assert (got['order'] == ref['order'])[got['facet'] == ref['facet']].all()
+
+
+# ---------------------------------------------------------------- elements
+def _single(elements, ph, u, **kw):
+    inst = mx.Sequence(elements=elements)
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(inst, ph, u, **kw)
+    compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    return ref, got, st
+
+
+def _photons(n, seed, spread=0.05, x0=100.):
+    rng = np.random.default_rng(seed)
+    d = np.column_stack([-np.ones(n), rng.normal(0, spread, (n, 2))])
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pol = np.cross(d, rng.normal(size=(n, 3)))
+    pol /= np.linalg.norm(pol, axis=1)[:, None]
+    pos = np.column_stack([np.full(n, x0), rng.uniform(-30, 30, (n, 2))])
+    return {'pos': pos, 'dir': d, 'pol': pol, 'energy': rng.uniform(0.3, 2, n),
+            'prob': np.ones(n)}, rng.random((4, n))
+
+
+def test_apertures(gpu):
+    ph, u = _photons(5000, 1)
+    del ph['pos']
+    for ap in (mx.CircleAperture(position=[50, 1, 2], zoom=[1, 30, 30], r_inner=10,
+                                 phi=[0.3, 2.5]),
+               mx.RectangleAperture(position=[50, 0, 0], zoom=[1, 20, 10],
+                                    orientation=mx.configs.axangle2mat([0, 1, 1], .4))):
+        _single([ap], ph, u[:2])
+
+
+def test_lenses(gpu):
+    ph, u = _photons(5000, 2)
+    _single([mx.PerfectLens(focallength=500, position=[10, 0, 5], zoom=[1, 40, 40],
+                            d_center_optical_axis=3.)], ph, u[:0])
+    _single([mx.ThinLens(focallength=300, position=[10, 1, 0], zoom=[1, 40, 40])], ph, u[:0])
+
+
+@pytest.mark.parametrize('cls', ['flat', 'cat', 'nonparallel', 'reflection'])
+def test_gratings(gpu, cls):
+    ph, u = _photons(8000, 3, spread=0.03)
+    sel = mx.OrderSelector([-3, -1, 0, 1, 2], [.1, .3, .2, .2, .1])
+    kw = dict(d=2e-4, order_selector=sel, zoom=[1, 25, 25], groove_angle=.2,
+              orientation=mx.configs.axangle2mat([0, 0, 1], .03))
+    if cls == 'flat':
+        g = mx.FlatGrating(**kw)
+    elif cls == 'cat':
+        g = mx.CATGrating(**kw)
+    elif cls == 'nonparallel':
+        g = mx.NonParallelCATGrating(d_blaze_mm=1e-3, blaze_center=0.01, **kw)
+    else:
+        g = mx.FlatGrating(transmission=False, **kw)
+    det = mx.FlatDetector(pixsize=0.01, position=[-300, 0, 0], zoom=[1, 200, 200])
+    _single([g, det], ph, u[:2])
+
+
+def test_efficiency_file_and_ccd_array(gpu):
+    ph, u = _photons(8000, 4, spread=0.02)
+    eff = configs.blaze_efficiency()
+    grat = mx.Parallel(elem_class=mx.CATGrating,
+                       elem_args={'d': 2e-4, 'order_selector': eff, 'zoom': [1, 10, 10]},
+                       elem_pos={'position': [[0, y, z] for y in (-20, 0, 20) for z in (-20, 0, 20)]},
+                       id_col='facet')
+    ccd = mx.Parallel(elem_class=mx.FlatDetector,
+                      elem_args={'pixsize': 0.024, 'zoom': [1, 12.288, 12.288]},
+                      elem_pos={'position': [[-400, y, 0] for y in np.arange(-200, 201, 25.)]},
+                      id_col='CCD')
+    ref, got, st = _single([grat, ccd], ph, u[:2])
+    assert (got['CCD'] >= 0).sum() > 100
+
+
+def test_multiple_interactions_and_draw_reuse(gpu):
+    """Two stacked grating planes in one stage: every photon interacts twice,
+    the second interaction uses the second draw; with one draw per stage the
+    first draw is reused (counted)."""
+    ph, u = _photons(4000, 5, spread=0.001)
+    sel = mx.OrderSelector([-1, 0, 1])
+    g = mx.Parallel(elem_class=mx.FlatGrating,
+                    elem_args={'d': 1e-3, 'order_selector': sel, 'zoom': [1, 40, 40]},
+                    elem_pos={'position': [[0, 0, 0], [-5, 0, 0]]}, id_col='facet')
+    inst = mx.Sequence(elements=[g])
+    for nd in (2, 1):
+        ref = oracle_trace(inst, ph, u[:nd], n_draws=nd)
+        got, st = gpu_trace(inst, ph, u[:nd], n_draws=nd)
+        compare(ref, got, rtol=RTOL)
+        assert st['multi_hit'] > 3000
+        assert (st['slot_reuse'] > 0) == (nd == 1)
+
+
+def test_empty_and_degenerate_inputs(gpu):
+    inst = configs.flat_grid_instrument()
+    ph, u = make_inputs(0, 1, 4)
+    got, st = gpu_trace(inst, ph, u)
+    assert got['pos'].shape == (0, 3)
+    # NaN directions and photons that miss everything
+    ph, u = make_inputs(64, 2, 4)
+    ph['dir'][:8] = np.nan
+    ph['dir'][8:16] = [1., 0, 0]
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(inst, ph, u)
+    compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    assert np.all(got['facet'][8:16] == -1)
+
+
+def test_status_errors_raise(gpu):
+    ph, u = _photons(100, 6)
+    bad = mx.EfficiencyFile.from_arrays([1.], [[0.6, 0.6]], [0, 1])
+    g = mx.FlatGrating(d=1e-3, order_selector=bad, zoom=[1, 40, 40])
+    with pytest.raises(ValueError, match='probability outside'):
+        gpu_trace(mx.Sequence(elements=[g]), ph, u[:2])
+
+
+def test_table_api_and_existing_columns(gpu):
+    """Element __call__ on a PhotonTable: reference known answers through the
+    GPU path (marxs/optics/tests/test_detector.py:16-35)."""
+    pos = np.array([[0, 0., -0.25, 1.], [0., 9.9, 1., 1.], [0., 10.1, 1., 1.]])
+    d = np.ones((3, 4))
+    d[:, 3] = 0
+    t = mx.PhotonTable({'pos': pos, 'dir': d, 'energy': [1, 2., 3.],
+                        'polarization': np.tile([0, 1., -1., 0], (3, 1)),
+                        'probability': [1., 1., 1.]})
+    det = mx.FlatDetector(zoom=[1., 10., 5.], pixsize=0.5)
+    t = det(t)
+    nan = np.nan
+    assert np.allclose(t['det_x'], [0, 9.9, nan], equal_nan=True)
+    assert np.allclose(t['det_y'], [-0.25, 1., nan], equal_nan=True)
+    assert np.allclose(t['detpix_x'], [19.5, 39.3, nan], equal_nan=True)
+    assert np.allclose(t['detpix_y'], [9, 11.5, nan], equal_nan=True)
+    # a second detector writing the same columns keeps values where it misses
+    det2 = mx.FlatDetector(zoom=[1., 1., 1.], pixsize=0.5, position=[0, 9.9, 1.])
+    t = det2(t)
+    # photon 0 misses det2 (keeps its det_x from det); photons 1, 2 hit it
+    assert np.allclose(t['det_x'], [0, 0, 0.2])
+    assert np.allclose(t['detpix_x'], [19.5, 1.5, 1.9])
+    # order column: float with NaN for photons that met no grating
+    g = mx.CATGrating(d=1. / 5000, order_selector=mx.OrderSelector([5]), zoom=2)
+    p = mx.generate_test_photons(5)
+    p['pos'][:, 1] = np.arange(5)
+    p = g(p)
+    assert np.all(p['order'][:3] == 5) and np.all(np.isnan(p['order'][3:]))
+    assert np.all(np.isnan(p['grat_y'][3:]))
+
+
+def test_parallel_numbering_hits_element_3(gpu):
+    """marxs/simulator/tests/test_parallel.py:11-23"""
+    pos = [[0, -10.1, -10.1], [0, .1, -10.1], [0, -10.1, .1], [0, .1, .1]]
+    det = mx.Parallel(elem_class=mx.FlatDetector, elem_args={'pixsize': 0.01, 'zoom': 5},
+                      elem_pos={'position': pos}, id_col='CCD_ID')
+    p = det(mx.generate_test_photons(5))
+    assert np.all(p['CCD_ID'] == 3)
+
+
+def test_cat_order_convention_table(gpu):
+    """marxs/optics/tests/test_grating.py:200-223 through the GPU path."""
+    dirs = np.array([[-1, 0., 0., 0], [-1, 0.01, -0.01, 0], [-1, 0.01, 0.01, 0],
+                     [-1, -0.01, 0.01, 0], [-1, -0.01, -0.01, 0]])
+    pol = np.tile([0., 0, 1, 0], (5, 1))
+    mk = lambda: mx.PhotonTable({'pos': np.ones((5, 4)), 'dir': dirs.copy(), 'energy': np.ones(5),
+                                 'polarization': pol, 'probability': np.ones(5)})
+    p5 = mx.CATGrating(d=1. / 5000, order_selector=mx.OrderSelector([5]), zoom=2)(mk())
+    m5 = mx.CATGrating(d=1. / 5000, order_selector=mx.OrderSelector([-5]), zoom=2)(mk())
+    assert np.all(p5['dir'][1:3, 1] > 0) and np.all(p5['dir'][3:, 1] < 0)
+    assert np.all(m5['dir'][1:3, 1] < 0) and np.all(m5['dir'][3:, 1] > 0)

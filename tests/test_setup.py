@@ -1,0 +1,336 @@
+"""Host-side set-up code of marxs_amd (CPU only): poses, containers, Rowland
+placement, selectors, the compile step and its culling grids."""
+import warnings
+
+import numpy as np
+import pytest
+
+import marxs_amd as mx
+from marxs_amd import configs
+from marxs_amd.compiler import compile_tree, build_grid, CompileError
+from marxs_amd.geometry import plane_frame
+from marxs_amd.mathutils import (compose, decompose44, euler2mat_sxyz, axangle2mat,
+                                 translation2aff, mat2aff, zoom2aff)
+from oracle import placement as PL
+from oracle import trace as OT
+
+
+# ------------------------------------------------------------ pose algebra
+def test_compose_decompose_roundtrip():
+    rng = np.random.default_rng(0)
+    for _ in range(10):
+        t = rng.normal(size=3)
+        r = euler2mat_sxyz(*rng.uniform(-3, 3, 3))
+        z = rng.uniform(.1, 5, 3)
+        a = compose(t, r, z)
+        assert np.allclose(a, translation2aff(t) @ mat2aff(r) @ zoom2aff(z))
+        t2, r2, z2, s2 = decompose44(a)
+        assert np.allclose(t2, t) and np.allclose(r2, r) and np.allclose(z2, z)
+        assert np.allclose(s2, 0)
+        t3, r3, z3, s3 = PL.decompose(a)
+        assert np.allclose(r3, r2) and np.allclose(z3, z2)
+
+
+def test_euler_sxyz_definition():
+    """static x, then y, then z (independent Rodrigues construction)."""
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        a = rng.uniform(-3, 3, 3)
+        assert np.allclose(euler2mat_sxyz(*a), PL.euler_sxyz(*a))
+
+
+def test_axangle_vectorised():
+    rng = np.random.default_rng(2)
+    ax = rng.normal(size=(7, 3))
+    ang = rng.uniform(-3, 3, 7)
+    m = axangle2mat(ax, ang)
+    for i in range(7):
+        assert np.allclose(m[i], PL.axis_rotation(ax[i], ang[i]))
+
+
+def test_parse_position_errors():
+    with pytest.raises(ValueError, match='positive-definite'):
+        mx.FlatDetector(zoom=[1, 0, 1])
+    with pytest.raises(ValueError, match='cannot be given at the same time'):
+        mx.FlatDetector(pos4d=np.eye(4), zoom=2)
+
+
+# -------------------------------------------------------------- elements
+def test_detector_npix():
+    """marxs/optics/tests/test_detector.py:16-35,38-41"""
+    det = mx.FlatDetector(zoom=[1., 10., 5.], pixsize=0.5)
+    assert det.npix == [40, 20] and det.centerpix == [19.5, 9.5]
+    assert mx.FlatDetector(pixsize=0.05).npix == [40, 40]
+    with pytest.warns(mx.SimulationSetupWarning, match='not an integer multiple'):
+        mx.FlatDetector(zoom=np.array([1., 2., 3.]), pixsize=0.3)
+
+
+def test_orderselector_validation():
+    """marxs/optics/tests/test_grating.py:247-258"""
+    with pytest.raises(ValueError, match='does not match'):
+        mx.OrderSelector([1, 2, 3, 4], p=[0.1, 0.2, 0.3])
+    with pytest.raises(ValueError, match='must be <= 1'):
+        mx.OrderSelector([1, 2, 3], p=[1, 0.2, 0.3])
+    with pytest.raises(ValueError, match='negative'):
+        mx.OrderSelector([1, 2, 3], p=[-0.1, 0.2, 0.3])
+
+
+def test_aperture_validation():
+    """marxs/optics/tests/test_apertures.py:20-41"""
+    with pytest.raises(ValueError, match='Did you use degrees'):
+        mx.CircleAperture(phi=[0, 360.])
+    with pytest.raises(ValueError, match='must be greater than'):
+        mx.CircleAperture(phi=[0.2, -0.2])
+    with pytest.raises(ValueError, match='less than 2 pi'):
+        mx.CircleAperture(phi=[-.5, 2 * np.pi])
+    with pytest.raises(ValueError, match='must be less than'):
+        mx.CircleAperture(r_inner=0.5, zoom=.2)
+    with pytest.raises(mx.GeometryError, match='does not have the same size'):
+        mx.CircleAperture(zoom=[2, 2, 3])
+    c = mx.CircleAperture(zoom=[1, 2, 2], phi=[1.3 * np.pi, 1.8 * np.pi], r_inner=1)
+    assert np.isclose(c.area, np.pi * 0.75)
+
+
+def test_grating_requires_d():
+    with pytest.raises(ValueError, match='Grating constant'):
+        mx.FlatGrating(order_selector=mx.OrderSelector([1]))
+
+
+def test_groove_frame_orthonormal():
+    """test_grating.py:149-151: groove and perpendicular directions."""
+    g = mx.FlatGrating(d=1. / 500, order_selector=mx.OrderSelector([1]), groove_angle=.3)
+    l, p, n = g.groove_frame()
+    assert np.isclose(np.dot(l, p), 0) and np.isclose(np.dot(l, n), 0)
+
+
+# ------------------------------------------------------------ containers
+def test_parallel_numbering_and_pose():
+    """marxs/simulator/tests/test_parallel.py:11-23,66-71"""
+    pos = [[0, -10.1, -10.1], [0, .1, -10.1], [0, -10.1, .1], [0, .1, .1]]
+    det = mx.Parallel(elem_class=mx.FlatDetector, elem_args={'pixsize': 0.01, 'zoom': 5},
+                      elem_pos={'position': pos}, id_col='CCD_ID', id_num_offset=100)
+    for i, e in enumerate(det.elements):
+        assert e.id_num == 100 + i
+        assert e.id_col == 'CCD_ID'
+        assert np.allclose(e.pos4d, compose(pos[i], np.eye(3), [5, 5, 5]))
+
+
+def test_parallel_pose_chain_matches_oracle():
+    """simulator.py:423-437 against the oracle's independent restatement."""
+    rng = np.random.default_rng(4)
+    gpos = compose(rng.normal(size=3), euler2mat_sxyz(*rng.normal(size=3)), [1, 1, 1])
+    elem_pos = [compose(rng.normal(size=3) * 10, euler2mat_sxyz(*rng.normal(size=3) * .1), [1, 1, 1])
+                for _ in range(5)]
+    par = mx.Parallel(elem_class=mx.FlatDetector, elem_pos=elem_pos,
+                      elem_args={'zoom': [1, 3, 4], 'orientation': euler2mat_sxyz(.1, .2, .3)},
+                      pos4d=gpos)
+    par.uncertainty = compose([.1, .2, .3], euler2mat_sxyz(.01, .02, .03), [1, 1, 1])
+    par.elem_uncertainty = [compose(rng.normal(size=3), euler2mat_sxyz(*rng.normal(size=3) * .01),
+                                    [1, 1, 1]) for _ in range(5)]
+    par.generate_elements()
+    eparse = compose([0, 0, 0], euler2mat_sxyz(.1, .2, .3), [1, 3, 4])
+    for i, e in enumerate(par.elements):
+        exp = PL.parallel_pose(gpos, par.uncertainty, elem_pos[i], par.elem_uncertainty[i], eparse)
+        assert np.allclose(e.pos4d, exp)
+
+
+def test_per_element_list_args():
+    """tests/test_simulator.py:50-71: a list of len(elem_pos) is per element."""
+    g = mx.Parallel(elem_class=mx.FlatGrating, elem_pos={'position': [[0, 0, 0], [0, 5, 0]]},
+                    elem_args={'d': [1e-4, 2e-4], 'order_selector': mx.OrderSelector([1])})
+    assert [e._d for e in g.elements] == [1e-4, 2e-4]
+
+
+# ---------------------------------------------------------------- Rowland
+def test_design_tilted_torus():
+    """marxs/design/tests/test_design.py:43-56"""
+    R, r, pos4d = mx.design_tilted_torus(10, 0., 0.)
+    assert R == 5 and r == 5 and np.all(pos4d == np.eye(4))
+    R, r, pos4d = mx.design_tilted_torus(10, np.deg2rad(3), np.deg2rad(6))
+    assert r == pytest.approx(5.0068617299896054)
+    assert R == pytest.approx(4.9794336175561327)
+    assert pos4d[0, 3] == pytest.approx(0.027390523158633273)
+    assert np.allclose(pos4d[[0, 1], 2], 0)
+
+
+def test_torus_parametric_on_quartic():
+    """test_design.py:68-81"""
+    t = mx.RowlandTorus(R=2., r=1.)
+    ang = np.arange(0, 2 * np.pi, 0.1)
+    phi, theta = np.meshgrid(ang, ang)
+    xyzw = t.parametric(theta.ravel(), phi.ravel())
+    assert np.allclose(t.quartic(xyzw[:, :3]), 0.)
+
+
+def test_solve_quartic_matches_reference_method():
+    """Own polynomial-root solver vs scipy.optimize.root from the reference's
+    starting guess (rowland.py:136-183), and the round trip of
+    test_design.py:84-99."""
+    t = mx.RowlandTorus(5., 4.)
+    xyzw = t.parametric([.34, 1.23], [.4, 4.5])
+    rng = np.random.default_rng(0)
+    for i in range(2):
+        d = rng.random(4) - 0.5
+        d[3] = 0
+        assert np.allclose(xyzw[i], t.solve_quartic(origin=xyzw[i] + 0.1 * d, v=d))
+    t2 = mx.RowlandTorus(5900., 6000., position=[100., 0, 0])
+    for y, z in [(0., 300.), (200., 450.), (-400., 100.)]:
+        o = np.array([12000., y, z, 1.])
+        v = np.array([1., 0, 0, 0])
+        mine = t2.solve_quartic(o, v)[:3]
+        ref = PL.torus_line(t2.R, t2.r, t2.pos4d, o, v)
+        assert np.allclose(mine, ref, atol=1e-6)
+
+
+def test_torus_normal_perpendicular():
+    """test_design.py:153-183"""
+    t = mx.RowlandTorus(R=2., r=1.)
+    ang = np.arange(0, 2 * np.pi, 0.1)
+    phi, theta = np.meshgrid(ang, ang)
+    phi, theta = phi.ravel(), theta.ravel()
+    n = t.normal(t.parametric(theta, phi))[:, :3]
+    dt = t.parametric(theta + 1e-5, phi)[:, :3] - t.parametric(theta - 1e-5, phi)[:, :3]
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.)
+    assert np.allclose(np.einsum('ij,ij->i', n, dt / np.linalg.norm(dt, axis=1)[:, None]), 0)
+
+
+class _Facet(mx.FlatOpticalElement):
+    pass
+
+
+def test_grating_array_structure_counts():
+    """test_design.py:209-238: 177 elements, radii 315..585 step 30,
+    12 on the inner arc."""
+    rowland = mx.RowlandTorus(10000., 10000.)
+    gas = mx.GratingArrayStructure(rowland=rowland, d_element=[30., 30.], radius=[300., 600.],
+                                   phi=[-0.2 * np.pi, 0.2 * np.pi], elem_class=_Facet)
+    assert gas.max_elements_on_arc(300.) == 12
+    ang = gas.distribute_elements_on_arc(315.) % (2. * np.pi)
+    assert np.all((ang < 0.2 * np.pi) | (ang > 1.8 * np.pi))
+    assert gas.max_elements_on_radius(gas.radius) == 10
+    assert np.all(gas.distribute_elements_on_radius() == np.arange(315., 600., 30.))
+    assert len(gas.elem_pos) == 177
+    for i in range(len(gas.elem_pos)):
+        assert np.allclose(gas.elem_pos[i], gas.elements[i].pos4d)
+    # every centre on the torus
+    xyz = np.array([e.pos4d[:3, 3] for e in gas.elements])
+    assert np.allclose(rowland.quartic(xyz) / rowland.R ** 4, 0, atol=1e-12)
+
+
+def test_rectangular_grid_edges_touch():
+    """marxs/simulator/tests/test_parallel.py:66-83"""
+    rowland = mx.RowlandTorus(5900., 6020.)
+    det = mx.RectangularGrid(rowland=rowland, elem_class=mx.FlatDetector,
+                             elem_args={'pixsize': 0.024, 'zoom': [1, 24.576, 12.288]},
+                             d_element=[49.652, 49.652], y_range=[-100, 100], guess_distance=25.)
+    e1, e2 = det.elements[0], det.elements[1]
+    edge1 = e1.geometry['center'] + e1.geometry['v_y']
+    edge2 = e2.geometry['center'] - e2.geometry['v_y']
+    assert np.all(np.abs(edge1 - edge2) < 3.)
+
+
+def test_configs_counts():
+    inst, parts = configs.cat_rowland_instrument(200)
+    assert len(parts['gratings'].elements) == 200
+    assert len(parts['ccds'].elements) == 16
+    assert [e.npix for e in parts['ccds'].elements[:1]] == [[1024, 1024]]
+    assert len(configs.flat_grid_instrument().elements[2].elements) == 121
+
+
+def test_onaxis_polarization():
+    """marxs/source/tests/test_pointing.py:30-42"""
+    ph = configs.onaxis_photons(np.ones(5), np.array([0., 90., 180., 270., 45.]) / 360.)
+    exp = [[0, 0, 1], [0, 1, 0], [0, 0, -1], [0, -1, 0], [0, 1 / np.sqrt(2), 1 / np.sqrt(2)]]
+    assert np.allclose(ph['pol'], exp)
+    assert np.allclose(ph['dir'], [-1, 0, 0])
+
+
+# ----------------------------------------------------------- compile step
+def test_compile_stages_and_columns():
+    t = compile_tree(configs.flat_grid_instrument())
+    assert [s.kind for s in t.stages] == [1, 3, 5, 6]
+    assert [len(s.elements) for s in t.stages] == [1, 1, 121, 1]
+    assert t.n_uniform_slots == 4
+    assert 'facet' in t.icols and 'order' in t.icols and 'grat_y' in t.fcols
+    assert t.elem_t.shape == (124, 16) and t.elem_t.dtype == np.float64
+
+
+def test_compile_rejects_uncompilable():
+    g = mx.FlatGrating(d=lambda c: np.ones(len(c)), order_selector=mx.OrderSelector([1]))
+    with pytest.raises(CompileError):
+        compile_tree(g)
+    g = mx.FlatGrating(d=1e-3, order_selector=lambda e, p, b: (np.ones(len(e)), np.ones(len(e))))
+    with pytest.raises(CompileError):
+        compile_tree(g)
+    with pytest.raises(CompileError):
+        compile_tree(mx.PerfectLens(focallength=10, reflectivity_interpolator=lambda *a, **k: 1))
+
+
+def test_compile_frames_match_oracle():
+    inst, parts = configs.cat_rowland_instrument(50)
+    t = compile_tree(inst)
+    for k, e in enumerate(t.elements):
+        fr = OT.frame(e.pos4d)
+        assert np.array_equal(t.elem_t[k, 0:3], fr['center'])
+        assert np.array_equal(t.elem_t[k, 3:6], fr['e_x'])
+        assert np.array_equal(t.elem_t[k, 12:14], [fr['hy'], fr['hz']])
+
+
+def _candidates(g, p, d):
+    """numpy restatement of the kernel's cell selection (cull_cell)."""
+    r = p - g['origin']
+    pa, da = r @ g['a'], d @ g['a']
+    if abs(da) <= 1e-12:
+        return None
+    k1, k2 = (g['s_lo'] - pa) / da, (g['s_hi'] - pa) / da
+    kmin, kmax = min(k1, k2), max(k1, k2)
+    if kmax < 0:
+        return []
+    kmin = max(kmin, 0.)
+    u = [r @ g['b1'] + k * (d @ g['b1']) for k in (kmin, kmax)]
+    w = [r @ g['b2'] + k * (d @ g['b2']) for k in (kmin, kmax)]
+    if max(u) < g['U0'] or min(u) > g['U1'] or max(w) < g['W0'] or min(w) > g['W1']:
+        return []
+    if max(u) - min(u) > 2 * g['delta'] or max(w) - min(w) > 2 * g['delta']:
+        return None
+    iu = int(np.clip(np.floor((0.5 * sum(u) - g['U0']) * g['inv_cu']), 0, g['G1'] - 1))
+    iw = int(np.clip(np.floor((0.5 * sum(w) - g['W0']) * g['inv_cw']), 0, g['G2'] - 1))
+    c = iu * g['G2'] + iw
+    return list(g['items'][g['cell_start'][c]:g['cell_start'][c + 1]])
+
+
+@pytest.mark.parametrize('which', ['flat', 'cat_facets', 'cat_ccds'])
+def test_grid_covers_every_hit(which):
+    """Every element the exhaustive test hits is a candidate of the cell the
+    kernel picks (or the ray falls back to the exhaustive scan)."""
+    rng = np.random.default_rng(11)
+    if which == 'flat':
+        els = configs.flat_grid_instrument().elements[2].elements
+        p = np.column_stack([np.full(4000, 9000.), rng.uniform(-400, 400, (4000, 2))])
+        tgt = np.column_stack([np.full(4000, 5000.), rng.uniform(-340, 340, (4000, 2))])
+    else:
+        inst, parts = configs.cat_rowland_instrument(200)
+        els = parts['gratings'].elements if which == 'cat_facets' else parts['ccds'].elements
+        cen = np.array([e.pos4d[:3, 3] for e in els])
+        if which == 'cat_facets':
+            # converging beam from the lens plane towards the focus
+            p = np.column_stack([np.full(4000, 12500.), rng.uniform(-600, 600, (4000, 2))])
+            tgt = rng.uniform(-50, 50, (4000, 3))
+        else:
+            # dispersed rays from the grating region onto the CCD row
+            p = np.column_stack([np.full(4000, 12000.), rng.uniform(-600, 600, (4000, 2))])
+            tgt = cen[rng.integers(0, len(cen), 4000)] + rng.uniform(-15, 15, (4000, 3))
+    d = tgt - p
+    frames = [plane_frame(e.geometry) for e in els]
+    g = build_grid(frames)
+    n_fall = 0
+    for j, fr in enumerate(frames):
+        ofr = OT.frame(els[j].pos4d)
+        hit, q, ly, lz = OT.intersect(ofr, p, d)
+        for i in np.nonzero(hit)[0]:
+            c = _candidates(g, p[i], d[i])
+            if c is None:
+                n_fall += 1
+                continue
+            assert j in c, (which, i, j)
+    assert n_fall < 0.05 * len(p)
