@@ -46,6 +46,10 @@ def parse():
     ap.add_argument('--cpu-sample', type=int, default=200_000)
     ap.add_argument('--seed', type=int, default=20261202)
     ap.add_argument('--grid', type=int, default=1)
+    ap.add_argument('--only', choices=['fp32', 'fp64'], default=None,
+                    help='profiling: time one precision only, skip comparisons')
+    ap.add_argument('--copy-calibration-mib', type=int, default=0,
+                    help='profiling: run one device copy of this size first (counter calibration)')
     return ap.parse_args()
 
 
@@ -129,6 +133,19 @@ def main():
         kern = [a.elapsed_time(b) * 1e-3 for a, b in ev]
         return wall, float(np.mean(kern))
 
+    if args.copy_calibration_mib:
+        a = torch.empty(args.copy_calibration_mib * 2 ** 20 // 8, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f'copy calibration: {a.numel() * 8} bytes read, {a.numel() * 8} bytes written')
+        del a, b
+    if args.only:
+        prec = 32 if args.only == 'fp32' else 64
+        wall, k = timed(out32 if prec == 32 else out64, prec)
+        raise_for_status(status.cpu().numpy())
+        log(f'{args.only}: kernel {k * 1e3:.3f} ms, {n / k:.4e} photons/s')
+        return
     wall32, k32 = timed(out32, 32)
     log(f'FP32: {wall32 / args.steps * 1e3:.3f} ms/step, kernel {k32 * 1e3:.3f} ms')
     wall64, k64 = timed(out64, 64)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 4
+#define MX_ABI_VERSION 7
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -45,7 +45,7 @@ extern "C" {
 #define MX_ET 16   /* double per element, intersection data */
 #define MX_EP 16   /* double per element, interaction parameters */
 #define MX_EI 8    /* int32 per element */
-#define MX_SELI 8  /* int32 per order selector */
+#define MX_SELI 12 /* int32 per order selector */
 
 /* stage kinds */
 #define MX_STAGE_APERTURE_CIRCLE 1
@@ -61,6 +61,9 @@ extern "C" {
 
 /* mx_trace flags */
 #define MX_FLAG_NO_GRID 1  /* test every element of every array stage */
+/* bits 8..15: workgroups launched per CU (persistent grid); 0 = as many as
+ * fit at once (occupancy query) */
+#define MX_FLAG_WG_PER_CU(n) (((n) & 0xff) << 8)
 
 /* status counters (unsigned long long, device memory, accumulated) */
 #define MX_ST_PROB_RANGE    0  /* probability factor outside [0, 1]          */
@@ -73,8 +76,11 @@ extern "C" {
 typedef struct MxTables {
     int32_t n_stages;
     int32_t n_elems;
+    int32_t lds_bytes32;       /* LDS staged per workgroup, FP32 path (records of stages with stage_i[20] >= 0) */
+    int32_t lds_bytes64;       /* LDS staged per workgroup, FP64 path (stage_i[21] >= 0) */
     const int32_t *stage_i;    /* [n_stages][MX_SI] */
     const double  *stage_d;    /* [n_stages][MX_SD] */
+    const float   *stage_f;    /* [n_stages][MX_SD]  float copy of the culling grid frame */
     const double  *elem_t;     /* [n_elems][MX_ET]  */
     const double  *elem_p;     /* [n_elems][MX_EP]  */
     const int32_t *elem_i;     /* [n_elems][MX_EI]  */
@@ -85,6 +91,10 @@ typedef struct MxTables {
     const int32_t *sel_o;      /* order numbers */
     const int32_t *cell_start; /* culling grid: CSR offsets */
     const int32_t *cell_items; /* culling grid: element index within stage */
+    const unsigned char *lds_image32;  /* bytes copied to LDS by each workgroup, FP32 */
+    const unsigned char *lds_image64;  /* the same for FP64 */
+    int32_t first_draw_slot;   /* uniform slot of the first grating draw (-1: none), prefetched */
+    int32_t reserved;
 } MxTables;
 
 typedef struct MxPhotons {

@@ -15,7 +15,7 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 4
+MX_ABI_VERSION = 7
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
 
@@ -24,9 +24,12 @@ P = ctypes.c_void_p
 
 class MxTables(ctypes.Structure):
     _fields_ = [('n_stages', ctypes.c_int32), ('n_elems', ctypes.c_int32),
-                ('stage_i', P), ('stage_d', P), ('elem_t', P), ('elem_p', P),
+                ('lds_bytes32', ctypes.c_int32), ('lds_bytes64', ctypes.c_int32),
+                ('stage_i', P), ('stage_d', P), ('stage_f', P), ('elem_t', P), ('elem_p', P),
                 ('elem_i', P), ('elem_t32', P), ('elem_p32', P), ('sel_i', P),
-                ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P)]
+                ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P),
+                ('lds_image32', P), ('lds_image64', P),
+                ('first_draw_slot', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
 class MxPhotons(ctypes.Structure):

@@ -35,7 +35,7 @@ STAGE_LENS_THIN = 4
 STAGE_GRATING = 5
 STAGE_DETECTOR = 6
 
-SI, SD, ET, EP, EI, SELI = 32, 32, 16, 16, 8, 8
+SI, SD, ET, EP, EI, SELI = 32, 32, 16, 16, 8, 12
 ORDER_NONE = -2 ** 31
 
 # output column fields of a stage record (stage_i[10:19]); bit = index
@@ -121,6 +121,7 @@ class CompiledTables:
     stages: list
     stage_i: np.ndarray
     stage_d: np.ndarray
+    stage_f: np.ndarray
     elem_t: np.ndarray
     elem_p: np.ndarray
     elem_i: np.ndarray
@@ -137,6 +138,11 @@ class CompiledTables:
     n_uniform_slots: int
     read_pos: bool
     elements: list = field(default_factory=list)
+    lds_bytes32: int = 0
+    lds_bytes64: int = 0
+    lds_image32: np.ndarray = None
+    lds_image64: np.ndarray = None
+    first_draw_slot: int = -1
 
     @property
     def n_elems(self):
@@ -352,10 +358,14 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
                 rec[10 + b] = fcols.index(name)
         rec[19] = 0
         rec[20:] = 0
+        rec[20:28] = -1
+        rec[28] = -1
     if len(fcols) > 16 or len(icols) > 8:
         raise CompileError('too many distinct output columns')
     tables = CompiledTables(
         stages=stages, stage_i=stage_i, stage_d=stage_d,
+        stage_f=np.concatenate([stage_d[:, 3:21], np.zeros((len(stages), SD - 18))],
+                               axis=1).astype(np.float32),
         elem_t=elem_t, elem_p=elem_p, elem_i=elem_i,
         elem_t32=elem_t32, elem_p32=elem_p32,
         sel_i=(np.array(sel_rows, dtype=np.int32).reshape(-1, SELI)
@@ -370,8 +380,125 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
         n_uniform_slots=uslot,
         read_pos=stages[0].kind not in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT),
         elements=leaves)
+    _build_lds(tables)
+    for st in stages:
+        if st.kind == STAGE_GRATING and st.n_uslot > 0:
+            tables.first_draw_slot = st.uslot
+            break
     tables.stage_i = tables.stage_i_for(set())
     return tables
+
+
+# LDS image budgets per workgroup (160 KiB of LDS per CU; one workgroup of
+# 768 (FP32) or 512 (FP64) work-items per CU).
+LDS_BUDGET32 = 150 * 1024
+LDS_BUDGET64 = 150 * 1024
+
+
+def _align(n, a=16):
+    return (n + a - 1) // a * a
+
+
+def _param_records32(tables, st):
+    """FP32 interaction parameters of a grating stage, 16 words per element:
+    l (3 f32), d = -e_perp (3 f32), period, blaze centre, blaze slope (f32),
+    id, selector, sign mode, transmission, npix x, npix y (i32)."""
+    n = len(st.elements)
+    rec = np.zeros((n, 16), dtype=np.float32)
+    ep = tables.elem_p[st.elem0:st.elem0 + n]
+    ei = tables.elem_i[st.elem0:st.elem0 + n]
+    rec[:, 0:9] = ep[:, 0:9].astype(np.float32)
+    iv = rec.view(np.int32)
+    iv[:, 9] = ei[:, 0]
+    iv[:, 10] = ei[:, 1]
+    iv[:, 11] = ei[:, 2]
+    iv[:, 12] = ei[:, 3]
+    iv[:, 13] = ei[:, 4]
+    iv[:, 14] = ei[:, 5]
+    return rec
+
+
+def _build_lds(tables, budget32=None, budget64=None):
+    """Byte images staged into LDS by every workgroup (one per precision).
+
+    Selector tables first (energies, cumulative probabilities and totals as
+    float64, then int32 orders), then per array stage of two or more
+    elements, while it fits: element test records (16 x f32 / f64), for FP32
+    also the interaction parameter records, then the culling grid as uint16
+    cell offsets and uint16 element indices.  Offsets (bytes, -1 = global
+    memory): stage_i[20]/[21] records FP32/FP64, [22] FP32 parameters,
+    [24]/[25] FP32 cell offsets/items, [26]/[27] FP64 cell offsets/items;
+    sel_i[6]/[7] selector block FP32/FP64.
+    """
+    budget32 = LDS_BUDGET32 if budget32 is None else budget32
+    budget64 = LDS_BUDGET64 if budget64 is None else budget64
+    si = tables.stage_i
+    si[:, 20:28] = -1
+    images = {}
+    for prec, budget in ((32, budget32), (64, budget64)):
+        parts = []
+        off = 0
+
+        def add(arr):
+            nonlocal off
+            b = np.ascontiguousarray(arr).view(np.uint8).ravel()
+            o = off
+            parts.append((o, b))
+            off = _align(off + b.nbytes)
+            return o
+        # selectors
+        sel_ok = True
+        sel_offs = []
+        need = 0
+        for k in range(tables.sel_i.shape[0]):
+            nr, no = int(tables.sel_i[k, 0]), int(tables.sel_i[k, 1])
+            need += _align(8 * (2 * nr + nr * no)) + _align(4 * no)
+        if need > budget:
+            sel_ok = False
+        if sel_ok and tables.sel_i.shape[0] and tables.sel_i[0, 0] > 0:
+            for k in range(tables.sel_i.shape[0]):
+                nr, no, oe, oc, ot, oo = (int(x) for x in tables.sel_i[k, :6])
+                blk = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[oc:oc + nr * no],
+                                      tables.sel_d[ot:ot + nr]])
+                o = add(blk)
+                add(tables.sel_o[oo:oo + no].astype(np.int32))
+                sel_offs.append(o)
+        col = 6 if prec == 32 else 7
+        tables.sel_i[:, col] = -1
+        for k, o in enumerate(sel_offs):
+            tables.sel_i[k, col] = o
+        for s, st in enumerate(tables.stages):
+            n = len(st.elements)
+            if st.kind < STAGE_LENS_PERFECT or n < 2:
+                continue
+            recs = (tables.elem_t32 if prec == 32 else tables.elem_t)[st.elem0:st.elem0 + n]
+            params = _param_records32(tables, st) if (prec == 32 and st.kind == STAGE_GRATING) else None
+            g = st.grid
+            if g is not None and (len(g['items']) >= 65536 or g['cell_start'][-1] >= 65536 or n >= 65536):
+                g = None
+            size = _align(recs.nbytes) + (_align(params.nbytes) if params is not None else 0)
+            if g is not None:
+                size += _align(2 * len(g['cell_start'])) + _align(2 * len(g['items']))
+            if off + size > budget or (st.grid is not None and g is None):
+                continue
+            c_rec = 20 if prec == 32 else 21
+            si[s, c_rec] = add(recs)
+            if params is not None:
+                si[s, 22] = add(params)
+            if g is not None:
+                c0, c1 = (24, 25) if prec == 32 else (26, 27)
+                si[s, c0] = add(g['cell_start'].astype(np.uint16))
+                si[s, c1] = add(g['items'].astype(np.uint16))
+        img = np.zeros(_align(max(off, 16)), dtype=np.uint8)
+        for o, b in parts:
+            img[o:o + b.nbytes] = b
+        images[prec] = img
+        if prec == 32:
+            tables.lds_bytes32 = off
+        else:
+            tables.lds_bytes64 = off
+    tables.lds_image32 = images[32]
+    tables.lds_image64 = images[64]
 
 
 def _selector_row(sel, sel_index, sel_rows, sel_d, sel_o):
@@ -386,7 +513,10 @@ def _selector_row(sel, sel_index, sel_rows, sel_d, sel_o):
         # the reference raises only for photons that use such a row; the
         # kernel counts them and the host raises after the trace
         pass
-    row = [nr, no, len(sel_d), 0, 0, len(sel_o), 0, 0]
+    increasing = bool(nr == 1 or np.all(np.diff(np.asarray(energy, dtype=float)) > 0))
+    monotone = bool(np.all(np.diff(np.asarray(cdf, dtype=float), axis=1) >= 0))
+    row = [nr, no, len(sel_d), 0, 0, len(sel_o), -1, -1, int(increasing) | (int(monotone) << 1),
+           0, 0, 0]
     sel_d.extend(np.asarray(energy, dtype=float).tolist())
     row[3] = len(sel_d)
     sel_d.extend(np.asarray(cdf, dtype=float).ravel().tolist())

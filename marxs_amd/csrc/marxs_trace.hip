@@ -37,7 +37,11 @@
 #include "marxs_hip.h"
 
 #define MX_E2W 1.2398419292004202e-06   /* keV -> mm, marxs/__init__.py:14 */
+#define MX_E2W_F 1.2398419292004202e-06f
 #define MX_ORDER_NONE (-2147483647 - 1)
+#define MX_BLOCK32 768          /* one workgroup per CU holds the LDS image   */
+#define MX_BLOCK64 512
+#define MX_DEFAULT_WG_PER_CU 8  /* persistent grid: workgroups per CU         */
 
 static thread_local char g_last_error[512] = "";
 
@@ -170,90 +174,132 @@ struct Counters {
 
 // Order selection (OrderSelector grating.py:51-57, EfficiencyFile 90-96):
 // row = nearest energy (first on ties); order index = first j with cdf > u.
-__device__ __forceinline__ bool select_order(const MxTables &T, int sel, double energy, double u,
-                                             int &order, double &ptot) {
-    const int *si = T.sel_i + (size_t)sel * MX_SELI;
-    int nr = si[0], no = si[1];
-    const double *eg = T.sel_d + si[2];
-    const double *cdf = T.sel_d + si[3];
-    const double *tot = T.sel_d + si[4];
-    const int *ord = T.sel_o + si[5];
+// Tables: energies eg[nr], cdf[nr*no], tot[nr] (float64), orders ord[no].
+// flags bit0: energies strictly increasing (binary search, equal to the
+// first-on-ties argmin); bit1: every cdf row non-decreasing (binary search
+// for the first entry > u).  Otherwise linear scans.
+template <typename DP, typename IP>
+__device__ __forceinline__ bool select_order_t(DP eg, DP cdf, DP tot, IP ord, int nr, int no, int fl,
+                                               double energy, double u, int &order, double &ptot) {
     int row = 0;
     if (nr > 1) {
-        double best = fabs(eg[0] - energy);
-        for (int r = 1; r < nr; ++r) {
-            double dd = fabs(eg[r] - energy);
-            if (dd < best) { best = dd; row = r; }
+        if (fl & 1) {
+            int lo = 0, hi = nr;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (eg[mid] < energy) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo >= nr) row = nr - 1;
+            else if (lo == 0) row = 0;
+            else row = (fabs(eg[lo - 1] - energy) <= fabs(eg[lo] - energy)) ? lo - 1 : lo;
+        } else {
+            double best = fabs(eg[0] - energy);
+            for (int r = 1; r < nr; ++r) {
+                const double dd = fabs(eg[r] - energy);
+                if (dd < best) { best = dd; row = r; }
+            }
         }
     }
-    const double *c = cdf + (size_t)row * no;
-    int j = 0;
-    while (j < no && !(c[j] > u)) ++j;
+    DP c = cdf + (size_t)row * no;
+    int j;
+    if (fl & 2) {
+        int lo = 0, hi = no;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (c[mid] > u) hi = mid;
+            else lo = mid + 1;
+        }
+        j = lo;
+    } else {
+        j = 0;
+        while (j < no && !(c[j] > u)) ++j;
+    }
     ptot = tot[row];
     if (j == no) { order = MX_ORDER_NONE; return false; }
     order = ord[j];
     return true;
 }
 
-// Candidate range of an array stage for one ray.  Returns 0: no element can
-// be hit, 1: use cell list [b, e), 2: test all elements.
-template <typename R>
-__device__ __forceinline__ int cull_cell(const int *si, const double *sd, R pa, R pb1, R pb2,
-                                         R da, R db1, R db2, int &b, int &e,
-                                         const int *__restrict__ cell_start) {
+template <int PREC>
+__device__ __forceinline__ bool select_order(const MxTables &T, int sel, double energy, double u,
+                                             int &order, double &ptot, const unsigned char *lds) {
+    const int *si = T.sel_i + (size_t)sel * MX_SELI;
+    const int nr = si[0], no = si[1], fl = si[8];
+    const int loff = si[PREC == 64 ? 7 : 6];
+    if (loff >= 0) {
+        const double *b = (const double *)(lds + loff);
+        const int *o = (const int *)(lds + loff + ((8 * (2 * nr + nr * no) + 15) & ~15));
+        return select_order_t(b, b + nr, b + nr + nr * no, o, nr, no, fl, energy, u, order, ptot);
+    }
+    return select_order_t(T.sel_d + si[2], T.sel_d + si[3], T.sel_d + si[4], T.sel_o + si[5],
+                          nr, no, fl, energy, u, order, ptot);
+}
+
+// Candidate range of an array stage for one ray r + k d, k >= kmin, with r
+// relative to the stage origin.  Returns 0: no element can be hit, 1: test
+// the cell list [b, e), 2: test all elements.  g: a, b1, b2, S_lo, S_hi, U0,
+// W0, 1/cu, 1/cw, U1, W1, delta (stage_f / stage_d layout, see compiler).
+// cs: the stage's cell offsets (cell c lists items [cs[c], cs[c+1])).
+template <typename R, typename V, typename CS>
+__device__ __forceinline__ int cull_cell(const int *si, const R *g, V r, V d, R kmin, int &b, int &e,
+                                         CS cs) {
+    R pa = r.x * g[0] + r.y * g[1] + r.z * g[2];
+    R da = d.x * g[0] + d.y * g[1] + d.z * g[2];
+    R pb1 = r.x * g[3] + r.y * g[4] + r.z * g[5];
+    R db1 = d.x * g[3] + d.y * g[4] + d.z * g[5];
+    R pb2 = r.x * g[6] + r.y * g[7] + r.z * g[8];
+    R db2 = d.x * g[6] + d.y * g[7] + d.z * g[8];
     if (!(pa == pa) || !(da == da) || !(pb1 == pb1) || !(pb2 == pb2) || !(db1 == db1) || !(db2 == db2))
         return 0;  // NaN ray: k is NaN for every element, nothing can be hit
-    if (!(fabs((double)da) > 1e-12)) return 2;
-    R k1 = ((R)sd[12] - pa) / da, k2 = ((R)sd[13] - pa) / da;
+    if (!(fabs(da) > (R)1e-12)) return 2;
+    R ida = (R)1 / da;
+    R k1 = (g[9] - pa) * ida, k2 = (g[10] - pa) * ida;
     R kmax = k1 > k2 ? k1 : k2;
-    R kmin = k1 > k2 ? k2 : k1;
-    if (kmax < (R)0) return 0;
-    if (kmin < (R)0) kmin = (R)0;
-    R u1 = pb1 + kmin * db1, u2 = pb1 + kmax * db1;
-    R w1 = pb2 + kmin * db2, w2 = pb2 + kmax * db2;
+    R kmn = k1 > k2 ? k2 : k1;
+    if (kmax < kmin) return 0;
+    if (kmn < kmin) kmn = kmin;
+    R u1 = pb1 + kmn * db1, u2 = pb1 + kmax * db1;
+    R w1 = pb2 + kmn * db2, w2 = pb2 + kmax * db2;
     R umin = u1 < u2 ? u1 : u2, umax = u1 < u2 ? u2 : u1;
     R wmin = w1 < w2 ? w1 : w2, wmax = w1 < w2 ? w2 : w1;
-    R U0 = (R)sd[14], W0 = (R)sd[15], U1 = (R)sd[18], W1 = (R)sd[19];
-    if (umax < U0 || umin > U1 || wmax < W0 || wmin > W1) return 0;
-    R delta2 = (R)(2.0 * sd[20]);
+    if (umax < g[11] || umin > g[15] || wmax < g[12] || wmin > g[16]) return 0;
+    R delta2 = g[17] + g[17];
     if ((umax - umin) > delta2 || (wmax - wmin) > delta2) return 2;
     int G1 = si[6], G2 = si[7];
-    R um = (R)0.5 * (umin + umax), wm = (R)0.5 * (wmin + wmax);
-    int iu = (int)floor((double)((um - U0) * (R)sd[16]));
-    int iw = (int)floor((double)((wm - W0) * (R)sd[17]));
+    int iu = (int)floor(((R)0.5 * (umin + umax) - g[11]) * g[13]);
+    int iw = (int)floor(((R)0.5 * (wmin + wmax) - g[12]) * g[14]);
     iu = iu < 0 ? 0 : (iu >= G1 ? G1 - 1 : iu);
     iw = iw < 0 ? 0 : (iw >= G2 ? G2 - 1 : iw);
-    const int *cs = cell_start + si[8] + iu * G2 + iw;
-    b = cs[0];
-    e = cs[1];
+    const int c = iu * G2 + iw;
+    b = cs[c];
+    e = cs[c + 1];
     return 1;
 }
 
-struct StageOut {
-    int any, id, order, pix0, pix1;
-    double loc0, loc1, blaze, dp0, dp1;
-};
+__device__ __forceinline__ void put_f(const MxPhotons &P, int c, int64_t i, double v) {
+    if (c >= 0) P.fcol[c][i] = v;
+}
+__device__ __forceinline__ void put_i(const MxPhotons &P, int c, int64_t i, int v) {
+    if (c >= 0) P.icol[c][i] = v;
+}
 
-__device__ __forceinline__ void write_stage_out(const MxPhotons &P, const int *si, const StageOut &o, int64_t i) {
-    int always = si[19];
+// Default values (NaN / -1) for the columns this stage creates, for a photon
+// that did not interact with any element of the stage
+// (add_output_cols, marxs/base/base.py:271-286).
+__device__ __forceinline__ void put_defaults(const MxPhotons &P, const int *si, int64_t i) {
+    const int always = si[19];
+    if (!always) return;
     const double qnan = __builtin_nan("");
-#define MX_WF(field, slot, bit)                                                  \
-    { int c = si[slot];                                                          \
-      if (c >= 0 && (o.any || ((always >> (bit)) & 1))) P.fcol[c][i] = o.any ? (field) : qnan; }
-#define MX_WI(field, slot, bit, dflt)                                            \
-    { int c = si[slot];                                                          \
-      if (c >= 0 && (o.any || ((always >> (bit)) & 1))) P.icol[c][i] = o.any ? (field) : (dflt); }
-    MX_WF(o.loc0, 10, 0)
-    MX_WF(o.loc1, 11, 1)
-    MX_WI(o.id, 12, 2, -1)
-    MX_WI(o.order, 13, 3, MX_ORDER_NONE)
-    MX_WF(o.blaze, 14, 4)
-    MX_WF(o.dp0, 15, 5)
-    MX_WF(o.dp1, 16, 6)
-    MX_WI(o.pix0, 17, 7, -1)
-    MX_WI(o.pix1, 18, 8, -1)
-#undef MX_WF
-#undef MX_WI
+    if (always & 1) put_f(P, si[10], i, qnan);
+    if (always & 2) put_f(P, si[11], i, qnan);
+    if (always & 4) put_i(P, si[12], i, -1);
+    if (always & 8) put_i(P, si[13], i, MX_ORDER_NONE);
+    if (always & 16) put_f(P, si[14], i, qnan);
+    if (always & 32) put_f(P, si[15], i, qnan);
+    if (always & 64) put_f(P, si[16], i, qnan);
+    if (always & 128) put_i(P, si[17], i, -1);
+    if (always & 256) put_i(P, si[18], i, -1);
 }
 
 __device__ __forceinline__ int pixel_index(double detpix, int npix) {
@@ -261,52 +307,341 @@ __device__ __forceinline__ int pixel_index(double detpix, int npix) {
     return (f >= 0.0 && f < (double)npix) ? (int)f : -1;
 }
 
+// Photon state carried through the stages.
+struct Ray {
+    d3 p, d, pol;        // FP64 state (d, pol unused by the FP32 path until the end)
+    f3 df, polf;         // FP32 direction / polarization
+    double prob, energy;
+    double u_first;      // prefetched first grating draw
+};
+
 // Uniform slot of the k-th interaction of a photon inside one stage.
-__device__ __forceinline__ double draw(const MxPhotons &P, int us, int nus, int k, int64_t i, Counters &cnt) {
+__device__ __forceinline__ double draw(const MxTables &T, const MxPhotons &P, const Ray &R, int us, int nus,
+                                       int k, int64_t i, Counters &cnt) {
     if (nus <= 0) { cnt.bad++; return 0.0; }
     int slot = k;
     if (k >= nus) { slot = k % nus; cnt.slot_reuse++; }
+    if (us + slot == T.first_draw_slot) return R.u_first;
     return P.uniforms[(int64_t)(us + slot) * P.uniform_stride + i];
 }
 
-// ======================================================= the trace kernel
-template <int PREC>
-__global__ __launch_bounds__(256) void mx_trace_kernel(MxTables T, MxPhotons P, int flags,
-                                                        unsigned long long *status) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    Counters cnt = {0, 0, 0, 0, 0, 0};
-    const bool use_grid = !(flags & MX_FLAG_NO_GRID);
+// ThinLens.process_photon, marxs/optics/mirror.py:130-148: rotation by
+// -|q - c| / f about dir x (q - c), axis-angle matrix in the transforms3d
+// operation order.
+__device__ __forceinline__ void thin_lens64(d3 c, double f, d3 q, d3 &d, d3 &pol) {
+    d3 r = sub3(q, c);
+    double dist = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+    if (dist == 0.0) return;
+    double ang = -dist / f;
+    d3 ax = cross3(d, r);
+    double an = sqrt(ax.x * ax.x + ax.y * ax.y + ax.z * ax.z);
+    double x = ax.x / an, y = ax.y / an, z = ax.z / an;
+    double cc = cos(ang), ss = sin(ang), C = 1 - cc;
+    double xs = x * ss, ys = y * ss, zs = z * ss;
+    double xC = x * C, yC = y * C, zC = z * C;
+    double xyC = x * yC, yzC = y * zC, zxC = z * xC;
+    double m00 = x * xC + cc, m01 = xyC - zs, m02 = zxC + ys;
+    double m10 = xyC + zs, m11 = y * yC + cc, m12 = yzC - xs;
+    double m20 = zxC - ys, m21 = yzC + xs, m22 = z * zC + cc;
+    d3 nd = mk(m00 * d.x + m01 * d.y + m02 * d.z,
+               m10 * d.x + m11 * d.y + m12 * d.z,
+               m20 * d.x + m21 * d.y + m22 * d.z);
+    d3 np_ = mk(m00 * pol.x + m01 * pol.y + m02 * pol.z,
+                m10 * pol.x + m11 * pol.y + m12 * pol.z,
+                m20 * pol.x + m21 * pol.y + m22 * pol.z);
+    d = nd;
+    pol = np_;
+}
 
-    d3 p = mk(0.0, 0.0, 0.0);
+// FlatGrating.diffract_photons (marxs/optics/grating.py:233-266), FP64 in
+// the reference's operation order.  Returns the blaze angle.
+__device__ __forceinline__ double diffract64(const double *et, const double *ep, int sign_mode,
+                                             double trans, double energy, double ly, int m,
+                                             d3 &d, d3 &pol) {
+    d3 pp = unit3(d);
+    d3 l = ld3(ep), dg = ld3(ep + 3), n = ld3(et + 3);
+    double wave = MX_E2W / energy;
+    double pl = dot3(pp, l);
+    d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
+    double cb = fabs(dot3(pperp, n));
+    cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
+    double blaze = acos(cb);
+    blaze = blaze + (ep[7] + ly * ep[8]);
+    double pd0 = dot3(pp, dg);
+    double sgn;
+    if (sign_mode == 0) sgn = -1.0;
+    else sgn = pd0 > 0.0 ? 1.0 : (pd0 < 0.0 ? -1.0 : (pd0 == 0.0 ? 1.0 : pd0));
+    double t = (sgn * (double)m) * wave;
+    double p_d = pd0 + t / ep[6];
+    double p_l = pl;
+    double p_n = sqrt(1. - p_d * p_d - p_l * p_l);
+    double pn0 = dot3(pp, n);
+    double dirn = pn0 > 0.0 ? 1.0 : (pn0 < 0.0 ? -1.0 : (pn0 == 0.0 ? 0.0 : pn0));
+    dirn = dirn * trans;
+    double w = dirn * p_n;
+    d3 nd = mk(p_d * dg.x + p_l * l.x + w * n.x,
+               p_d * dg.y + p_l * l.y + w * n.y,
+               p_d * dg.z + p_l * l.z + w * n.z);
+    pol = ptrans64(d, nd, pol);
+    d = nd;
+    return blaze;
+}
+
+// The same in float.  n: facet normal; pr: parameter record (l, d, period,
+// blaze centre, blaze slope as f32; sign mode and transmission as i32 in
+// words 11, 12).  Returns the blaze angle including the modifier.
+template <typename FP>
+__device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, int m, f3 &d, f3 &pol) {
+#pragma clang fp contract(fast)
+    f3 pp = unitf(d);
+    f3 l = mkf(pr[0], pr[1], pr[2]), dg = mkf(pr[3], pr[4], pr[5]);
+    const int sign_mode = __float_as_int(pr[11]);
+    const float trans = __float_as_int(pr[12]) ? 1.0f : -1.0f;
+    float pl = dotf(pp, l);
+    f3 pperp = unitf(mkf(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
+    float cb = fabsf(dotf(pperp, n));
+    cb = cb > 1.0f ? 1.0f : cb;
+    float blaze = acosf(cb) + (pr[7] + ly * pr[8]);
+    float pd0 = dotf(pp, dg);
+    float sgn = sign_mode == 0 ? -1.0f : (pd0 < 0.0f ? -1.0f : 1.0f);
+    float p_d = pd0 + (sgn * (float)m) * wave / pr[6];
+    float p_n = sqrtf(1.0f - p_d * p_d - pl * pl);
+    float pn0 = dotf(pp, n);
+    float w = (pn0 > 0.0f ? 1.0f : (pn0 < 0.0f ? -1.0f : 0.0f)) * trans * p_n;
+    f3 nd = mkf(p_d * dg.x + pl * l.x + w * n.x,
+                p_d * dg.y + pl * l.y + w * n.y,
+                p_d * dg.z + pl * l.z + w * n.z);
+    pol = ptrans32(d, nd, pol);
+    d = nd;
+    return blaze;
+}
+
+// Next hit of the ray among candidates [tb, te) (cell list when `list`,
+// element indices otherwise) with index > last.  ET points at the stage's
+// element records (LDS or global).  FP64: q, ly, lz of the hit; FP32: kf,
+// ly, lz relative to the advanced float ray.
+template <int PREC, typename ETP, typename ITP>
+__device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, int te,
+                                         int last, d3 p, d3 d, f3 pr, f3 df, float kmin32,
+                                         d3 &q, double &ly, double &lz, float &kf) {
+    for (int t = tb; t < te; ++t) {
+        const int j = list ? (int)items[t] : t;
+        if (j <= last) continue;
+        bool h;
+        if (PREC == 64) {
+            h = plane_test64((const double *)et + (size_t)j * MX_ET, p, d, q, ly, lz);
+        } else {
+            float fy, fz;
+            h = plane_test32((const float *)et + (size_t)j * MX_ET, pr, df, kmin32, kf, fy, fz);
+            ly = fy; lz = fz;
+        }
+        if (h) return j;
+    }
+    return -1;
+}
+
+// One array stage (lens, grating facets, detectors) for one photon.  L: the
+// stage's records, parameters and culling lists are in LDS.
+template <int PREC, bool L>
+__device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &P, const int s,
+                                            const int64_t i, const bool use_grid, Counters &cnt,
+                                            const unsigned char *lds, Ray &R) {
+    const int *si = T.stage_i + (size_t)s * MX_SI;
+    const double *sd = T.stage_d + (size_t)s * MX_SD;
+    const float *sf = T.stage_f + (size_t)s * MX_SD;
+    const int kind = si[0], n_el = si[1], e0 = si[2];
+    const bool grid = use_grid && (si[3] & 1);
+    const d3 O = ld3(sd);
+    // tables of this stage
+    const unsigned char *rec_l = lds + (L ? si[PREC == 64 ? 21 : 20] : 0);
+    const unsigned short *cs_l = (const unsigned short *)(lds + (L ? si[PREC == 64 ? 26 : 24] : 0));
+    const unsigned short *it_l = (const unsigned short *)(lds + (L ? si[PREC == 64 ? 27 : 25] : 0));
+    const int *cs_g = T.cell_start + si[8];
+    int last = -1, nhit = 0;
+    for (;;) {
+        int cb = 0, ce = 0, mode = 2;
+        // FP32: ray relative to the stage origin (one FP64 subtraction),
+        // moved along itself to the stage mid-plane; k >= kmin32 is forward
+        f3 pr = mkf(0, 0, 0);
+        float kadv = 0.0f, kmin32 = 0.0f;
+        if (PREC == 32) {
+#pragma clang fp contract(fast)
+            f3 r0 = tof(sub3(R.p, O));
+            f3 a = ldf3(sf);
+            float da = dotf(R.df, a);
+            if (da != 0.0f && da == da) {
+                kadv = -dotf(r0, a) / da;
+                if (!(fabsf(kadv) < 1e30f)) kadv = 0.0f;
+            }
+            pr = mkf(r0.x + kadv * R.df.x, r0.y + kadv * R.df.y, r0.z + kadv * R.df.z);
+            kmin32 = -kadv;
+            if (grid) {
+                if (L && si[24] >= 0) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l);
+                else mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_g);
+            }
+        } else if (grid) {
+            if (L && si[26] >= 0) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l);
+            else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g);
+        }
+        if (mode == 0) break;
+        if (mode == 2 && grid) cnt.grid_fallback++;
+        // ---- next hit in ascending element order
+        d3 q = mk(0, 0, 0);
+        double ly = 0, lz = 0;
+        float kf = 0.0f;
+        const bool list = mode != 2;
+        const int tb = list ? cb : last + 1;
+        const int te = list ? ce : n_el;
+        const bool lds_list = L && si[PREC == 64 ? 27 : 25] >= 0;
+        int hit;
+        if (L) {
+            if (lds_list)
+                hit = scan_next<PREC>(rec_l, list, it_l, tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+            else
+                hit = scan_next<PREC>(rec_l, list, T.cell_items, tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+        } else if (PREC == 64) {
+            hit = scan_next<PREC>(T.elem_t + (size_t)e0 * MX_ET, list, T.cell_items, tb, te, last,
+                                  R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+        } else {
+            hit = scan_next<PREC>(T.elem_t32 + (size_t)e0 * MX_ET, list, T.cell_items, tb, te, last,
+                                  R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+        }
+        if (hit < 0) break;
+        if (PREC == 32) {
+            const float kk = kadv + kf;
+            q = mk(R.p.x + (double)kk * (double)R.df.x, R.p.y + (double)kk * (double)R.df.y,
+                   R.p.z + (double)kk * (double)R.df.z);
+        }
+
+        // ---- interaction with element `hit`
+        const size_t eg = (size_t)(e0 + hit);
+        const double *et = T.elem_t + eg * MX_ET;
+        const double *ep = T.elem_p + eg * MX_EP;
+        const int *ei = T.elem_i + eg * MX_EI;
+        int id;
+        if (nhit > 0) cnt.multi_hit++;
+
+        if (kind == MX_STAGE_GRATING) {
+            const double u = draw(T, P, R, si[4], si[5], nhit, i, cnt);
+            int m; double ptot;
+            double blaze;
+            if (PREC == 64) {
+                id = ei[0];
+                if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) cnt.order_empty++;
+                blaze = diffract64(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
+            } else {
+                const float wave = MX_E2W_F / (float)R.energy;
+                if (L && si[22] >= 0) {
+                    const float *prm = (const float *)(lds + si[22]) + (size_t)hit * 16;
+                    const float *er = (const float *)rec_l + (size_t)hit * MX_ET;
+                    id = __float_as_int(prm[9]);
+                    if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds))
+                        cnt.order_empty++;
+                    blaze = (double)diffract32(ldf3(er + 3), prm, wave, (float)ly, m, R.df, R.polf);
+                } else {
+                    id = ei[0];
+                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) cnt.order_empty++;
+                    float prm[16];
+                    for (int k = 0; k < 9; ++k) prm[k] = T.elem_p32[eg * MX_EP + k];
+                    prm[11] = __int_as_float(ei[2]);
+                    prm[12] = __int_as_float(ei[3]);
+                    blaze = (double)diffract32(ldf3(T.elem_t32 + eg * MX_ET + 3), prm, wave, (float)ly, m,
+                                               R.df, R.polf);
+                }
+            }
+            if (ptot < 0.0 || ptot > 1.0) cnt.prob_range++;
+            R.prob = R.prob * ptot;
+            put_i(P, si[13], i, m);
+            put_f(P, si[14], i, blaze);
+        } else if (kind == MX_STAGE_DETECTOR) {
+            id = ei[0];
+            if (PREC == 32) {
+                // final detector coordinate in FP64
+                double y64, z64;
+                plane_test64(et, R.p, tod(R.df), q, y64, z64);
+                ly = y64; lz = z64;
+            }
+            const double dp0 = ly / ep[6] + ep[7];
+            const double dp1 = lz / ep[6] + ep[8];
+            put_f(P, si[15], i, dp0);
+            put_f(P, si[16], i, dp1);
+            put_i(P, si[17], i, pixel_index(dp0, ei[4]));
+            put_i(P, si[18], i, pixel_index(dp1, ei[5]));
+        } else if (kind == MX_STAGE_LENS_PERFECT) {
+            // PerfectLens.specific_process_photons, mirror.py:53-82
+            id = ei[0];
+            d3 popt = ld3(ep);
+            double f = ep[6];
+            if (PREC == 64) {
+                d3 dn = unit3(R.d);
+                d3 fp = mk(popt.x + f * dn.x, popt.y + f * dn.y, popt.z + f * dn.z);
+                d3 nd = unit3(sub3(fp, q));
+                R.pol = ptrans64(R.d, nd, R.pol);
+                R.d = nd;
+            } else {
+                f3 dn = unitf(R.df);
+                d3 fp = mk(popt.x + f * (double)dn.x, popt.y + f * (double)dn.y, popt.z + f * (double)dn.z);
+                f3 nd = unitf(tof(sub3(fp, q)));
+                R.polf = ptrans32(R.df, nd, R.polf);
+                R.df = nd;
+            }
+        } else {  // MX_STAGE_LENS_THIN
+            id = ei[0];
+            if (PREC == 64) {
+                thin_lens64(ld3(et), ep[6], q, R.d, R.pol);
+            } else {
+                d3 dd = tod(R.df), pp = tod(R.polf);
+                thin_lens64(ld3(et), ep[6], q, dd, pp);
+                R.df = tof(dd);
+                R.polf = tof(pp);
+            }
+        }
+        // process_photons bookkeeping, optics/base.py:201-209
+        put_f(P, si[10], i, ly);
+        put_f(P, si[11], i, lz);
+        put_i(P, si[12], i, id);
+        R.p = q;
+        last = hit;
+        ++nhit;
+        if (last >= n_el - 1) break;
+    }
+    if (nhit == 0) put_defaults(P, si, i);
+}
+
+// ======================================================= the trace kernel
+// One photon through all stages.
+template <int PREC>
+__device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
+                                          const bool use_grid, Counters &cnt,
+                                          const unsigned char *lds) {
+    Ray R;
+    R.p = mk(0.0, 0.0, 0.0);
     if (P.read_pos) {
         const double *const *pin = P.pos_in[0] ? P.pos_in : (const double *const *)P.pos;
-        p = mk(pin[0][i], pin[1][i], pin[2][i]);
+        R.p = mk(pin[0][i], pin[1][i], pin[2][i]);
     }
     const double *const *din = P.dir_in[0] ? P.dir_in : (const double *const *)P.dir;
     const double *const *qin = P.pol_in[0] ? P.pol_in : (const double *const *)P.pol;
-    d3 d = mk(din[0][i], din[1][i], din[2][i]);
-    d3 pol = mk(qin[0][i], qin[1][i], qin[2][i]);
-    double prob = P.prob_in ? P.prob_in[i] : P.prob[i];
-    double energy = P.energy ? P.energy[i] : 1.0;
-    // FP32 working copies
-    f3 df = tof(d), polf = tof(pol);
+    R.d = mk(din[0][i], din[1][i], din[2][i]);
+    R.pol = mk(qin[0][i], qin[1][i], qin[2][i]);
+    R.df = tof(R.d);
+    R.polf = tof(R.pol);
+    R.prob = P.prob_in ? P.prob_in[i] : P.prob[i];
+    R.energy = P.energy ? P.energy[i] : 1.0;
+    R.u_first = (T.first_draw_slot >= 0 && T.first_draw_slot < P.n_uniform_slots)
+                    ? P.uniforms[(int64_t)T.first_draw_slot * P.uniform_stride + i] : 0.0;
 
     for (int s = 0; s < T.n_stages; ++s) {
         const int *si = T.stage_i + (size_t)s * MX_SI;
-        const double *sd = T.stage_d + (size_t)s * MX_SD;
-        const int kind = si[0], n_el = si[1], e0 = si[2];
-        StageOut o;
-        o.any = 0; o.id = -1; o.order = MX_ORDER_NONE; o.pix0 = -1; o.pix1 = -1;
-        o.loc0 = 0; o.loc1 = 0; o.blaze = 0; o.dp0 = 0; o.dp1 = 0;
+        const int kind = si[0], e0 = si[2];
 
         if (kind == MX_STAGE_APERTURE_CIRCLE || kind == MX_STAGE_APERTURE_RECT) {
             // aperture.py:68-78,92-95,138-146
             const double *et = T.elem_t + (size_t)e0 * MX_ET;
             const double *ep = T.elem_p + (size_t)e0 * MX_EP;
-            double u0 = draw(P, si[4], si[5], 0, i, cnt);
-            double u1 = draw(P, si[4], si[5], 1, i, cnt);
-            double x, y;
+            double u0 = draw(T, P, R, si[4], si[5], 0, i, cnt);
+            double u1 = draw(T, P, R, si[4], si[5], 1, i, cnt);
+            double x, y, f;
             if (kind == MX_STAGE_APERTURE_CIRCLE) {
                 if (PREC == 64) {
                     double phi = ep[6] + ep[7] * u0;
@@ -325,253 +660,49 @@ __global__ __launch_bounds__(256) void mx_trace_kernel(MxTables T, MxPhotons P, 
                 x = u0 * 2. - 1.;
                 y = u1 * 2. - 1.;
             }
-            d3 c = ld3(et), vy = ld3(ep), vz = ld3(ep + 3), nex = ld3(ep + 10);
-            p = mk(c.x + x * vy.x + y * vz.x, c.y + x * vy.y + y * vz.y, c.z + x * vy.z + y * vz.z);
-            double f;
-            if (PREC == 64) f = dot3(d, nex);
-            else f = (double)dotf(df, tof(nex));
+            d3 c = ld3(et), vy = ld3(ep), vz = ld3(ep + 3);
+            R.p = mk(c.x + x * vy.x + y * vz.x, c.y + x * vy.y + y * vz.y, c.z + x * vy.z + y * vz.z);
+            if (PREC == 64) f = dot3(R.d, ld3(ep + 10));
+            else f = (double)dotf(R.df, ldf3(T.elem_p32 + (size_t)e0 * MX_EP + 10));
             f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
-            if (f < 0.0 || f > 1.0) cnt.prob_range++;
-            prob = prob * f;
-            o.any = 1; o.loc0 = x; o.loc1 = y;
-            o.id = T.elem_i[(size_t)e0 * MX_EI];
-            write_stage_out(P, si, o, i);
+            R.prob = R.prob * f;
+            put_f(P, si[10], i, x);
+            put_f(P, si[11], i, y);
+            put_i(P, si[12], i, T.elem_i[(size_t)e0 * MX_EI]);
             continue;
         }
-
         if (kind < MX_STAGE_LENS_PERFECT || kind > MX_STAGE_DETECTOR) { cnt.bad++; continue; }
-
-        // ------------------------------------------------ array stages
-        const bool grid = use_grid && (si[3] & 1);
-        int last = -1, nhit = 0;
-        // FP32: ray relative to the stage origin, advanced to the stage
-        // mid-plane so that float coordinates stay small.
-        d3 O = ld3(sd);
-        for (;;) {
-            // ---- candidates for the current ray
-            int cb = 0, ce = 0, mode = 2;
-            d3 pbase = p;           // FP32: point the float ray starts from
-            float kmin32 = 0.0f;
-            f3 pr = mkf(0, 0, 0);
-            if (PREC == 32) {
-                d3 a = ld3(sd + 3);
-                double da = dot3(d, a);
-                double kadv = 0.0;
-                if (da != 0.0 && da == da) {
-                    kadv = dot3(sub3(O, p), a) / da;
-                    if (!(kadv == kadv) || fabs(kadv) > 1e300) kadv = 0.0;
-                }
-                // only move forward: k >= 0 must stay representable
-                pbase = mk(p.x + kadv * d.x, p.y + kadv * d.y, p.z + kadv * d.z);
-                pr = tof(sub3(pbase, O));
-                kmin32 = (float)(-kadv);
-            }
-            if (grid) {
-                if (PREC == 64) {
-                    d3 r = sub3(p, O);
-                    d3 a = ld3(sd + 3), b1 = ld3(sd + 6), b2 = ld3(sd + 9);
-                    double pa = dot3(r, a), da = dot3(d, a);
-                    double pb1 = dot3(r, b1), pb2 = dot3(r, b2);
-                    mode = cull_cell<double>(si, sd, pa, pb1, pb2, da, dot3(d, b1), dot3(d, b2),
-                                             cb, ce, T.cell_start);
-                } else {
-                    d3 r = sub3(p, O);
-                    d3 a = ld3(sd + 3), b1 = ld3(sd + 6), b2 = ld3(sd + 9);
-                    mode = cull_cell<double>(si, sd, dot3(r, a), dot3(r, b1), dot3(r, b2),
-                                             dot3(d, a), dot3(d, b1), dot3(d, b2), cb, ce, T.cell_start);
-                }
-                if (mode == 0) break;
-                if (mode == 2) cnt.grid_fallback++;
-            }
-            // ---- scan candidates in ascending order for the next hit
-            int hit = -1;
-            d3 q = mk(0, 0, 0);
-            double ly = 0, lz = 0;
-            if (mode == 2) {
-                for (int j = last + 1; j < n_el; ++j) {
-                    bool h;
-                    if (PREC == 64) {
-                        h = plane_test64(T.elem_t + (size_t)(e0 + j) * MX_ET, p, d, q, ly, lz);
-                    } else {
-                        float k, fy, fz;
-                        h = plane_test32(T.elem_t32 + (size_t)(e0 + j) * MX_ET, pr, df, kmin32, k, fy, fz);
-                        if (h) {
-                            q = mk(pbase.x + (double)k * d.x, pbase.y + (double)k * d.y, pbase.z + (double)k * d.z);
-                            ly = fy; lz = fz;
-                        }
-                    }
-                    if (h) { hit = j; break; }
-                }
-            } else {
-                // cell_start holds offsets into the concatenated item array
-                for (int t = cb; t < ce; ++t) {
-                    int j = T.cell_items[t];
-                    if (j <= last) continue;
-                    bool h;
-                    if (PREC == 64) {
-                        h = plane_test64(T.elem_t + (size_t)(e0 + j) * MX_ET, p, d, q, ly, lz);
-                    } else {
-                        float k, fy, fz;
-                        h = plane_test32(T.elem_t32 + (size_t)(e0 + j) * MX_ET, pr, df, kmin32, k, fy, fz);
-                        if (h) {
-                            q = mk(pbase.x + (double)k * d.x, pbase.y + (double)k * d.y, pbase.z + (double)k * d.z);
-                            ly = fy; lz = fz;
-                        }
-                    }
-                    if (h) { hit = j; break; }
-                }
-            }
-            if (hit < 0) break;
-
-            // ---- interaction with element `hit`
-            const size_t eg = (size_t)(e0 + hit);
-            const double *et = T.elem_t + eg * MX_ET;
-            const double *ep = T.elem_p + eg * MX_EP;
-            const int *ei = T.elem_i + eg * MX_EI;
-            if (nhit > 0) cnt.multi_hit++;
-
-            if (kind == MX_STAGE_GRATING) {
-                // FlatGrating.diffract_photons, grating.py:233-266
-                double u = draw(P, si[4], si[5], nhit, i, cnt);
-                int m; double ptot;
-                if (!select_order(T, ei[1], energy, u, m, ptot)) cnt.order_empty++;
-                if (ptot < 0.0 || ptot > 1.0) cnt.prob_range++;
-                const int sign_mode = ei[2];
-                const double trans = ei[3] ? 1.0 : -1.0;
-                double blaze;
-                if (PREC == 64) {
-                    d3 pp = unit3(d);
-                    d3 l = ld3(ep), dg = ld3(ep + 3), n = ld3(et + 3);
-                    double wave = MX_E2W / energy;
-                    double pl = dot3(pp, l);
-                    d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
-                    double cb_ = fabs(dot3(pperp, n));
-                    cb_ = cb_ < 0.0 ? 0.0 : (cb_ > 1.0 ? 1.0 : cb_);
-                    blaze = acos(cb_);
-                    blaze = blaze + (ep[7] + ly * ep[8]);
-                    double pd0 = dot3(pp, dg);
-                    double sgn;
-                    if (sign_mode == 0) sgn = -1.0;
-                    else sgn = pd0 > 0.0 ? 1.0 : (pd0 < 0.0 ? -1.0 : (pd0 == 0.0 ? 1.0 : pd0));
-                    double t = (sgn * (double)m) * wave;
-                    double p_d = pd0 + t / ep[6];
-                    double p_l = pl;
-                    double p_n = sqrt(1. - p_d * p_d - p_l * p_l);
-                    double pn0 = dot3(pp, n);
-                    double dirn = pn0 > 0.0 ? 1.0 : (pn0 < 0.0 ? -1.0 : (pn0 == 0.0 ? 0.0 : pn0));
-                    dirn = dirn * trans;
-                    double w = dirn * p_n;
-                    d3 nd = mk(p_d * dg.x + p_l * l.x + w * n.x,
-                               p_d * dg.y + p_l * l.y + w * n.y,
-                               p_d * dg.z + p_l * l.z + w * n.z);
-                    pol = ptrans64(d, nd, pol);
-                    d = nd;
-                } else {
-#pragma clang fp contract(fast)
-                    const float *ep32 = T.elem_p32 + eg * MX_EP;
-                    const float *et32 = T.elem_t32 + eg * MX_ET;
-                    f3 pp = unitf(df);
-                    f3 l = ldf3(ep32), dg = ldf3(ep32 + 3), n = ldf3(et32 + 3);
-                    float wave = (float)(MX_E2W / energy);
-                    float pl = dotf(pp, l);
-                    f3 pperp = unitf(mkf(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
-                    float cbf = fabsf(dotf(pperp, n));
-                    cbf = cbf > 1.0f ? 1.0f : cbf;
-                    blaze = (double)acosf(cbf) + ((double)ep[7] + ly * ep[8]);
-                    float pd0 = dotf(pp, dg);
-                    float sgn = sign_mode == 0 ? -1.0f : (pd0 < 0.0f ? -1.0f : 1.0f);
-                    float p_d = pd0 + (sgn * (float)m) * wave / ep32[6];
-                    float p_n = sqrtf(1.0f - p_d * p_d - pl * pl);
-                    float pn0 = dotf(pp, n);
-                    float w = (pn0 > 0.0f ? 1.0f : (pn0 < 0.0f ? -1.0f : 0.0f)) * (float)trans * p_n;
-                    f3 nd = mkf(p_d * dg.x + pl * l.x + w * n.x,
-                                p_d * dg.y + pl * l.y + w * n.y,
-                                p_d * dg.z + pl * l.z + w * n.z);
-                    polf = ptrans32(df, nd, polf);
-                    df = nd;
-                    d = tod(df);
-                }
-                prob = prob * ptot;
-                o.order = m;
-                o.blaze = blaze;
-            } else if (kind == MX_STAGE_DETECTOR) {
-                // FlatDetector.specific_process_photons, detector.py:72-75
-                if (PREC == 32) {
-                    // final detector coordinate in FP64
-                    d3 q64; double y64, z64;
-                    plane_test64(et, p, d, q64, y64, z64);
-                    q = q64; ly = y64; lz = z64;
-                }
-                o.dp0 = ly / ep[6] + ep[7];
-                o.dp1 = lz / ep[6] + ep[8];
-                o.pix0 = pixel_index(o.dp0, ei[4]);
-                o.pix1 = pixel_index(o.dp1, ei[5]);
-            } else if (kind == MX_STAGE_LENS_PERFECT) {
-                // PerfectLens.specific_process_photons, mirror.py:53-82
-                if (PREC == 64) {
-                    d3 popt = ld3(ep);
-                    double f = ep[6];
-                    d3 dn = unit3(d);
-                    d3 fp = mk(popt.x + f * dn.x, popt.y + f * dn.y, popt.z + f * dn.z);
-                    d3 nd = unit3(sub3(fp, q));
-                    pol = ptrans64(d, nd, pol);
-                    d = nd;
-                } else {
-                    d3 popt = ld3(ep);
-                    double f = ep[6];
-                    f3 dn = unitf(df);
-                    d3 fp = mk(popt.x + f * (double)dn.x, popt.y + f * (double)dn.y, popt.z + f * (double)dn.z);
-                    f3 nd = unitf(tof(sub3(fp, q)));
-                    polf = ptrans32(df, nd, polf);
-                    df = nd;
-                    d = tod(df);
-                }
-            } else if (kind == MX_STAGE_LENS_THIN) {
-                // ThinLens.process_photon, mirror.py:130-148 (axis-angle
-                // rotation in the transforms3d operation order)
-                d3 c = ld3(et);
-                d3 r = sub3(q, c);
-                double dist = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
-                if (dist != 0.0) {
-                    double ang = -dist / ep[6];
-                    d3 ax = cross3(d, r);
-                    double an = sqrt(ax.x * ax.x + ax.y * ax.y + ax.z * ax.z);
-                    double x = ax.x / an, y = ax.y / an, z = ax.z / an;
-                    double cc = cos(ang), ss = sin(ang), C = 1 - cc;
-                    double xs = x * ss, ys = y * ss, zs = z * ss;
-                    double xC = x * C, yC = y * C, zC = z * C;
-                    double xyC = x * yC, yzC = y * zC, zxC = z * xC;
-                    double m00 = x * xC + cc, m01 = xyC - zs, m02 = zxC + ys;
-                    double m10 = xyC + zs, m11 = y * yC + cc, m12 = yzC - xs;
-                    double m20 = zxC - ys, m21 = yzC + xs, m22 = z * zC + cc;
-                    if (PREC == 32) { d = tod(df); pol = tod(polf); }
-                    d3 nd = mk(m00 * d.x + m01 * d.y + m02 * d.z,
-                               m10 * d.x + m11 * d.y + m12 * d.z,
-                               m20 * d.x + m21 * d.y + m22 * d.z);
-                    d3 np_ = mk(m00 * pol.x + m01 * pol.y + m02 * pol.z,
-                                m10 * pol.x + m11 * pol.y + m12 * pol.z,
-                                m20 * pol.x + m21 * pol.y + m22 * pol.z);
-                    d = nd; pol = np_;
-                    df = tof(d); polf = tof(pol);
-                }
-            }
-            // process_photons bookkeeping, optics/base.py:201-209
-            o.any = 1;
-            o.loc0 = ly; o.loc1 = lz;
-            o.id = ei[0];
-            p = q;
-            last = hit;
-            ++nhit;
-            if (last >= n_el - 1) break;
-        }
-        write_stage_out(P, si, o, i);
+        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true>(T, P, s, i, use_grid, cnt, lds, R);
+        else array_stage<PREC, false>(T, P, s, i, use_grid, cnt, lds, R);
     }
 
-    if (PREC == 32) { d = tod(df); pol = tod(polf); }
-    P.pos[0][i] = p.x; P.pos[1][i] = p.y; P.pos[2][i] = p.z;
-    P.dir[0][i] = d.x; P.dir[1][i] = d.y; P.dir[2][i] = d.z;
-    P.pol[0][i] = pol.x; P.pol[1][i] = pol.y; P.pol[2][i] = pol.z;
-    P.prob[i] = prob;
+    if (PREC == 32) { R.d = tod(R.df); R.pol = tod(R.polf); }
+    P.pos[0][i] = R.p.x; P.pos[1][i] = R.p.y; P.pos[2][i] = R.p.z;
+    P.dir[0][i] = R.d.x; P.dir[1][i] = R.d.y; P.dir[2][i] = R.d.z;
+    P.pol[0][i] = R.pol.x; P.pol[1][i] = R.pol.y; P.pol[2][i] = R.pol.z;
+    P.prob[i] = R.prob;
+}
+
+// Persistent kernel: each workgroup copies the LDS image (element records,
+// parameters, culling lists and selector tables of the stages marked in
+// stage_i/sel_i) once, then walks photons with a grid stride.
+template <int PREC, int BS>
+__global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, int flags,
+                                                       unsigned long long *status) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lds_bytes = PREC == 64 ? T.lds_bytes64 : T.lds_bytes32;
+    if (lds_bytes > 0) {
+        const uint4 *src = (const uint4 *)(PREC == 64 ? T.lds_image64 : T.lds_image32);
+        uint4 *dst = (uint4 *)lds;
+        const int nvec = (lds_bytes + 15) >> 4;
+        for (int v = threadIdx.x; v < nvec; v += BS) dst[v] = src[v];
+        __syncthreads();
+    }
+    Counters cnt = {0, 0, 0, 0, 0, 0};
+    const bool use_grid = !(flags & MX_FLAG_NO_GRID);
+    const int64_t stride = (int64_t)gridDim.x * BS;
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
+        trace_one<PREC>(T, P, i, use_grid, cnt, lds);
 
     if (status) {
         if (cnt.prob_range) atomicAdd(status + MX_ST_PROB_RANGE, (unsigned long long)cnt.prob_range);
@@ -587,7 +718,7 @@ __global__ __launch_bounds__(256) void mx_trace_kernel(MxTables T, MxPhotons P, 
 static int check_tables(const MxTables *t) {
     if (!t) { set_error("tables is NULL"); return 1; }
     if (t->n_stages < 0 || t->n_stages > 4096) { set_error("n_stages out of range"); return 1; }
-    if (t->n_stages > 0 && (!t->stage_i || !t->stage_d || !t->elem_t || !t->elem_p || !t->elem_i ||
+    if (t->n_stages > 0 && (!t->stage_i || !t->stage_d || !t->stage_f || !t->elem_t || !t->elem_p || !t->elem_i ||
                             !t->elem_t32 || !t->elem_p32 || !t->sel_i || !t->sel_d || !t->sel_o ||
                             !t->cell_start || !t->cell_items)) {
         set_error("a table pointer is NULL");
@@ -618,15 +749,28 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
         return 1;
     }
     if (photons->n == 0 || tables->n_stages == 0) return 0;
-    const int block = 256;
+    const int block = precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32;
     const int64_t nblk = (photons->n + block - 1) / block;
-    if (nblk > 0x7fffffffLL) { set_error("too many photons for one launch"); return 1; }
     hipStream_t st = (hipStream_t)stream;
+    const int lds = precision == MX_PREC_FP64 ? tables->lds_bytes64 : tables->lds_bytes32;
+    if (lds < 0 || lds > 160 * 1024) { set_error("LDS request out of range"); return 1; }
+    if (lds > 0 && !(precision == MX_PREC_FP64 ? tables->lds_image64 : tables->lds_image32)) {
+        set_error("LDS image pointer is NULL");
+        return 1;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if (cus <= 0) cus = 256;
+    int per_cu = (flags >> 8) & 0xff;   // workgroups per CU; 0 = default
+    if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
+    int64_t grid = (int64_t)cus * per_cu;
+    if (grid > nblk) grid = nblk;
     if (precision == MX_PREC_FP64)
-        hipLaunchKernelGGL(mx_trace_kernel<64>, dim3((unsigned)nblk), dim3(block), 0, st,
+        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else
-        hipLaunchKernelGGL(mx_trace_kernel<32>, dim3((unsigned)nblk), dim3(block), 0, st,
+        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {

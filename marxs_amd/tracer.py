@@ -51,8 +51,9 @@ class DeviceTables:
         self.device = _dev(device)
         t = tables
         self._keep = {}
-        for name in ('stage_i', 'stage_d', 'elem_t', 'elem_p', 'elem_i', 'elem_t32',
-                     'elem_p32', 'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items'):
+        for name in ('stage_i', 'stage_d', 'stage_f', 'elem_t', 'elem_p', 'elem_i', 'elem_t32',
+                     'elem_p32', 'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items',
+                     'lds_image32', 'lds_image64'):
             arr = np.ascontiguousarray(getattr(t, name))
             self._keep[name] = torch.from_numpy(arr).to(self.device)
         self._stage_i_cache = {frozenset(): self._keep['stage_i']}
@@ -63,10 +64,14 @@ class DeviceTables:
         s = _lib.MxTables()
         s.n_stages = self.host.stage_i.shape[0]
         s.n_elems = self.host.elem_t.shape[0]
+        s.lds_bytes32 = self.host.lds_bytes32
+        s.lds_bytes64 = self.host.lds_bytes64
         s.stage_i = stage_i.data_ptr()
-        for name in ('stage_d', 'elem_t', 'elem_p', 'elem_i', 'elem_t32', 'elem_p32',
-                     'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items'):
+        for name in ('stage_d', 'stage_f', 'elem_t', 'elem_p', 'elem_i', 'elem_t32', 'elem_p32',
+                     'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items', 'lds_image32',
+                     'lds_image64'):
             setattr(s, name, k[name].data_ptr())
+        s.first_draw_slot = self.host.first_draw_slot
         return s
 
     def struct_for(self, existing):
@@ -160,6 +165,9 @@ class CompiledInstrument:
         self.device = _dev(device)
         self.dev_tables = DeviceTables(self.tables, self.device)
         self.grid = grid
+        # persistent-grid size per precision (0 = occupancy query); set by
+        # marxs_amd.autotune
+        self.wg_per_cu = {}
 
     @property
     def n_uniform_slots(self):
@@ -183,7 +191,7 @@ class CompiledInstrument:
                           generator=generator or mxrandom.generator(self.device))
 
     def launch(self, dp, uniforms=None, precision=64, status=None, stream=None,
-               existing=(), grid=None, src=None):
+               existing=(), grid=None, src=None, wg_per_cu=None):
         """Launch the trace kernel on device photons (asynchronous).
 
         ``src``: optional DevicePhotons to read pos/dir/pol/prob from (left
@@ -239,6 +247,8 @@ class CompiledInstrument:
             stream = torch.cuda.current_stream(self.device).cuda_stream
         use_grid = self.grid if grid is None else grid
         flags = 0 if use_grid else 1
+        wpc = self.wg_per_cu.get(prec, 0) if wg_per_cu is None else wg_per_cu
+        flags |= (int(wpc) & 0xff) << 8
         _lib.check(_lib.lib.mx_trace(ctypes.byref(tables), ctypes.byref(P), prec, flags,
                                      ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
         dp.has_pos = True
