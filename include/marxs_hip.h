@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 7
+#define MX_ABI_VERSION 8
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -91,6 +91,8 @@ typedef struct MxTables {
     const int32_t *sel_o;      /* order numbers */
     const int32_t *cell_start; /* culling grid: CSR offsets */
     const int32_t *cell_items; /* culling grid: element index within stage */
+    const int32_t *succ_start; /* per element: offsets of its successor list */
+    const int32_t *succ_items; /* successors: later elements a ray leaving it can reach */
     const unsigned char *lds_image32;  /* bytes copied to LDS by each workgroup, FP32 */
     const unsigned char *lds_image64;  /* the same for FP64 */
     int32_t first_draw_slot;   /* uniform slot of the first grating draw (-1: none), prefetched */

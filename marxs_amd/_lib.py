@@ -15,9 +15,10 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 7
+MX_ABI_VERSION = 8
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
+LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
 
 P = ctypes.c_void_p
 
@@ -28,7 +29,7 @@ class MxTables(ctypes.Structure):
                 ('stage_i', P), ('stage_d', P), ('stage_f', P), ('elem_t', P), ('elem_p', P),
                 ('elem_i', P), ('elem_t32', P), ('elem_p32', P), ('sel_i', P),
                 ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P),
-                ('lds_image32', P), ('lds_image64', P),
+                ('succ_start', P), ('succ_items', P), ('lds_image32', P), ('lds_image64', P),
                 ('first_draw_slot', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 

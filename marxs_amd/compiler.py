@@ -132,6 +132,8 @@ class CompiledTables:
     sel_o: np.ndarray
     cell_start: np.ndarray
     cell_items: np.ndarray
+    succ_start: np.ndarray
+    succ_items: np.ndarray
     fcols: list            # float output column names, by slot
     icols: list            # int output column names, by slot
     icol_kind: dict        # name -> 'id' | 'order' | 'pix'
@@ -248,11 +250,29 @@ def build_grid(frames, eps_abs=1e-3, eps_rel=1e-6, theta_max=0.35, cell_frac=0.5
             idx = np.nonzero(ok)[0]
             items.extend(idx.tolist())
             starts.append(len(items))
+    # successor lists: after a hit on element j, a ray within theta_max of
+    # the axis can only reach elements k > j whose box overlaps j's box grown
+    # by (largest depth difference of their corners) * tan(theta_max)
+    tt = math.tan(theta_max)
+    slo, shi = s.min(axis=1), s.max(axis=1)
+    sstart = [0]
+    sitems = []
+    for j in range(n):
+        ks = np.arange(j + 1, n)
+        if len(ks):
+            depth = np.maximum(np.abs(shi[j] - slo[ks]), np.abs(shi[ks] - slo[j])) + 2 * eps
+            grow = depth * tt + 2 * eps
+            ok = ((umax[ks] >= umin[j] - grow) & (umin[ks] <= umax[j] + grow) &
+                  (wmax[ks] >= wmin[j] - grow) & (wmin[ks] <= wmax[j] + grow))
+            sitems.extend(ks[ok].tolist())
+        sstart.append(len(sitems))
     return {'origin': origin, 'a': a, 'b1': b1, 'b2': b2,
             's_lo': s_lo, 's_hi': s_hi, 'U0': U0, 'W0': W0, 'U1': U1, 'W1': W1,
             'inv_cu': 1.0 / cu, 'inv_cw': 1.0 / cw, 'delta': delta,
             'G1': G1, 'G2': G2, 'cell_start': np.array(starts, dtype=np.int32),
-            'items': np.array(items, dtype=np.int32), 'eps': eps}
+            'items': np.array(items, dtype=np.int32), 'eps': eps, 'tan_theta': tt,
+            'succ_start': np.array(sstart, dtype=np.int32),
+            'succ_items': np.array(sitems, dtype=np.int32)}
 
 
 def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
@@ -271,6 +291,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
     stage_d = np.zeros((len(stages), SD))
     sel_rows, sel_d, sel_o, sel_index = [], [], [], {}
     cell_start, cell_items = [], []
+    succ_start, succ_items = [], []
     fcols, icols, icol_kind = [], [], {}
     uslot = 0
     e = 0
@@ -307,6 +328,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
         e += len(st.elements)
         # stage records
         rec = stage_i[s]
+        rec[:] = -1
         rec[:10] = 0
         rec[0] = st.kind
         rec[1] = len(st.elements)
@@ -333,6 +355,11 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
             stage_d[s, 18] = g['U1']
             stage_d[s, 19] = g['W1']
             stage_d[s, 20] = g['delta']
+            stage_d[s, 21] = g['tan_theta']
+            rec[28] = sum(len(c) for c in succ_start)
+            rec[29] = sum(len(c) for c in succ_items)
+            succ_start.append(g['succ_start'])
+            succ_items.append(g['succ_items'])
         else:
             # FP32 ray advance needs an axis even without a grid
             stage_d[s, 3:6] = frames[0]['e_x']
@@ -357,14 +384,15 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
                     fcols.append(name)
                 rec[10 + b] = fcols.index(name)
         rec[19] = 0
-        rec[20:] = 0
-        rec[20:28] = -1
-        rec[28] = -1
+        rec[20:28] = -1     # LDS offsets, set by _build_lds
+        rec[30:32] = -1
+        if g is None:
+            rec[28:30] = -1
     if len(fcols) > 16 or len(icols) > 8:
         raise CompileError('too many distinct output columns')
     tables = CompiledTables(
         stages=stages, stage_i=stage_i, stage_d=stage_d,
-        stage_f=np.concatenate([stage_d[:, 3:21], np.zeros((len(stages), SD - 18))],
+        stage_f=np.concatenate([stage_d[:, 3:22], np.zeros((len(stages), SD - 19))],
                                axis=1).astype(np.float32),
         elem_t=elem_t, elem_p=elem_p, elem_i=elem_i,
         elem_t32=elem_t32, elem_p32=elem_p32,
@@ -376,6 +404,10 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
                     else np.zeros(1, dtype=np.int32)),
         cell_items=(np.concatenate(cell_items).astype(np.int32) if cell_items
                     else np.zeros(1, dtype=np.int32)),
+        succ_start=(np.concatenate(succ_start).astype(np.int32) if succ_start
+                    else np.zeros(1, dtype=np.int32)),
+        succ_items=(np.concatenate(succ_items + [np.zeros(1, np.int32)]).astype(np.int32)
+                    if succ_items else np.zeros(1, dtype=np.int32)),
         fcols=fcols, icols=icols, icol_kind=icol_kind,
         n_uniform_slots=uslot,
         read_pos=stages[0].kind not in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT),
@@ -434,6 +466,7 @@ def _build_lds(tables, budget32=None, budget64=None):
     budget64 = LDS_BUDGET64 if budget64 is None else budget64
     si = tables.stage_i
     si[:, 20:28] = -1
+    si[:, 30:32] = -1
     images = {}
     for prec, budget in ((32, budget32), (64, budget64)):
         parts = []
@@ -474,11 +507,13 @@ def _build_lds(tables, budget32=None, budget64=None):
             recs = (tables.elem_t32 if prec == 32 else tables.elem_t)[st.elem0:st.elem0 + n]
             params = _param_records32(tables, st) if (prec == 32 and st.kind == STAGE_GRATING) else None
             g = st.grid
-            if g is not None and (len(g['items']) >= 65536 or g['cell_start'][-1] >= 65536 or n >= 65536):
+            if g is not None and (len(g['items']) >= 65536 or n >= 65536 or
+                                  len(g['succ_items']) >= 65536):
                 g = None
             size = _align(recs.nbytes) + (_align(params.nbytes) if params is not None else 0)
             if g is not None:
                 size += _align(2 * len(g['cell_start'])) + _align(2 * len(g['items']))
+                size += _align(2 * len(g['succ_start'])) + _align(2 * len(g['succ_items']) + 2)
             if off + size > budget or (st.grid is not None and g is None):
                 continue
             c_rec = 20 if prec == 32 else 21
@@ -489,6 +524,14 @@ def _build_lds(tables, budget32=None, budget64=None):
                 c0, c1 = (24, 25) if prec == 32 else (26, 27)
                 si[s, c0] = add(g['cell_start'].astype(np.uint16))
                 si[s, c1] = add(g['items'].astype(np.uint16))
+                # successor block: (n + 1) uint16 offsets, padded to 16 bytes,
+                # then the uint16 element indices
+                blk = np.zeros(_align(2 * len(g['succ_start'])) // 2 + len(g['succ_items']) + 1,
+                               dtype=np.uint16)
+                blk[:len(g['succ_start'])] = g['succ_start']
+                o0 = _align(2 * len(g['succ_start'])) // 2
+                blk[o0:o0 + len(g['succ_items'])] = g['succ_items']
+                si[s, 30 if prec == 32 else 31] = add(blk)
         img = np.zeros(_align(max(off, 16)), dtype=np.uint8)
         for o, b in parts:
             img[o:o + b.nbytes] = b

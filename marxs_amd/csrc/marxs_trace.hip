@@ -39,8 +39,12 @@
 #define MX_E2W 1.2398419292004202e-06   /* keV -> mm, marxs/__init__.py:14 */
 #define MX_E2W_F 1.2398419292004202e-06f
 #define MX_ORDER_NONE (-2147483647 - 1)
+#ifndef MX_BLOCK32
 #define MX_BLOCK32 768          /* one workgroup per CU holds the LDS image   */
+#endif
+#ifndef MX_BLOCK64
 #define MX_BLOCK64 512
+#endif
 #define MX_DEFAULT_WG_PER_CU 8  /* persistent grid: workgroups per CU         */
 
 static thread_local char g_last_error[512] = "";
@@ -417,25 +421,52 @@ __device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, i
 }
 
 // Next hit of the ray among candidates [tb, te) (cell list when `list`,
-// element indices otherwise) with index > last.  ET points at the stage's
-// element records (LDS or global).  FP64: q, ly, lz of the hit; FP32: kf,
-// ly, lz relative to the advanced float ray.
+// element indices otherwise) with index > last, tested in ascending order.
+// ET points at the stage's element records (LDS or global).  FP64: q, ly,
+// lz of the hit; FP32: kf, ly, lz relative to the advanced float ray.
+// Candidates are fetched and tested MX_SCAN_GROUP at a time (independent
+// tests overlap their memory latency); the first hit in order is returned,
+// so the result equals testing one candidate after another.
+#ifndef MX_SCAN_GROUP
+#define MX_SCAN_GROUP 1
+#endif
 template <int PREC, typename ETP, typename ITP>
 __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, int te,
                                          int last, d3 p, d3 d, f3 pr, f3 df, float kmin32,
                                          d3 &q, double &ly, double &lz, float &kf) {
-    for (int t = tb; t < te; ++t) {
-        const int j = list ? (int)items[t] : t;
-        if (j <= last) continue;
-        bool h;
-        if (PREC == 64) {
-            h = plane_test64((const double *)et + (size_t)j * MX_ET, p, d, q, ly, lz);
-        } else {
-            float fy, fz;
-            h = plane_test32((const float *)et + (size_t)j * MX_ET, pr, df, kmin32, kf, fy, fz);
-            ly = fy; lz = fz;
+    constexpr int G = MX_SCAN_GROUP;
+    for (int t = tb; t < te; t += G) {
+        int jj[G];
+        bool hh[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int tt = t + g;
+            jj[g] = tt < te ? (list ? (int)items[tt] : tt) : -1;
         }
-        if (h) return j;
+        if (PREC == 64) {
+            d3 qq[G];
+            double yy[G], zz[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int j = jj[g] > last ? jj[g] : 0;
+                hh[g] = plane_test64((const double *)et + (size_t)j * MX_ET, p, d, qq[g], yy[g], zz[g])
+                        && jj[g] > last;
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                if (hh[g]) { q = qq[g]; ly = yy[g]; lz = zz[g]; return jj[g]; }
+        } else {
+            float kk[G], yy[G], zz[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int j = jj[g] > last ? jj[g] : 0;
+                hh[g] = plane_test32((const float *)et + (size_t)j * MX_ET, pr, df, kmin32, kk[g], yy[g], zz[g])
+                        && jj[g] > last;
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                if (hh[g]) { kf = kk[g]; ly = yy[g]; lz = zz[g]; return jj[g]; }
+        }
     }
     return -1;
 }
@@ -454,11 +485,18 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
     const d3 O = ld3(sd);
     // tables of this stage
     const unsigned char *rec_l = lds + (L ? si[PREC == 64 ? 21 : 20] : 0);
-    const unsigned short *cs_l = (const unsigned short *)(lds + (L ? si[PREC == 64 ? 26 : 24] : 0));
-    const unsigned short *it_l = (const unsigned short *)(lds + (L ? si[PREC == 64 ? 27 : 25] : 0));
+    const bool lds_lists = L && si[PREC == 64 ? 26 : 24] >= 0;
+    const unsigned short *cs_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 26 : 24] : 0));
+    const unsigned short *it_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 27 : 25] : 0));
+    const unsigned short *su_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 31 : 30] : 0));
+    const int su_o = ((2 * (n_el + 1) + 15) & ~15) >> 1;   // items after the offsets
     const int *cs_g = T.cell_start + si[8];
+    const int *ss_g = T.succ_start + si[28];
+    const int *su_g = T.succ_items + si[29];
     int last = -1, nhit = 0;
     for (;;) {
+        // mode 0: nothing can be hit; 1: culling-grid cell; 2: every element
+        // after `last`; 3: successors of the element hit last
         int cb = 0, ce = 0, mode = 2;
         // FP32: ray relative to the stage origin (one FP64 subtraction),
         // moved along itself to the stage mid-plane; k >= kmin32 is forward
@@ -475,15 +513,34 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             }
             pr = mkf(r0.x + kadv * R.df.x, r0.y + kadv * R.df.y, r0.z + kadv * R.df.z);
             kmin32 = -kadv;
-            if (grid) {
-                if (L && si[24] >= 0) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l);
-                else mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_g);
+        }
+        if (grid && last >= 0) {
+            // after a hit: a ray within theta_max of the stage axis can only
+            // reach the successors of the element it left
+            bool steep;
+            if (PREC == 32) {
+                const float da = fabsf(dotf(R.df, ldf3(sf))) * sf[18];
+                steep = fabsf(dotf(R.df, ldf3(sf + 3))) <= da && fabsf(dotf(R.df, ldf3(sf + 6))) <= da;
+            } else {
+                const double da = fabs(dot3(R.d, ld3(sd + 3))) * sd[21];
+                steep = fabs(dot3(R.d, ld3(sd + 6))) <= da && fabs(dot3(R.d, ld3(sd + 9))) <= da;
+            }
+            if (steep) {
+                mode = 3;
+                if (lds_lists) { cb = su_l[last]; ce = su_l[last + 1]; }
+                else { cb = ss_g[last]; ce = ss_g[last + 1]; }
+                if (cb >= ce) break;
             }
         } else if (grid) {
-            if (L && si[26] >= 0) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l);
-            else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g);
+            if (PREC == 32) {
+                if (lds_lists) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l);
+                else mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_g);
+            } else {
+                if (lds_lists) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l);
+                else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g);
+            }
+            if (mode == 0) break;
         }
-        if (mode == 0) break;
         if (mode == 2 && grid) cnt.grid_fallback++;
         // ---- next hit in ascending element order
         d3 q = mk(0, 0, 0);
@@ -492,19 +549,20 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         const bool list = mode != 2;
         const int tb = list ? cb : last + 1;
         const int te = list ? ce : n_el;
-        const bool lds_list = L && si[PREC == 64 ? 27 : 25] >= 0;
         int hit;
         if (L) {
-            if (lds_list)
-                hit = scan_next<PREC>(rec_l, list, it_l, tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+            if (lds_lists)
+                hit = scan_next<PREC>(rec_l, list, mode == 3 ? su_l + su_o : it_l, tb, te, last,
+                                      R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
             else
-                hit = scan_next<PREC>(rec_l, list, T.cell_items, tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+                hit = scan_next<PREC>(rec_l, list, mode == 3 ? su_g : T.cell_items, tb, te, last,
+                                      R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
         } else if (PREC == 64) {
-            hit = scan_next<PREC>(T.elem_t + (size_t)e0 * MX_ET, list, T.cell_items, tb, te, last,
-                                  R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+            hit = scan_next<PREC>(T.elem_t + (size_t)e0 * MX_ET, list, mode == 3 ? su_g : T.cell_items,
+                                  tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
         } else {
-            hit = scan_next<PREC>(T.elem_t32 + (size_t)e0 * MX_ET, list, T.cell_items, tb, te, last,
-                                  R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+            hit = scan_next<PREC>(T.elem_t32 + (size_t)e0 * MX_ET, list, mode == 3 ? su_g : T.cell_items,
+                                  tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
         }
         if (hit < 0) break;
         if (PREC == 32) {
@@ -718,7 +776,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, i
 static int check_tables(const MxTables *t) {
     if (!t) { set_error("tables is NULL"); return 1; }
     if (t->n_stages < 0 || t->n_stages > 4096) { set_error("n_stages out of range"); return 1; }
-    if (t->n_stages > 0 && (!t->stage_i || !t->stage_d || !t->stage_f || !t->elem_t || !t->elem_p || !t->elem_i ||
+    if (t->n_stages > 0 && (!t->stage_i || !t->stage_d || !t->stage_f || !t->succ_start || !t->succ_items || !t->elem_t || !t->elem_p || !t->elem_i ||
                             !t->elem_t32 || !t->elem_p32 || !t->sel_i || !t->sel_d || !t->sel_o ||
                             !t->cell_start || !t->cell_items)) {
         set_error("a table pointer is NULL");

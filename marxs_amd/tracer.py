@@ -53,7 +53,7 @@ class DeviceTables:
         self._keep = {}
         for name in ('stage_i', 'stage_d', 'stage_f', 'elem_t', 'elem_p', 'elem_i', 'elem_t32',
                      'elem_p32', 'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items',
-                     'lds_image32', 'lds_image64'):
+                     'succ_start', 'succ_items', 'lds_image32', 'lds_image64'):
             arr = np.ascontiguousarray(getattr(t, name))
             self._keep[name] = torch.from_numpy(arr).to(self.device)
         self._stage_i_cache = {frozenset(): self._keep['stage_i']}
@@ -68,8 +68,8 @@ class DeviceTables:
         s.lds_bytes64 = self.host.lds_bytes64
         s.stage_i = stage_i.data_ptr()
         for name in ('stage_d', 'stage_f', 'elem_t', 'elem_p', 'elem_i', 'elem_t32', 'elem_p32',
-                     'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items', 'lds_image32',
-                     'lds_image64'):
+                     'sel_i', 'sel_d', 'sel_o', 'cell_start', 'cell_items', 'succ_start',
+                     'succ_items', 'lds_image32', 'lds_image64'):
             setattr(s, name, k[name].data_ptr())
         s.first_draw_slot = self.host.first_draw_slot
         return s
