@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 8
+#define MX_ABI_VERSION 9
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -127,6 +127,46 @@ typedef struct MxPhotons {
 int mx_trace(const MxTables *tables, const MxPhotons *photons,
              int32_t precision, int32_t flags, void *stream,
              unsigned long long *status);
+
+/* ------------------------------------------------------------ adjoint
+ * Pathwise derivative of a detector coordinate with respect to the frames
+ * and periods of the facets of one grating array.  No reference counterpart
+ * (the reference has no gradients); the perturbations it differentiates are
+ * the 6-DOF facet misalignments of marxs/design/uncertainties.py:36-38 and
+ * the period changes of marxs/design/tolerancing.py:175-201.  Photons are
+ * given as they enter the grating stage (after aperture and lens); facet,
+ * order and detector choices are frozen per photon.  Facet geometry record
+ * (MX_AJ_GEOM doubles): centre (3), e_x (3), e_y (3), e_z (3), period,
+ * groove angle, sign mode (0 FlatGrating, 1 CATGrating), transmission (1/0).
+ * Gradients (13 per facet): d/d centre (3), e_x (3), e_y (3), e_z (3), period.
+ */
+#define MX_AJ_GEOM 16
+#define MX_AJ_NGRAD 13
+
+typedef struct MxAdjointArgs {
+    int64_t n;
+    const double *pos[3];      /* photons entering the grating stage */
+    const double *dir[3];
+    const double *energy;
+    const double *weight;      /* NULL = 1; a zero weight excludes the photon */
+    const int32_t *facet;      /* frozen facet index in the array, < 0: unused */
+    const int32_t *order;      /* frozen diffraction order */
+    const int32_t *ccd;        /* frozen detector index, < 0: unused */
+    const double *geom;        /* [n_facet][MX_AJ_GEOM] */
+    const double *ccd_t;       /* [n_ccd][MX_ET] detector frames (centre, e_x, e_y, e_z, ...) */
+    const double *seed;        /* dL/dy per photon; NULL: forward only */
+    double *y_out;             /* coordinate per photon (NaN if unused); may be NULL */
+    double *grad;              /* [n_facet][MX_AJ_NGRAD], accumulated; NULL: forward only */
+    double *moments;           /* [3] accumulated sum w, sum w y, sum w y^2; may be NULL */
+    int32_t n_facet;
+    int32_t n_ccd;
+    int32_t coord;             /* coordinate y: 0 global x, 1 global y, 2 global z */
+    int32_t precision;         /* MX_PREC_FP64 or MX_PREC_FP32 (accumulation is FP64) */
+    int32_t lds_accumulate;    /* 1: per-workgroup LDS partial sums (n_facet <= 630) */
+    int32_t reserved;
+} MxAdjointArgs;
+
+int mx_adjoint(const MxAdjointArgs *args, void *stream);
 
 /* Fill n int32 values with v (a kernel, so that it can be captured in a
  * hipGraph without a memset node).  No reference counterpart: device

@@ -9,13 +9,13 @@ import os
 
 import torch  # noqa: F401  (loads the HIP runtime first)
 
-__all__ = ['lib', 'MxTables', 'MxPhotons', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
+__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
            'MX_NSTATUS', 'LIB_PATH', 'check']
 
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 8
+MX_ABI_VERSION = 9
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -42,6 +42,19 @@ class MxPhotons(ctypes.Structure):
                 ('pos_in', P * 3), ('dir_in', P * 3), ('pol_in', P * 3), ('prob_in', P)]
 
 
+class MxAdjointArgs(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int64), ('pos', P * 3), ('dir', P * 3), ('energy', P),
+                ('weight', P), ('facet', P), ('order', P), ('ccd', P), ('geom', P),
+                ('ccd_t', P), ('seed', P), ('y_out', P), ('grad', P), ('moments', P),
+                ('n_facet', ctypes.c_int32), ('n_ccd', ctypes.c_int32),
+                ('coord', ctypes.c_int32), ('precision', ctypes.c_int32),
+                ('lds_accumulate', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+
+MX_AJ_GEOM = 16
+MX_AJ_NGRAD = 13
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -61,11 +74,8 @@ def _load():
                                          ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32,
                                          P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_int32, P, P]
-    if hasattr(lib, 'mx_adjoint'):
-        lib.mx_adjoint.restype = ctypes.c_int
-        lib.mx_adjoint.argtypes = [P, P, P, P, P, P, P, ctypes.c_int64, P, P, ctypes.c_int32,
-                                   P, ctypes.c_int32, ctypes.c_int32, P, P, P,
-                                   ctypes.c_int32, P]
+    lib.mx_adjoint.restype = ctypes.c_int
+    lib.mx_adjoint.argtypes = [ctypes.POINTER(MxAdjointArgs), P]
     v = lib.mx_abi_version()
     if v != MX_ABI_VERSION:
         raise ImportError(f'libmarxs_hip.so ABI {v}, expected {MX_ABI_VERSION}: rebuild it')

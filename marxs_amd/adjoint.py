@@ -1,0 +1,260 @@
+"""Gradients of a detector-plane line centroid and width with respect to the
+6-DOF poses and periods of grating facets (pathwise adjoint).
+
+The reference has no derivatives: it estimates alignment tolerances by
+re-tracing perturbed instruments (``run_tolerances``,
+``marxs/design/tolerancing.py:224-283``) with facet perturbations
+``compose(t, euler2mat(rx, ry, rz, 'sxyz'), 1)`` (``uncertainties.py:36-38``)
+inserted into the ``Parallel`` pose chain (``simulator.py:423-437``).  Here
+the same perturbations are parameters ``theta[f] = (dx, dy, dz, rx, ry, rz)``
+plus the period ``D[f]``; with every photon's facet, order and CCD frozen
+(pathwise derivative) the loss
+
+    L = mu + sigma,   mu = sum w y / W,   sigma^2 = sum w (y - mu)^2 / W
+
+of the global detector coordinate ``y`` of the selected photons is smooth in
+them.  ``facet_geometry`` builds the facet frames from the parameters with
+torch (differentiable); :class:`FacetLoss` evaluates ``L`` with the HIP
+replay kernel and its backward runs the HIP adjoint kernel
+(``mx_adjoint``), which recomputes every intersection instead of storing it.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mathutils import decompose44
+from .compiler import compile_tree, STAGE_GRATING, STAGE_DETECTOR
+
+__all__ = ['facet_geometry', 'FacetLoss', 'AdjointProblem', 'euler_sxyz_torch',
+           'central_differences']
+
+
+def _rot(axis, a):
+    c, s = torch.cos(a), torch.sin(a)
+    o, z = torch.ones_like(a), torch.zeros_like(a)
+    if axis == 0:
+        rows = [[o, z, z], [z, c, -s], [z, s, c]]
+    elif axis == 1:
+        rows = [[c, z, s], [z, o, z], [-s, z, c]]
+    else:
+        rows = [[c, -s, z], [s, c, z], [z, z, o]]
+    return torch.stack([torch.stack(r, dim=-1) for r in rows], dim=-2)
+
+
+def euler_sxyz_torch(r):
+    """(..., 3) angles -> (..., 3, 3): fixed-axis rotation x, then y, then z."""
+    return _rot(2, r[..., 2]) @ _rot(1, r[..., 1]) @ _rot(0, r[..., 0])
+
+
+def facet_geometry(base, theta, period, groove, sign_mode, transmission):
+    """Facet geometry records (F, 16) from pose parameters.
+
+    base: dict of float64 tensors from :meth:`AdjointProblem.pose_base`:
+    ``GU`` (4, 4) = container pose . uncertainty, ``t0`` (F, 3) = element
+    centre + common offset, ``Rpost`` (F, 3, 3) = element rotation . common
+    rotation, ``zoom`` (F, 3).  theta: (F, 6) = translation, sxyz angles of
+    the per-element uncertainty.  period: (F,).
+    """
+    GU = base['GU']
+    t = theta[:, :3] + base['t0']
+    rot = euler_sxyz_torch(theta[:, 3:6]) @ base['Rpost']
+    lin = GU[:3, :3] @ (rot * base['zoom'][:, None, :])
+    center = (GU[:3, :3] @ t[..., None])[..., 0] + GU[:3, 3]
+    ex = lin[:, :, 0] / torch.linalg.norm(lin[:, :, 0], dim=1, keepdim=True)
+    ey = lin[:, :, 1] / torch.linalg.norm(lin[:, :, 1], dim=1, keepdim=True)
+    ez = lin[:, :, 2] / torch.linalg.norm(lin[:, :, 2], dim=1, keepdim=True)
+    extra = torch.stack([groove, sign_mode, transmission], dim=1)
+    return torch.cat([center, ex, ey, ez, period[:, None], extra], dim=1)
+
+
+def _launch(args, stream):
+    _lib.check(_lib.lib.mx_adjoint(ctypes.byref(args), ctypes.c_void_p(stream)))
+
+
+class AdjointProblem:
+    """Frozen photon paths through one facet array and one detector array.
+
+    Built from a traced instrument: ``entrance`` holds the photons as they
+    reach the grating stage (pos, dir: (3, N) float64 tensors), ``facet``,
+    ``order``, ``ccd`` the frozen choices (int32, -1 = not used), ``weight``
+    the photon weights (0 excludes a photon), ``energy`` (keV).
+    """
+
+    def __init__(self, gratings, detectors, pos, dir, energy, weight, facet, order, ccd,
+                 coord=1, device=None):
+        self.device = torch.device(device) if device is not None else pos.device
+        self.gratings = gratings
+        self.facets = list(gratings.elements)
+        self.coord = coord
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self.pos = pos.to(**f64).contiguous()
+        self.dir = dir.to(**f64).contiguous()
+        self.energy = energy.to(**f64).contiguous()
+        self.weight = weight.to(**f64).contiguous()
+        self.facet = facet.to(dtype=torch.int32, device=self.device).contiguous()
+        self.order = order.to(dtype=torch.int32, device=self.device).contiguous()
+        self.ccd = ccd.to(dtype=torch.int32, device=self.device).contiguous()
+        self.n = self.pos.shape[1]
+        ct = compile_tree(list(detectors.elements) if hasattr(detectors, 'elements') else detectors)
+        self.ccd_t = torch.as_tensor(ct.elem_t, **f64).contiguous()
+        self.n_ccd = self.ccd_t.shape[0]
+        F = len(self.facets)
+        self.groove = torch.tensor([e.geometry['groove_angle'] for e in self.facets], **f64)
+        self.sign_mode = torch.tensor([1. if e.sign_convention == 'cat' else 0. for e in self.facets], **f64)
+        self.transmission = torch.tensor([1. if e.transmission else 0. for e in self.facets], **f64)
+        self.period0 = torch.tensor([float(e._d) for e in self.facets], **f64)
+        self.base = self.pose_base()
+        self.theta0 = self.current_theta()
+        self.F = F
+
+    # ------------------------------------------------------------ poses
+    def pose_base(self):
+        """Fixed parts of the element pose chain (simulator.py:423-437)."""
+        g = self.gratings
+        f64 = dict(dtype=torch.float64, device=self.device)
+        GU = torch.as_tensor(g.pos4d @ g.uncertainty, **f64)
+        t0, rpost, zoom = [], [], []
+        for i, e in enumerate(self.facets):
+            from .base import parse_position_keywords
+            args = g._specific_args(i)
+            ep = parse_position_keywords(args)
+            te, re, ze, _ = decompose44(ep)
+            tc, rc, zc, _ = decompose44(g.elem_pos[i])
+            t0.append(tc + te)
+            rpost.append(rc @ re)
+            zoom.append(zc * ze)
+        return {'GU': GU, 't0': torch.as_tensor(np.array(t0), **f64),
+                'Rpost': torch.as_tensor(np.array(rpost), **f64),
+                'zoom': torch.as_tensor(np.array(zoom), **f64)}
+
+    def current_theta(self):
+        """6-DOF parameters of the current elem_uncertainty (translation and
+        sxyz angles recovered from each 4x4 matrix)."""
+        th = []
+        for u in self.gratings.elem_uncertainty:
+            t, r, z, s = decompose44(u)
+            # sxyz: R = Rz(c) Ry(b) Rx(a)
+            b = -math.asin(max(-1., min(1., r[2, 0])))
+            a = math.atan2(r[2, 1], r[2, 2])
+            c = math.atan2(r[1, 0], r[0, 0])
+            th.append([t[0], t[1], t[2], a, b, c])
+        return torch.tensor(th, dtype=torch.float64, device=self.device)
+
+    def geometry(self, theta, period):
+        return facet_geometry(self.base, theta, period, self.groove, self.sign_mode,
+                              self.transmission)
+
+    # ----------------------------------------------------------- kernels
+    def _args(self, geom, precision, seed=None, y_out=None, grad=None, moments=None):
+        a = _lib.MxAdjointArgs()
+        a.n = self.n
+        for k in range(3):
+            a.pos[k] = self.pos[k].data_ptr()
+            a.dir[k] = self.dir[k].data_ptr()
+        a.energy = self.energy.data_ptr()
+        a.weight = self.weight.data_ptr()
+        a.facet = self.facet.data_ptr()
+        a.order = self.order.data_ptr()
+        a.ccd = self.ccd.data_ptr()
+        a.geom = geom.data_ptr()
+        a.ccd_t = self.ccd_t.data_ptr()
+        a.seed = seed.data_ptr() if seed is not None else None
+        a.y_out = y_out.data_ptr() if y_out is not None else None
+        a.grad = grad.data_ptr() if grad is not None else None
+        a.moments = moments.data_ptr() if moments is not None else None
+        a.n_facet = geom.shape[0]
+        a.n_ccd = self.n_ccd
+        a.coord = self.coord
+        a.precision = {64: 64, 32: 32}[precision]
+        a.lds_accumulate = 1 if geom.shape[0] * 13 * 8 <= 64 * 1024 else 0
+        return a
+
+    def coordinates(self, geom, precision=64):
+        """Per-photon detector coordinate (NaN for unused photons)."""
+        geom = geom.detach().contiguous()
+        y = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        _launch(self._args(geom, precision, y_out=y), torch.cuda.current_stream(self.device).cuda_stream)
+        return y
+
+    def backward_kernel(self, geom, seed, precision=64):
+        geom = geom.detach().contiguous()
+        grad = torch.zeros((geom.shape[0], 13), dtype=torch.float64, device=self.device)
+        _launch(self._args(geom, precision, seed=seed.contiguous(), grad=grad),
+                torch.cuda.current_stream(self.device).cuda_stream)
+        return grad
+
+    def restrict_to_traced(self, y_traced, rtol=1e-10):
+        """Zero the weight of photons whose traced detector coordinate the
+        one-facet replay does not reproduce (photons that met more than one
+        facet or detector in the trace).  Returns the number excluded."""
+        y = self.coordinates(self.geometry(self.theta0, self.period0))
+        used = self.weight != 0
+        bad = used & ~(torch.abs(y - y_traced) <= rtol * torch.abs(y_traced).clamp(min=1.0))
+        self.weight = torch.where(bad, torch.zeros_like(self.weight), self.weight).contiguous()
+        return int(bad.sum())
+
+    def loss(self, theta, period, precision=64):
+        """L(theta, period) = centroid + RMS width (differentiable)."""
+        geom = self.geometry(theta, period)
+        return FacetLoss.apply(geom, self, precision)
+
+
+def _moments(y, w):
+    ok = ~torch.isnan(y)
+    wv = torch.where(ok, w, torch.zeros_like(w))
+    yv = torch.where(ok, y, torch.zeros_like(y))
+    W = wv.sum()
+    mu = (wv * yv).sum() / W
+    var = (wv * (yv - mu) ** 2).sum() / W
+    return W, mu, torch.sqrt(var), ok
+
+
+class FacetLoss(torch.autograd.Function):
+    """L = mu + sigma of the detector coordinate; backward = HIP adjoint."""
+
+    @staticmethod
+    def forward(ctx, geom, prob, precision):
+        y = prob.coordinates(geom, precision)
+        W, mu, sigma, ok = _moments(y, prob.weight)
+        ctx.prob = prob
+        ctx.precision = precision
+        ctx.save_for_backward(geom.detach(), y, mu, sigma, W)
+        return mu + sigma
+
+    @staticmethod
+    def backward(ctx, gL):
+        geom, y, mu, sigma, W = ctx.saved_tensors
+        prob = ctx.prob
+        ok = ~torch.isnan(y)
+        seed = torch.where(ok, prob.weight / W * (1 + (y - mu) / sigma), torch.zeros_like(y))
+        g = prob.backward_kernel(geom, seed * gL, ctx.precision)
+        full = torch.zeros_like(geom)
+        full[:, :13] = g
+        return full, None, None
+
+
+def central_differences(prob, theta, period, h_t=1e-4, h_r=1e-6, h_d=1e-10, facets=None,
+                        precision=64):
+    """Central finite differences of L for each facet parameter (columns:
+    dx, dy, dz, rx, ry, rz, period), with the photon paths frozen."""
+    F = theta.shape[0]
+    facets = range(F) if facets is None else facets
+    out = torch.full((F, 7), float('nan'), dtype=torch.float64, device=theta.device)
+    steps = [h_t] * 3 + [h_r] * 3 + [h_d]
+    with torch.no_grad():
+        for f in facets:
+            for k in range(7):
+                vals = []
+                for sgn in (1., -1.):
+                    th = theta.clone()
+                    pe = period.clone()
+                    if k < 6:
+                        th[f, k] += sgn * steps[k]
+                    else:
+                        pe[f] += sgn * steps[k]
+                    vals.append(FacetLoss.apply(prob.geometry(th, pe), prob, precision))
+                out[f, k] = (vals[0] - vals[1]) / (2 * steps[k])
+    return out

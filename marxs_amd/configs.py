@@ -92,7 +92,7 @@ def _gas_with_n(rowland, n, radius, d_element, elem_args):
 
 
 def cat_rowland_instrument(n_facets=200, R=5900., r=6000., blaze_deg=1.5,
-                           n_ccd=16, ccd_y=(200., 600.), efficiency=None,
+                           n_ccd=16, ccd_y=(320., 720.), efficiency=None,
                            focal=12500., d=2e-4):
     """Aperture, PerfectLens, a CAT facet array on a Rowland torus and a row of
     CCDs on the Rowland circle.  Returns (Sequence, parts dict)."""

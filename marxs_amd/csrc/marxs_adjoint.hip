@@ -1,0 +1,255 @@
+// marxs_adjoint.hip -- pathwise adjoint of the detector coordinate with
+// respect to grating-facet frames and periods (MI355X, gfx950).
+//
+// A photon that entered the grating stage at p0 with direction d0 and was
+// diffracted by facet f into order m and then detected on CCD c lands at
+//
+//   q1 = p0 + k1 d0,          k1 = ((c_f - p0) . n) / (d0 . n)     facet plane
+//   d1 = p_d dg + p_l l + s_n p_n n                                 grating eq.
+//        p_d = p . dg + s m lambda / D,  p_l = p . l,
+//        p_n = sqrt(1 - p_d^2 - p_l^2),  p = d0 / |d0|
+//   q2 = q1 + k2 d1,          k2 = ((C_c - q1) . N_c) / (d1 . N_c)  CCD plane
+//   y  = q2 . u                                                     coordinate
+//
+// (FinitePlane.intersect, marxs/math/geometry.py:233-253; the grating
+// equation of FlatGrating.diffract_photons, marxs/optics/grating.py:233-266,
+// with the groove frame l = sin(-g) e_y + cos(-g) e_z, dg = -(cos(-g) e_y -
+// sin(-g) e_z), n = e_x of grating.py:186-207).  With the facet, the order,
+// the CCD and both sign choices (s, s_n) frozen per photon, y is a smooth
+// function of the facet's centre c_f, axes e_x, e_y, e_z and period D.  The
+// kernel recomputes the forward quantities per photon (nothing is stored
+// between the forward and the backward pass) and back-propagates a seed
+// dL/dy into the 13 facet quantities, accumulated per facet in LDS and then
+// in global memory (float64).  The 6-DOF pose parameters are reached from
+// these by the host (torch autograd over the pose chain).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdio.h>
+#include <math.h>
+
+#include "marxs_hip.h"
+
+#define AJ_E2W 1.2398419292004202e-06   /* keV -> mm, marxs/__init__.py:14 */
+#define AJ_BLOCK 256
+#define AJ_NG 13                        /* c(3) e_x(3) e_y(3) e_z(3) D */
+
+__attribute__((visibility("hidden"))) void mx_set_error(const char *msg);
+
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> __device__ __forceinline__ V3<T> v3(T x, T y, T z) { V3<T> r; r.x = x; r.y = y; r.z = z; return r; }
+template <typename T> __device__ __forceinline__ T dt(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> __device__ __forceinline__ V3<T> ad(V3<T> a, V3<T> b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename T> __device__ __forceinline__ V3<T> sb(V3<T> a, V3<T> b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename T> __device__ __forceinline__ V3<T> sc(T s, V3<T> a) { return v3(s * a.x, s * a.y, s * a.z); }
+template <typename T> __device__ __forceinline__ V3<T> ld(const double *p) { return v3((T)p[0], (T)p[1], (T)p[2]); }
+
+// Forward replay and (optionally) backward pass for one photon in precision
+// T.  Positions enter relative to the facet centre (exact in FP64, one
+// rounding in FP32): r0 = p0 - c_f, rc = C_c - c_f.  Returns y - c_f . u.
+template <typename T>
+__device__ __forceinline__ T photon_adjoint(V3<T> r0, V3<T> d0, V3<T> rc, V3<T> N, V3<T> u,
+                                            V3<T> ex, V3<T> ey, V3<T> ez, T D, T sg, T cg,
+                                            int sign_mode, T trans, T wave, int m,
+                                            bool backward, T yb, T *g) {
+    // facet plane: q1 = r0 + k1 d0 (relative to c_f), plane through 0
+    const V3<T> n = ex;
+    const T num1 = -dt(r0, n);
+    const T den1 = dt(d0, n);
+    const T k1 = num1 / den1;
+    const V3<T> q1 = ad(r0, sc(k1, d0));
+    // groove frame and diffraction
+    const V3<T> l = ad(sc(sg, ey), sc(cg, ez));
+    const V3<T> dg = sb(sc(sg, ez), sc(cg, ey));
+    const T inv = (T)1 / sqrt(dt(d0, d0));
+    const V3<T> p = sc(inv, d0);
+    const T pd0 = dt(p, dg);
+    const T s = sign_mode == 0 ? (T)-1 : (pd0 < (T)0 ? (T)-1 : (T)1);
+    const T a = s * (T)m * wave;
+    const T p_d = pd0 + a / D;
+    const T p_l = dt(p, l);
+    const T p_n = sqrt((T)1 - p_d * p_d - p_l * p_l);
+    const T pn0 = dt(p, n);
+    const T sn = (pn0 > (T)0 ? (T)1 : (pn0 < (T)0 ? (T)-1 : (T)0)) * trans;
+    const T w = sn * p_n;
+    const V3<T> d1 = ad(ad(sc(p_d, dg), sc(p_l, l)), sc(w, n));
+    // CCD plane
+    const T num2 = dt(sb(rc, q1), N);
+    const T den2 = dt(d1, N);
+    const T k2 = num2 / den2;
+    const V3<T> q2 = ad(q1, sc(k2, d1));
+    const T y = dt(q2, u);
+    if (!backward) return y;
+    // ---- backward: seed yb = dL/dy
+    V3<T> q2b = sc(yb, u);
+    V3<T> q1b = q2b;
+    const T k2b = dt(q2b, d1);
+    V3<T> d1b = sc(k2, q2b);
+    const T num2b = k2b / den2;
+    const T den2b = -k2b * k2 / den2;
+    q1b = sb(q1b, sc(num2b, N));
+    d1b = ad(d1b, sc(den2b, N));
+    // d1 = p_d dg + p_l l + w n
+    T p_db = dt(d1b, dg);
+    T p_lb = dt(d1b, l);
+    const T wb = dt(d1b, n);
+    V3<T> dgb = sc(p_d, d1b);
+    V3<T> lb = sc(p_l, d1b);
+    V3<T> nb = sc(w, d1b);
+    const T p_nb = sn * wb;
+    p_db = p_db - (p_d / p_n) * p_nb;
+    p_lb = p_lb - (p_l / p_n) * p_nb;
+    // p_d = p . dg + a / D ; p_l = p . l
+    dgb = ad(dgb, sc(p_db, p));
+    const T Db = -p_db * a / (D * D);
+    lb = ad(lb, sc(p_lb, p));
+    // l = sg ey + cg ez ; dg = sg ez - cg ey
+    V3<T> eyb = sb(sc(sg, lb), sc(cg, dgb));
+    V3<T> ezb = ad(sc(cg, lb), sc(sg, dgb));
+    // q1 = r0 + k1 d0 ; k1 = num1 / den1 ; num1 = (c_f - p0) . n = -r0 . n
+    // (c_f enters through r0 = p0 - c_f and rc = C - c_f)
+    const T k1b = dt(q1b, d0);
+    const T num1b = k1b / den1;
+    const T den1b = -k1b * k1 / den1;
+    nb = ad(nb, ad(sc(-num1b, r0), sc(den1b, d0)));
+    // d c_f: q1 depends on c_f through r0 (q1 = r0 + k1 d0 with r0 = p0 - c_f)
+    // and y = (q2 + c_f) . u; rc = C - c_f.
+    //   dr0 = -dc ; drc = -dc ; dy_abs = u . dc
+    const V3<T> r0b = ad(q1b, sc(-num1b, n));     // from q1 = r0 + ... and num1 = -r0.n
+    const V3<T> rcb = sc(num2b, N);               // num2 = (rc - q1) . N
+    const V3<T> cb = sb(sb(sc(yb, u), r0b), rcb);
+    g[0] = cb.x; g[1] = cb.y; g[2] = cb.z;
+    g[3] = nb.x; g[4] = nb.y; g[5] = nb.z;
+    g[6] = eyb.x; g[7] = eyb.y; g[8] = eyb.z;
+    g[9] = ezb.x; g[10] = ezb.y; g[11] = ezb.z;
+    g[12] = Db;
+    return y;
+}
+
+// wave64 sum
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
+    extern __shared__ __align__(16) double acc[];     // [n_facet][AJ_NG] when A.grad
+    const bool lds_acc = A.grad && A.lds_accumulate;
+    if (lds_acc) {
+        for (int k = threadIdx.x; k < A.n_facet * AJ_NG; k += AJ_BLOCK) acc[k] = 0.0;
+        __syncthreads();
+    }
+    double m0 = 0, m1 = 0, m2 = 0;
+    const int64_t stride = (int64_t)gridDim.x * AJ_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * AJ_BLOCK + threadIdx.x; i < A.n; i += stride) {
+        const int f = A.facet[i], c = A.ccd[i];
+        const double wgt = A.weight ? A.weight[i] : 1.0;
+        if (f < 0 || f >= A.n_facet || c < 0 || c >= A.n_ccd || !(wgt != 0.0)) {
+            if (A.y_out) A.y_out[i] = __builtin_nan("");
+            continue;
+        }
+        const double *G = A.geom + (size_t)f * MX_AJ_GEOM;
+        const double *C = A.ccd_t + (size_t)c * MX_ET;
+        const double cu = A.coord == 0 ? G[0] : (A.coord == 1 ? G[1] : G[2]);
+        const double ang = G[13];
+        const double sgd = sin(-ang), cgd = cos(-ang);
+        const int sign_mode = (int)G[14];
+        const double trans = G[15] != 0.0 ? 1.0 : -1.0;
+        const double wave = AJ_E2W / A.energy[i];
+        const int m = A.order[i];
+        const bool bw = A.seed != nullptr && A.grad != nullptr;
+        const double yb = bw ? A.seed[i] : 0.0;
+        double y;
+        double g[AJ_NG];
+        // relative positions in FP64 (exact origin subtraction)
+        const double r0x = A.pos[0][i] - G[0], r0y = A.pos[1][i] - G[1], r0z = A.pos[2][i] - G[2];
+        const double rcx = C[0] - G[0], rcy = C[1] - G[1], rcz = C[2] - G[2];
+        if (PREC == 64) {
+            double gg[AJ_NG];
+            y = photon_adjoint<double>(v3(r0x, r0y, r0z), v3(A.dir[0][i], A.dir[1][i], A.dir[2][i]),
+                                       v3(rcx, rcy, rcz), ld<double>(C + 3),
+                                       v3((double)(A.coord == 0), (double)(A.coord == 1), (double)(A.coord == 2)),
+                                       ld<double>(G + 3), ld<double>(G + 6), ld<double>(G + 9), G[12],
+                                       sgd, cgd, sign_mode, trans, wave, m, bw, yb, gg);
+            if (bw) for (int k = 0; k < AJ_NG; ++k) g[k] = gg[k];
+        } else {
+            float gf[AJ_NG];
+            y = (double)photon_adjoint<float>(v3((float)r0x, (float)r0y, (float)r0z),
+                                              v3((float)A.dir[0][i], (float)A.dir[1][i], (float)A.dir[2][i]),
+                                              v3((float)rcx, (float)rcy, (float)rcz), ld<float>(C + 3),
+                                              v3((float)(A.coord == 0), (float)(A.coord == 1), (float)(A.coord == 2)),
+                                              ld<float>(G + 3), ld<float>(G + 6), ld<float>(G + 9), (float)G[12],
+                                              (float)sgd, (float)cgd, sign_mode, (float)trans, (float)wave, m,
+                                              bw, (float)yb, gf);
+            if (bw) for (int k = 0; k < AJ_NG; ++k) g[k] = (double)gf[k];
+        }
+        y += cu;
+        if (A.y_out) A.y_out[i] = y;
+        m0 += wgt;
+        m1 += wgt * y;
+        m2 += wgt * y * y;
+        if (bw) {
+            if (lds_acc) {
+                for (int k = 0; k < AJ_NG; ++k) atomicAdd(&acc[f * AJ_NG + k], g[k]);
+            } else {
+                for (int k = 0; k < AJ_NG; ++k) atomicAdd(&A.grad[(size_t)f * AJ_NG + k], g[k]);
+            }
+        }
+    }
+    if (A.moments) {
+        m0 = wave_sum(m0);
+        m1 = wave_sum(m1);
+        m2 = wave_sum(m2);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(A.moments + 0, m0);
+            atomicAdd(A.moments + 1, m1);
+            atomicAdd(A.moments + 2, m2);
+        }
+    }
+    if (lds_acc) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < A.n_facet * AJ_NG; k += AJ_BLOCK)
+            if (acc[k] != 0.0) atomicAdd(&A.grad[k], acc[k]);
+    }
+}
+
+extern "C" int mx_adjoint(const MxAdjointArgs *a, void *stream) {
+    if (!a) { mx_set_error("args is NULL"); return 1; }
+    if (a->n < 0 || a->n_facet < 0 || a->n_ccd < 0) { mx_set_error("negative size"); return 1; }
+    if (a->n == 0) return 0;
+    if (!a->pos[0] || !a->pos[1] || !a->pos[2] || !a->dir[0] || !a->dir[1] || !a->dir[2] ||
+        !a->energy || !a->facet || !a->order || !a->ccd || !a->geom || !a->ccd_t) {
+        mx_set_error("a required pointer is NULL");
+        return 1;
+    }
+    if (a->coord < 0 || a->coord > 2) { mx_set_error("coord must be 0, 1 or 2"); return 1; }
+    if (a->precision != MX_PREC_FP64 && a->precision != MX_PREC_FP32) {
+        mx_set_error("precision must be 64 or 32");
+        return 1;
+    }
+    if (a->seed && !a->grad) { mx_set_error("seed given without grad"); return 1; }
+    const size_t lds = (a->grad && a->lds_accumulate) ? (size_t)a->n_facet * AJ_NG * sizeof(double) : 0;
+    if (lds > 64 * 1024) { mx_set_error("too many facets for LDS accumulation"); return 1; }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    int64_t nblk = (a->n + AJ_BLOCK - 1) / AJ_BLOCK;
+    int64_t grid = (int64_t)cus * 4;
+    if (grid > nblk) grid = nblk;
+    if (a->precision == MX_PREC_FP64)
+        hipLaunchKernelGGL(mx_adjoint_kernel<64>, dim3((unsigned)grid), dim3(AJ_BLOCK), lds,
+                           (hipStream_t)stream, *a);
+    else
+        hipLaunchKernelGGL(mx_adjoint_kernel<32>, dim3((unsigned)grid), dim3(AJ_BLOCK), lds,
+                           (hipStream_t)stream, *a);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "adjoint launch failed: %s", hipGetErrorString(err));
+        mx_set_error(buf);
+        return 2;
+    }
+    return 0;
+}

@@ -55,6 +55,7 @@ static void set_error(const char *msg) {
 }
 
 extern "C" const char *mx_last_error(void) { return g_last_error; }
+__attribute__((visibility("hidden"))) void mx_set_error(const char *msg) { set_error(msg); }
 extern "C" int mx_abi_version(void) { return MX_ABI_VERSION; }
 
 // ------------------------------------------------------------------ FP64

@@ -1,0 +1,114 @@
+"""Adjoint of the detector line centroid + width w.r.t. facet poses and
+periods (needs an MI355X).
+
+Checks: the replay coordinate equals the parity-tested trace; the FP64
+adjoint agrees with central finite differences (frozen paths) to <= 1e-4
+relative; the FP32 adjoint agrees with the FP64 adjoint to within the
+relative error its FP32 detector coordinates put into the seeds
+(rms(y32 - y64) / sigma, at least 1e-4)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def problem(gpu):
+    import torch
+    from marxs_amd import configs, tolerancing
+    from marxs_amd.adjoint import AdjointProblem
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    import marxs_amd as mx
+    inst, parts = configs.cat_rowland_instrument(200)
+    tolerancing.wiggle(parts['gratings'], dx=0.1, dy=0.1, dz=0.1, rx=1.4544e-4, ry=1.4544e-4,
+                       rz=1.4544e-4, rng=np.random.default_rng(5))
+    n = 200_000
+    rng = np.random.default_rng(6)
+    ci = CompiledInstrument(inst)
+    u = rng.random((ci.n_uniform_slots + 1, n))
+    ph = configs.onaxis_photons(np.full(n, 0.65), u[-1])
+    uni = torch.as_tensor(np.ascontiguousarray(u[:ci.n_uniform_slots]), device=gpu)
+    full = DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'], device=gpu)
+    ci.trace_device(full, uniforms=uni)
+    front = CompiledInstrument(mx.Sequence(elements=[parts['aperture'], parts['lens']]))
+    ent = DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'], device=gpu)
+    front.trace_device(ent, uniforms=uni[:2].contiguous())
+    sel = (full.icols['order'] == -6) & (full.icols['CCD'] >= 0)
+    weight = torch.where(sel, full.prob, torch.zeros_like(full.prob))
+    prob = AdjointProblem(parts['gratings'], parts['ccds'], ent.pos, ent.dir, ent.energy, weight,
+                          full.icols['facet'], full.icols['order'], full.icols['CCD'], coord=1)
+    y = prob.coordinates(prob.geometry(prob.theta0, prob.period0))
+    dev = (torch.abs(y - full.pos[1]) / torch.abs(full.pos[1]))[sel]
+    print('replay vs trace: max rel', float(dev.max()), 'n >1e-10', int((dev > 1e-10).sum()),
+          'of', int(sel.sum()))
+    prob.n_selected = int(sel.sum())
+    prob.n_dropped = prob.restrict_to_traced(full.pos[1])
+    sel = sel & (prob.weight != 0)
+    return prob, full, sel
+
+
+def test_replay_matches_trace(problem):
+    prob, full, sel = problem
+    import torch
+    y = prob.coordinates(prob.geometry(prob.theta0, prob.period0))
+    ok = sel.cpu().numpy()
+    assert ok.sum() > 1000
+    # The one-facet replay reproduces the traced coordinate to 1e-10 except
+    # for photons that met two facets (or two CCDs) in the trace; those few
+    # are excluded from the loss.
+    assert prob.n_dropped <= 0.01 * prob.n_selected
+    yt = full.pos[1].cpu().numpy()
+    yr = y.cpu().numpy()
+    assert np.allclose(yr[ok], yt[ok], rtol=1e-10, atol=1e-9)
+    assert np.isnan(yr[~ok]).all()
+
+
+def test_adjoint_vs_finite_differences(problem):
+    import torch
+    prob, full, sel = problem
+    th = prob.theta0.clone().requires_grad_(True)
+    pe = prob.period0.clone().requires_grad_(True)
+    L = prob.loss(th, pe, precision=64)
+    L.backward()
+    g = torch.cat([th.grad, pe.grad[:, None]], dim=1)
+    # the facets carrying most of the selected photons
+    counts = torch.bincount(prob.facet[sel].long(), minlength=prob.F)
+    top = torch.argsort(counts, descending=True)[:6].tolist()
+    from marxs_amd.adjoint import central_differences
+    fd = central_differences(prob, prob.theta0, prob.period0, facets=top)
+    for k, name in enumerate(['dx', 'dy', 'dz', 'rx', 'ry', 'rz', 'period']):
+        a = g[top, k]
+        b = fd[top, k]
+        scale = torch.abs(g[:, k]).max()
+        err = float((torch.abs(a - b) / scale).max())
+        print(name, float(scale), err)
+        assert err <= 1e-4, (name, err)
+
+
+def test_fp32_adjoint_matches_fp64(problem):
+    import torch
+    prob, full, sel = problem
+    grads = {}
+    for prec in (64, 32):
+        th = prob.theta0.clone().requires_grad_(True)
+        pe = prob.period0.clone().requires_grad_(True)
+        prob.loss(th, pe, precision=prec).backward()
+        grads[prec] = torch.cat([th.grad, pe.grad[:, None]], dim=1)
+    g64 = prob.geometry(prob.theta0, prob.period0)
+    y64 = prob.coordinates(g64, 64)
+    y32 = prob.coordinates(g64, 32)
+    ok = ~torch.isnan(y64)
+    w = prob.weight[ok]
+    mu = (w * y64[ok]).sum() / w.sum()
+    sigma = float(torch.sqrt((w * (y64[ok] - mu) ** 2).sum() / w.sum()))
+    rms = float(torch.sqrt(((y32 - y64)[ok] ** 2).mean()))
+    print('sigma', sigma, 'fp32 y rms error', rms, 'max', float(torch.abs(y32 - y64)[ok].max()))
+    # The seeds dL/dy = w/W (1 + (y - mu)/sigma) inherit the FP32 coordinate
+    # error: relative error ~ rms(y32 - y64)/sigma (~1e-3 for a 60 um line and
+    # 0.07 um FP32 error), which bounds the FP32 gradient error.
+    tol = max(1e-4, rms / sigma)
+    for k in range(7):
+        scale = torch.abs(grads[64][:, k]).max()
+        err = float((torch.abs(grads[32][:, k] - grads[64][:, k]) / scale).max())
+        print(k, float(scale), err, tol)
+        assert err <= tol, (k, err, tol)
