@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 9
+#define MX_ABI_VERSION 10
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -127,6 +127,35 @@ typedef struct MxPhotons {
 int mx_trace(const MxTables *tables, const MxPhotons *photons,
              int32_t precision, int32_t flags, void *stream,
              unsigned long long *status);
+
+/* ------------------------------------------------------------ batched trace
+ * Many instrument variants, each with its own tables (ragged element counts)
+ * and its own photons, in one persistent launch.  Replaces the per-variant
+ * trace inside the reference's sequential tolerancing loop run_tolerances
+ * (marxs/design/tolerancing.py:224-283: modify the instrument, trace
+ * photons_in.copy(), analyse); each variant's result equals mx_trace on that
+ * variant.  The work is a list of tiles: variant v owns tiles
+ * tile_first[v] .. tile_first[v+1]-1 of tile_photons[v] photons each, in the
+ * order of the arrays.  Workgroups take tiles from the device counter
+ * queue[0] (set to zero by this call with a fill kernel, so that the call can
+ * be captured in a hipGraph) and reload their LDS image only when the
+ * variant changes.  The host arrays are checked here; the device arrays must
+ * hold the same structs (the kernel reads those). */
+typedef struct MxBatch {
+    int32_t n_variants;
+    int32_t grid;                    /* workgroups; 0 = CUs x MX_FLAG_WG_PER_CU */
+    int64_t n_tiles;                 /* = tile_first[n_variants] */
+    const MxTables *tables;          /* HOST array [n_variants] */
+    const MxPhotons *photons;        /* HOST array [n_variants] */
+    const MxTables *tables_dev;      /* device copy of tables[] */
+    const MxPhotons *photons_dev;    /* device copy of photons[] */
+    const int64_t *tile_first;       /* device [n_variants + 1], non-decreasing, [0] = 0 */
+    const int32_t *tile_photons;     /* device [n_variants], >= 1 */
+    unsigned long long *queue;       /* device workspace: one counter */
+    unsigned long long *status;      /* device [n_variants][MX_NSTATUS] or NULL */
+} MxBatch;
+
+int mx_trace_batched(const MxBatch *batch, int32_t precision, int32_t flags, void *stream);
 
 /* ------------------------------------------------------------ adjoint
  * Pathwise derivative of a detector coordinate with respect to the frames

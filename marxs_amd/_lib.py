@@ -9,13 +9,13 @@ import os
 
 import torch  # noqa: F401  (loads the HIP runtime first)
 
-__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
+__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
            'MX_NSTATUS', 'LIB_PATH', 'check']
 
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 9
+MX_ABI_VERSION = 10
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -51,6 +51,13 @@ class MxAdjointArgs(ctypes.Structure):
                 ('lds_accumulate', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
+class MxBatch(ctypes.Structure):
+    _fields_ = [('n_variants', ctypes.c_int32), ('grid', ctypes.c_int32),
+                ('n_tiles', ctypes.c_int64), ('tables', P), ('photons', P),
+                ('tables_dev', P), ('photons_dev', P), ('tile_first', P),
+                ('tile_photons', P), ('queue', P), ('status', P)]
+
+
 MX_AJ_GEOM = 16
 MX_AJ_NGRAD = 13
 
@@ -68,12 +75,8 @@ def _load():
                              ctypes.c_int32, ctypes.c_int32, P, P]
     lib.mx_fill_i32.restype = ctypes.c_int
     lib.mx_fill_i32.argtypes = [P, ctypes.c_int64, ctypes.c_int32, P]
-    if hasattr(lib, 'mx_trace_batched'):
-        lib.mx_trace_batched.restype = ctypes.c_int
-        lib.mx_trace_batched.argtypes = [ctypes.POINTER(MxTables), ctypes.POINTER(MxPhotons),
-                                         ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32,
-                                         P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                         ctypes.c_int32, P, P]
+    lib.mx_trace_batched.restype = ctypes.c_int
+    lib.mx_trace_batched.argtypes = [ctypes.POINTER(MxBatch), ctypes.c_int32, ctypes.c_int32, P]
     lib.mx_adjoint.restype = ctypes.c_int
     lib.mx_adjoint.argtypes = [ctypes.POINTER(MxAdjointArgs), P]
     v = lib.mx_abi_version()

@@ -32,7 +32,7 @@ from .optics import (CircleAperture, PerfectLens, FlatGrating, CATGrating,
 from .rowland import RowlandTorus, GratingArrayStructure, RectangularGrid
 from .simulator import Sequence, Parallel
 
-__all__ = ['flat_grid_instrument', 'cat_rowland_instrument', 'blaze_efficiency',
+__all__ = ['flat_grid_instrument', 'cat_rowland_instrument', 'blaze_efficiency', 'design_variants',
            'onaxis_photons', 'line_spectrum_energies', 'uniform_energies',
            'CAT_ORDERS', 'LINES']
 
@@ -66,24 +66,34 @@ def blaze_efficiency(energy_grid=None, orders=CAT_ORDERS, center=-6., sigma=3.):
     return EfficiencyFile.from_arrays(energy_grid, prob, orders)
 
 
+def _gas_count(radius, d_element, phi):
+    """Number of facets GratingArrayStructure places for these arguments
+    (its yz layout only, without solving for positions on the torus)."""
+    probe = GratingArrayStructure.__new__(GratingArrayStructure)
+    probe.radius, probe.d_element, probe.phi = radius, d_element, phi
+    return len(probe.elemposyz()[0])
+
+
 def _gas_with_n(rowland, n, radius, d_element, elem_args):
     """GratingArrayStructure over a wedge around +z, widened until it holds at
     least ``n`` facets, then cut to the first ``n``."""
     lo, hi = 0.01, math.pi
-    gas = None
+    best = None
+    if _gas_count(radius, d_element, [-hi, hi]) < n:
+        raise ValueError(f'cannot place {n} facets in radius {radius}')
     for _ in range(60):
         half = 0.5 * (lo + hi)
-        g = GratingArrayStructure(rowland=rowland, d_element=d_element, radius=radius,
-                                  phi=[-half, half], elem_class=CATGrating,
-                                  elem_args=dict(elem_args), id_col='facet')
-        if len(g.elem_pos) >= n:
-            gas, hi = g, half
+        c = _gas_count(radius, d_element, [-half, half])
+        if c >= n:
+            best, hi = half, half
         else:
             lo = half
-        if gas is not None and len(gas.elem_pos) == n:
+        if c == n:
             break
-    if gas is None:
-        raise ValueError(f'cannot place {n} facets in radius {radius}')
+    half = best if best is not None else hi
+    gas = GratingArrayStructure(rowland=rowland, d_element=d_element, radius=radius,
+                                phi=[-half, half], elem_class=CATGrating,
+                                elem_args=dict(elem_args), id_col='facet')
     if len(gas.elem_pos) > n:
         gas.elem_pos = gas.elem_pos[:n]
         gas.elem_uncertainty = gas.elem_uncertainty[:n]
@@ -117,6 +127,29 @@ def cat_rowland_instrument(n_facets=200, R=5900., r=6000., blaze_deg=1.5,
     inst = Sequence(elements=[aper, lens, gas, ccd])
     return inst, {'aperture': aper, 'lens': lens, 'gratings': gas, 'ccds': ccd,
                   'rowland': rowland, 'efficiency': eff}
+
+
+def design_variants(n_variants=64, rng=None, R_range=(5000., 7000.), blaze_range=(1.0, 2.0),
+                    facet_range=(50, 800), n_ccd=16):
+    """Instrument design variants for a batched trade study: Rowland torus
+    R ~ U(R_range) mm with r = R + 100 mm, blaze ~ U(blaze_range) deg, facet
+    count ~ U{facet_range}; aperture and lens at x = 2 r + 500 mm (focus at
+    the origin) and the CCD row scaled with r so that it covers the same
+    dispersion angles as the r = 6000 mm layout.  Returns a list of
+    (Sequence, parts, parameters) triples."""
+    rng = np.random.default_rng() if rng is None else rng
+    out = []
+    for _ in range(n_variants):
+        R = float(rng.uniform(*R_range))
+        blaze = float(rng.uniform(*blaze_range))
+        n = int(rng.integers(facet_range[0], facet_range[1] + 1))
+        r = R + 100.
+        scale = r / 6000.
+        inst, parts = cat_rowland_instrument(n, R=R, r=r, blaze_deg=blaze, n_ccd=n_ccd,
+                                             ccd_y=(320. * scale, 720. * scale),
+                                             focal=2 * r + 500.)
+        out.append((inst, parts, {'R': R, 'r': r, 'blaze_deg': blaze, 'n_facets': n}))
+    return out
 
 
 def uniform_energies(u, lo=0.3, hi=2.0):

@@ -773,7 +773,84 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, i
     }
 }
 
+// Wave-reduced status counters of one tile, added to one variant's block.
+__device__ __forceinline__ void flush_counters(unsigned long long *st, const Counters &c) {
+    unsigned v[6] = {c.prob_range, c.order_empty, c.slot_reuse, c.grid_fallback, c.multi_hit, c.bad};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (v[k]) atomicAdd(st + k, (unsigned long long)v[k]);
+    }
+}
+
+// Batched persistent kernel: workgroups take tiles (variant, photon range)
+// from a device counter until the list is exhausted.  Thread 0 claims the
+// tile and finds its variant (binary search over tile_first); the LDS image
+// is reloaded only when the variant changes.  Every exit is uniform over the
+// workgroup (the tile index is read from LDS after a barrier).
+template <int PREC, int BS>
+__global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
+        const MxTables *__restrict__ tabs, const MxPhotons *__restrict__ phs,
+        const int64_t *__restrict__ tile_first, const int32_t *__restrict__ tile_photons,
+        int n_variants, long long n_tiles, int flags, unsigned long long *queue,
+        unsigned long long *status) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ long long s_tile;
+    __shared__ int s_var;
+    const bool use_grid = !(flags & MX_FLAG_NO_GRID);
+    int cur = -1;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const long long t = (long long)atomicAdd(queue, 1ULL);
+            int v = -1;
+            if (t < n_tiles) {
+                int lo = 0, hi = n_variants - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (tile_first[mid] <= t) lo = mid; else hi = mid - 1;
+                }
+                v = lo;
+            }
+            s_tile = t;
+            s_var = v;
+        }
+        __syncthreads();
+        const long long t = s_tile;
+        const int v = __builtin_amdgcn_readfirstlane(s_var);
+        if (v < 0) break;
+        // the structs are read through the constant address space: they are
+        // invariant for the whole launch, so the compiler may keep them in
+        // scalar registers or reload them with scalar loads
+        const MxTables &T = *(const MxTables *)((const __attribute__((address_space(4))) MxTables *)tabs + v);
+        const MxPhotons &P = *(const MxPhotons *)((const __attribute__((address_space(4))) MxPhotons *)phs + v);
+        if (v != cur) {
+            const int nb = PREC == 64 ? T.lds_bytes64 : T.lds_bytes32;
+            if (nb > 0) {
+                const uint4 *src = (const uint4 *)(PREC == 64 ? T.lds_image64 : T.lds_image32);
+                uint4 *dst = (uint4 *)lds;
+                const int nvec = (nb + 15) >> 4;
+                for (int k = threadIdx.x; k < nvec; k += BS) dst[k] = src[k];
+            }
+            cur = v;
+            __syncthreads();
+        }
+        const int64_t tp = tile_photons[v];
+        const int64_t b = (t - tile_first[v]) * tp;
+        int64_t e = b + tp;
+        if (e > P.n) e = P.n;
+        if (b < 0 || tp <= 0) e = b;
+        Counters cnt = {0, 0, 0, 0, 0, 0};
+        for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC>(T, P, i, use_grid, cnt, lds);
+        if (status) flush_counters(status + (size_t)v * MX_NSTATUS, cnt);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- C ABI
+extern "C" int mx_fill_i32(int32_t *dst, int64_t n, int32_t v, void *stream);
 static int check_tables(const MxTables *t) {
     if (!t) { set_error("tables is NULL"); return 1; }
     if (t->n_stages < 0 || t->n_stages > 4096) { set_error("n_stages out of range"); return 1; }
@@ -835,6 +912,72 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     if (err != hipSuccess) {
         char buf[256];
         snprintf(buf, sizeof(buf), "kernel launch failed: %s", hipGetErrorString(err));
+        set_error(buf);
+        return 2;
+    }
+    return 0;
+}
+
+extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t flags, void *stream) {
+    if (!b) { set_error("batch is NULL"); return 1; }
+    if (b->n_variants < 0 || b->n_tiles < 0) { set_error("negative variant or tile count"); return 1; }
+    if (precision != MX_PREC_FP64 && precision != MX_PREC_FP32) {
+        set_error("precision must be 64 or 32");
+        return 1;
+    }
+    if (b->n_variants == 0) return 0;
+    if (!b->tables || !b->photons || !b->tables_dev || !b->photons_dev || !b->tile_first ||
+        !b->tile_photons || !b->queue) {
+        set_error("a batch pointer is NULL");
+        return 1;
+    }
+    int lds = 0;
+    for (int v = 0; v < b->n_variants; ++v) {
+        if (check_tables(b->tables + v) || check_photons(b->photons + v)) {
+            char buf[512];
+            snprintf(buf, sizeof(buf), "variant %d: %s", v, g_last_error);
+            set_error(buf);
+            return 1;
+        }
+        const MxTables &t = b->tables[v];
+        const int l = precision == MX_PREC_FP64 ? t.lds_bytes64 : t.lds_bytes32;
+        if (l < 0 || l > 160 * 1024) { set_error("LDS request out of range"); return 1; }
+        if (l > 0 && !(precision == MX_PREC_FP64 ? t.lds_image64 : t.lds_image32)) {
+            set_error("LDS image pointer is NULL");
+            return 1;
+        }
+        if (l > lds) lds = l;
+    }
+    if (b->n_tiles == 0) return 0;
+    const int block = precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32;
+    int64_t grid = b->grid;
+    if (grid <= 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+        if (cus <= 0) cus = 256;
+        int per_cu = (flags >> 8) & 0xff;
+        if (per_cu <= 0) per_cu = 1;
+        grid = (int64_t)cus * per_cu;
+    }
+    if (grid > b->n_tiles) grid = b->n_tiles;
+    hipStream_t st = (hipStream_t)stream;
+    if (mx_fill_i32((int32_t *)b->queue, 2, 0, stream) != 0) {
+        set_error("queue reset failed");
+        return 2;
+    }
+    if (precision == MX_PREC_FP64)
+        hipLaunchKernelGGL((mx_trace_batched_kernel<64, MX_BLOCK64>), dim3((unsigned)grid), dim3(block), lds, st,
+                           b->tables_dev, b->photons_dev, b->tile_first, b->tile_photons, b->n_variants,
+                           (long long)b->n_tiles, (int)flags, b->queue, b->status);
+    else
+        hipLaunchKernelGGL((mx_trace_batched_kernel<32, MX_BLOCK32>), dim3((unsigned)grid), dim3(block), lds, st,
+                           b->tables_dev, b->photons_dev, b->tile_first, b->tile_photons, b->n_variants,
+                           (long long)b->n_tiles, (int)flags, b->queue, b->status);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "batched kernel launch failed: %s", hipGetErrorString(err));
         set_error(buf);
         return 2;
     }

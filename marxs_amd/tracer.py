@@ -165,8 +165,8 @@ class CompiledInstrument:
         self.device = _dev(device)
         self.dev_tables = DeviceTables(self.tables, self.device)
         self.grid = grid
-        # persistent-grid size per precision (0 = occupancy query); set by
-        # marxs_amd.autotune
+        # persistent-grid size per precision (workgroups per CU, 0 = library
+        # default); marxs_amd.autotune.tune_trace sets it from measurements
         self.wg_per_cu = {}
 
     @property
@@ -198,17 +198,39 @@ class CompiledInstrument:
         unchanged); results go to ``dp``.  Without it the trace is in place.
         """
         prec = PREC[precision]
+        if status is None:
+            status = torch.zeros(_lib.MX_NSTATUS, dtype=torch.int64, device=self.device)
+        P = self.photons_struct(dp, uniforms, src)
+        if dp.n == 0:
+            return status
+        tables = self.dev_tables.struct_for(existing)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        use_grid = self.grid if grid is None else grid
+        flags = 0 if use_grid else 1
+        wpc = self.wg_per_cu.get(prec, 0) if wg_per_cu is None else wg_per_cu
+        flags |= (int(wpc) & 0xff) << 8
+        _lib.check(_lib.lib.mx_trace(ctypes.byref(tables), ctypes.byref(P), prec, flags,
+                                     ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
+        dp.has_pos = True
+        return status
+
+    def photons_struct(self, dp, uniforms=None, src=None):
+        """Check the photon columns against the tables, allocate missing
+        output columns and return the MxPhotons struct (device pointers)."""
         t = self.tables
         has_pos = src.has_pos if src is not None else dp.has_pos
         if t.read_pos and not has_pos:
             raise ValueError("photons need a 'pos' column unless the first element is an aperture")
         if src is not None and (src.n != dp.n or src.device != dp.device):
             raise ValueError('src and dp must have the same length and device')
-        if status is None:
-            status = torch.zeros(_lib.MX_NSTATUS, dtype=torch.int64, device=self.device)
+        if dp.device != self.device:
+            raise ValueError('photons are on another device than the tables')
+        P = _lib.MxPhotons()
+        P.n = dp.n
         if dp.n == 0:
             self._alloc_outputs(dp)
-            return status
+            return P
         if t.n_uniform_slots > 0:
             if uniforms is None:
                 raise ValueError('uniforms required')
@@ -219,8 +241,6 @@ class CompiledInstrument:
                 raise ValueError(f'uniforms must be (>= {t.n_uniform_slots}, >= {dp.n}) '
                                  'row-major float64')
         self._alloc_outputs(dp)
-        P = _lib.MxPhotons()
-        P.n = dp.n
         P.read_pos = 1 if t.read_pos else 0
         P.n_uniform_slots = t.n_uniform_slots
         if t.n_uniform_slots > 0:
@@ -242,17 +262,7 @@ class CompiledInstrument:
             P.fcol[k] = dp.fcols[name].data_ptr()
         for k, name in enumerate(t.icols):
             P.icol[k] = dp.icols[name].data_ptr()
-        tables = self.dev_tables.struct_for(existing)
-        if stream is None:
-            stream = torch.cuda.current_stream(self.device).cuda_stream
-        use_grid = self.grid if grid is None else grid
-        flags = 0 if use_grid else 1
-        wpc = self.wg_per_cu.get(prec, 0) if wg_per_cu is None else wg_per_cu
-        flags |= (int(wpc) & 0xff) << 8
-        _lib.check(_lib.lib.mx_trace(ctypes.byref(tables), ctypes.byref(P), prec, flags,
-                                     ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
-        dp.has_pos = True
-        return status
+        return P
 
     def trace_device(self, dp, uniforms=None, precision=64, generator=None, existing=(),
                      grid=None):

@@ -42,7 +42,8 @@ def test_struct_layout_matches_header(lib, tmp_path):
     """sizeof/offsetof from the C compiler against the ctypes mirror."""
     fields = {'MxTables': [f[0] for f in lib.MxTables._fields_],
               'MxPhotons': [f[0] for f in lib.MxPhotons._fields_],
-              'MxAdjointArgs': [f[0] for f in lib.MxAdjointArgs._fields_]}
+              'MxAdjointArgs': [f[0] for f in lib.MxAdjointArgs._fields_],
+              'MxBatch': [f[0] for f in lib.MxBatch._fields_]}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "marxs_hip.h"',
            'int main(void) {']
     for st, fs in fields.items():
@@ -75,3 +76,19 @@ def test_argument_errors_need_no_gpu(lib):
     p.n = 0
     rc = lib.lib.mx_trace(ctypes.byref(t), ctypes.byref(p), 16, 0, None, None)
     assert rc != 0 and b'precision' in lib.lib.mx_last_error()
+
+
+def test_batched_argument_errors_need_no_gpu(lib):
+    b = lib.MxBatch()
+    assert lib.lib.mx_trace_batched(None, 64, 0, None) != 0
+    assert b'NULL' in lib.lib.mx_last_error()
+    b.n_variants = -1
+    assert lib.lib.mx_trace_batched(ctypes.byref(b), 64, 0, None) != 0
+    assert b'negative' in lib.lib.mx_last_error()
+    b.n_variants = 0
+    assert lib.lib.mx_trace_batched(ctypes.byref(b), 16, 0, None) != 0
+    assert b'precision' in lib.lib.mx_last_error()
+    assert lib.lib.mx_trace_batched(ctypes.byref(b), 32, 0, None) == 0
+    b.n_variants = 2
+    assert lib.lib.mx_trace_batched(ctypes.byref(b), 32, 0, None) != 0
+    assert b'NULL' in lib.lib.mx_last_error()

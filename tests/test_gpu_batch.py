@@ -1,0 +1,154 @@
+"""Batched trace of many instrument variants in one launch (needs an MI355X).
+
+Each variant's batched result must equal the single-variant trace of the
+same variant bit for bit in FP64 (and the oracle, through the single-trace
+parity), whatever the tile plan; hipGraph replays must repeat the eager
+result; the autotuner must leave a plan that still gives the same result."""
+import numpy as np
+import pytest
+
+from helpers import make_inputs, oracle_trace, compare, near_pixel_boundary
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [20000, 7001, 30000, 0, 12345]
+
+
+@pytest.fixture(scope='module')
+def study(gpu):
+    import torch
+    from marxs_amd import configs
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    variants = configs.design_variants(len(SIZES) - 1, np.random.default_rng(11),
+                                       facet_range=(50, 300))
+    trees = [v[0] for v in variants] + [configs.flat_grid_instrument()]
+    cis, inputs, unis, hosts = [], [], [], []
+    for k, (tree, n) in enumerate(zip(trees, SIZES)):
+        ci = CompiledInstrument(tree)
+        ph, u = make_inputs(n, 100 + k, ci.n_uniform_slots)
+        cis.append(ci)
+        inputs.append(DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'], device=gpu))
+        unis.append(torch.as_tensor(u, device=gpu))
+        hosts.append((tree, ph, u))
+    return cis, inputs, unis, hosts
+
+
+def _single(ci, src, u, precision):
+    from marxs_amd.batch import outputs_like
+    dp = outputs_like(src)
+    st = ci.launch(dp, u, precision, src=src)
+    return dp, st
+
+
+def _columns(dp):
+    out = {'pos': dp.pos, 'dir': dp.dir, 'pol': dp.pol, 'prob': dp.prob}
+    out.update(dp.fcols)
+    out.update(dp.icols)
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def _assert_identical(a, b):
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=a[k].dtype.kind == 'f'), k
+
+
+def test_batched_fp64_equals_single_variant_traces(study):
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = study
+    bt = BatchedTrace(cis, inputs, unis, precision=64)
+    st = bt.run()
+    for v, ci in enumerate(cis):
+        dp, sst = _single(ci, inputs[v], unis[v], 64)
+        _assert_identical(_columns(dp), _columns(bt.outputs[v]))
+        assert st[v]['multi_hit'] == int(sst.cpu()[4])
+    assert len(bt.outputs[3].prob) == 0
+
+
+def test_batched_variant_against_oracle(study):
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = study
+    bt = BatchedTrace(cis, inputs, unis, precision=64)
+    bt.run()
+    v = 1
+    tree, ph, u = hosts[v]
+    ref = oracle_trace(tree, ph, u)
+    got = _columns(bt.outputs[v])
+    got = {k: (x.T if x.ndim == 2 else x) for k, x in got.items()}
+    got = {k: (x.astype(np.int64) if x.dtype.kind == 'i' else x) for k, x in got.items()}
+    compare(ref, got, rtol=1e-9, near=near_pixel_boundary(ref))
+
+
+@pytest.mark.parametrize('plan', [
+    dict(tile_photons=512, order='given'),
+    dict(tile_photons=1, order='cost'),
+    dict(tile_photons=4096, scale_by_cost=True),
+    dict(tile_photons=[100, 2000, 512, 7, 999], order=[4, 2, 0, 1, 3], wg_per_cu=2),
+    dict(tile_photons=1024, lds_budget=8192),
+])
+def test_tile_plans_give_identical_results(study, plan):
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = study
+    ref = BatchedTrace(cis, inputs, unis, precision=64)
+    ref.run()
+    bt = BatchedTrace(cis, inputs, unis, precision=64)
+    bt.set_plan(**plan)
+    bt.run()
+    for v in range(len(cis)):
+        _assert_identical(_columns(ref.outputs[v]), _columns(bt.outputs[v]))
+
+
+def test_batched_fp32_matches_single_fp32(study):
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = study
+    bt = BatchedTrace(cis, inputs, unis, precision=32)
+    bt.run()
+    for v, ci in enumerate(cis):
+        dp, _ = _single(ci, inputs[v], unis[v], 32)
+        a, b = _columns(dp), _columns(bt.outputs[v])
+        for k in a:
+            if a[k].dtype.kind == 'f':
+                ok = ~np.isnan(a[k])
+                assert np.array_equal(np.isnan(a[k]), np.isnan(b[k])), k
+                assert np.allclose(a[k][ok], b[k][ok], rtol=0, atol=1e-6), k
+            else:
+                assert np.array_equal(a[k], b[k]), k
+
+
+def test_graph_replay_repeats_eager_result(study):
+    import torch
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = study
+    bt = BatchedTrace(cis, inputs, unis, precision=32)
+    bt.run()
+    eager = [_columns(o) for o in bt.outputs]
+    bt.capture()
+    for rep in range(3):
+        for o in bt.outputs:
+            o.prob.fill_(-1.0)
+            o.pos.fill_(float('nan'))
+        # unrelated eager work between replays
+        _single(cis[0], inputs[0], unis[0], 64)
+        torch.cuda.synchronize()
+        bt.replay()
+        torch.cuda.synchronize()
+        assert int(bt.raw_status()[:, 5].sum()) == 0
+        for v in range(len(cis)):
+            _assert_identical(eager[v], _columns(bt.outputs[v]))
+
+
+def test_autotuner_picks_a_plan_with_the_same_result(study):
+    from marxs_amd.batch import BatchedTrace
+    from marxs_amd.autotune import tune_batched, tune_trace
+    cis, inputs, unis, hosts = study
+    bt = BatchedTrace(cis, inputs, unis, precision=64)
+    bt.run()
+    ref = [_columns(o) for o in bt.outputs]
+    best, table = tune_batched(bt, reps=2)
+    assert best in [c for c, _ in table] and all(ms > 0 for _, ms in table)
+    bt.run()
+    for v in range(len(cis)):
+        _assert_identical(ref[v], _columns(bt.outputs[v]))
+    from marxs_amd.batch import outputs_like
+    dp = outputs_like(inputs[0])
+    wpc, res = tune_trace(cis[0], dp, unis[0], 64, candidates=(2, 8), reps=2, src=inputs[0])
+    assert cis[0].wg_per_cu[64] == wpc and set(res) == {2, 8}
