@@ -14,7 +14,23 @@ region; every step reads the same input columns and writes fresh outputs.
 kernel's rate, their ratio, the FP32 detector error against FP64 and the CPU
 oracle's rate on a bounded sample are reported beside it.
 
+Two more sections run after it (``--sections``):
+
+``adjoint``  200 CAT facets, 1e6 photons at 0.65 keV, facet poses perturbed by
+             N(0, 0.1 mm) and N(0, 30 arcsec); one step = loss (centroid + RMS
+             width of order -6 on the CCDs) and its gradient with respect to
+             200 x 6 pose parameters and 200 periods (FP32 adjoint kernel,
+             torch autograd over the pose chain).  Reports adjoints/s, the
+             agreement with central finite differences and with the FP64
+             adjoint.
+``batched``  64 design variants (Rowland R ~ U(5000, 7000) mm, blaze ~
+             U(1, 2) deg, 50-800 facets, own CCD rows), 2e5 photons each at
+             0.3-2.0 keV, traced in one autotuned, hipGraph-replayed launch,
+             against the same variants launched one after the other; the FP64
+             batched results are compared with the sequential FP64 traces.
+
     python bench.py [--steps K] [--warmup W] [--photons N] [--cpu-sample S]
+                    [--sections trace,adjoint,batched]
 """
 import argparse
 import json
@@ -24,6 +40,9 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+FP64_VALU_PEAK_TFLOPS = 78.6
+FLOPS_PER_TEST = 33   # ray-plane test: (c-p).n 8, d.n 5, divide 1, p+k d 6, q-c 3, two dots 10
 METRIC = 'photons/sec, 600-facet CAT array: FP32 vs FP64 GPU at stated µm error; adjoint/s'
 
 # algorithmic bytes per photon, FP64 columns, each touched once:
@@ -43,14 +62,30 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--photons', type=int, default=10_000_000)
     ap.add_argument('--facets', type=int, default=600)
-    ap.add_argument('--cpu-sample', type=int, default=200_000)
+    ap.add_argument('--cpu-sample', type=int, default=1_000_000)
     ap.add_argument('--seed', type=int, default=20261202)
     ap.add_argument('--grid', type=int, default=1)
     ap.add_argument('--only', choices=['fp32', 'fp64'], default=None,
                     help='profiling: time one precision only, skip comparisons')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='profiling: run one device copy of this size first (counter calibration)')
+    ap.add_argument('--sections', default='trace,adjoint,batched')
+    ap.add_argument('--cpu-procs', type=int, default=0,
+                    help='processes for the multi-process CPU baseline (0 = min(16, cpu count))')
+    ap.add_argument('--variants', type=int, default=64)
+    ap.add_argument('--variant-photons', type=int, default=200_000)
     return ap.parse_args()
+
+
+def _oracle_shard(job):
+    """One process of the multi-process CPU baseline: trace a photon shard
+    with the numpy oracle (imports no GPU library)."""
+    els, ph, us = job
+    os.environ['OMP_NUM_THREADS'] = '1'
+    from oracle import trace as OT
+    t = time.perf_counter()
+    OT.trace(els, ph, us)
+    return time.perf_counter() - t
 
 
 def log(*a):
@@ -140,6 +175,7 @@ def main():
         torch.cuda.synchronize()
         log(f'copy calibration: {a.numel() * 8} bytes read, {a.numel() * 8} bytes written')
         del a, b
+    sections = set(args.sections.split(','))
     if args.only:
         prec = 32 if args.only == 'fp32' else 64
         wall, k = timed(out32 if prec == 32 else out64, prec)
@@ -204,6 +240,43 @@ def main():
                          f'uniforms), numpy float64 element-major oracle, one thread, '
                          f'{tc:.1f} s'}
         log('CPU oracle', cpu)
+        # the same sample split over processes (photons are independent)
+        import multiprocessing as mp
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        edges = np.linspace(0, s, procs + 1).astype(int)
+        jobs = [(els, {k: v[a:b] for k, v in ph.items()}, us[:, a:b])
+                for a, b in zip(edges[:-1], edges[1:])]
+        t = time.perf_counter()
+        with mp.get_context('spawn').Pool(procs) as pool:
+            pool.map(_oracle_shard, jobs)
+        tp = time.perf_counter() - t
+        cpu_mp = {'value': s / tp, 'unit': 'photons/s', 'cores': procs, 'kind': 'port',
+                  'sample': f'the same {s} photons in {procs} spawned processes '
+                            f'(one contiguous shard each, one thread each), {tp:.1f} s '
+                            'including process start-up', 'cpu_count': os.cpu_count()}
+        log('CPU oracle, processes', cpu_mp)
+
+    # ---- the same trace through the host photon-table API (PCIe included)
+    host_rate = None
+    if 'hosttable' in sections or 'trace' in sections:
+        import marxs_amd as mx
+        m = min(n, 1_000_000)
+        tab = mx.PhotonTable({'dir': np.c_[src.dir[:, :m].cpu().numpy().T, np.zeros(m)],
+                              'polarization': np.c_[src.pol[:, :m].cpu().numpy().T, np.zeros(m)],
+                              'energy': src.energy[:m].cpu().numpy(),
+                              'probability': src.prob[:m].cpu().numpy()})
+        um = uni[:, :m].cpu().numpy()
+        ci.process_photons(tab.copy(), uniforms=um, precision=32)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ci.process_photons(tab.copy(), uniforms=um, precision=32)
+        torch.cuda.synchronize()
+        th = time.perf_counter() - t
+        host_rate = {'photons_per_s': m / th, 'photons': m, 'seconds': th,
+                     'what': 'CompiledInstrument.process_photons on a host PhotonTable: '
+                             'columns and uniforms copied to HBM, FP32 trace, all result '
+                             'columns copied back into the table'}
+        log('host table path', host_rate)
 
     pps32 = n / k32
     pps64 = n / k64
@@ -241,7 +314,196 @@ def main():
         'bytes_per_photon': BYTES_PER_PHOTON,
         'status': {'grid_fallback': int(st[3]), 'multi_hit': int(st[4])},
     }
+    # brute-force equivalent VALU fractions: 33 flops per photon-element test
+    res['valu_fraction_brute_force_equiv'] = {
+        'fp32': pps32 * n_elem * FLOPS_PER_TEST / (FP32_VALU_PEAK_TFLOPS * 1e12),
+        'fp64': pps64 * n_elem * FLOPS_PER_TEST / (FP64_VALU_PEAK_TFLOPS * 1e12),
+        'flops_per_test': FLOPS_PER_TEST, 'tests_per_photon': n_elem,
+        'peaks_tflops': {'fp32': FP32_VALU_PEAK_TFLOPS, 'fp64': FP64_VALU_PEAK_TFLOPS}}
+    if cpu is not None:
+        res['cpu_baseline_multiprocess'] = cpu_mp
+    if host_rate is not None:
+        res['host_table_path'] = host_rate
+    del out32, out64, src, uni, u, energy
+    torch.cuda.empty_cache()
+    if 'adjoint' in sections:
+        res['adjoint'] = bench_adjoint(args, dev)
+    if 'batched' in sections:
+        res['batched'] = bench_batched(args, dev)
     print(json.dumps(res), flush=True)
+
+
+def bench_adjoint(args, dev):
+    """Adjoint workload: 200-facet array, 1e6 photons at 0.65 keV."""
+    import numpy as np
+    import torch
+    import marxs_amd as mx
+    from marxs_amd import configs, tolerancing
+    from marxs_amd.adjoint import AdjointProblem, central_differences
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    t0 = time.time()
+    inst, parts = configs.cat_rowland_instrument(200)
+    tolerancing.wiggle(parts['gratings'], dx=0.1, dy=0.1, dz=0.1, rx=1.4544e-4, ry=1.4544e-4,
+                       rz=1.4544e-4, rng=np.random.default_rng(args.seed + 1))
+    n = 1_000_000
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 2)
+    ci = CompiledInstrument(inst, device=dev)
+    u = torch.rand((ci.n_uniform_slots + 1, n), dtype=torch.float64, device=dev, generator=g)
+    phi = 2 * np.pi * u[-1]
+    dirs = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    dirs[0] = -1.
+    pol = torch.stack([torch.zeros_like(phi), torch.sin(phi), torch.cos(phi)])
+    energy = torch.full((n,), 0.65, dtype=torch.float64, device=dev)
+    prob = torch.ones(n, dtype=torch.float64, device=dev)
+    uni = u[:ci.n_uniform_slots].contiguous()
+    full = DevicePhotons(dirs, pol, energy, prob, device=dev)
+    ci.trace_device(full, uniforms=uni)
+    front = CompiledInstrument(mx.Sequence(elements=[parts['aperture'], parts['lens']]), device=dev)
+    ent = DevicePhotons(dirs, pol, energy, prob, device=dev)
+    front.trace_device(ent, uniforms=uni[:2].contiguous())
+    sel = (full.icols['order'] == -6) & (full.icols['CCD'] >= 0)
+    weight = torch.where(sel, full.prob, torch.zeros_like(full.prob))
+    prob_ = AdjointProblem(parts['gratings'], parts['ccds'], ent.pos, ent.dir, ent.energy, weight,
+                           full.icols['facet'], full.icols['order'], full.icols['CCD'], coord=1)
+    dropped = prob_.restrict_to_traced(full.pos[1])
+    log(f'adjoint problem: {int(sel.sum())} order -6 photons on CCDs ({dropped} met two facets), '
+        f'set up in {time.time() - t0:.1f}s')
+
+    def step(prec):
+        th = prob_.theta0.clone().requires_grad_(True)
+        pe = prob_.period0.clone().requires_grad_(True)
+        L = prob_.loss(th, pe, precision=prec)
+        L.backward()
+        return L, th.grad, pe.grad
+
+    out = {}
+    for prec in (32, 64):
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            L, gt, gp = step(prec)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        out[prec] = (dt, torch.cat([gt, gp[:, None]], dim=1))
+    # kernel time of one backward launch (FP32), events on the launch stream
+    geom = prob_.geometry(prob_.theta0, prob_.period0).detach()
+    seed = torch.where(prob_.weight != 0, prob_.weight / prob_.weight.sum(), torch.zeros_like(prob_.weight))
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    prob_.backward_kernel(geom, seed, 32)
+    ev[0].record()
+    for _ in range(args.steps):
+        prob_.backward_kernel(geom, seed, 32)
+    ev[1].record()
+    torch.cuda.synchronize()
+    k_bw = ev[0].elapsed_time(ev[1]) * 1e-3 / args.steps
+    g32, g64 = out[32][1], out[64][1]
+    counts = torch.bincount(prob_.facet[prob_.weight != 0].long(), minlength=prob_.F)
+    top = torch.argsort(counts, descending=True)[:3].tolist()
+    fd = central_differences(prob_, prob_.theta0, prob_.period0, facets=top)
+    names = ['dx', 'dy', 'dz', 'rx', 'ry', 'rz', 'period']
+    fd_err = {}
+    f32_err = {}
+    for k, nm in enumerate(names):
+        scale = torch.abs(g64[:, k]).max()
+        fd_err[nm] = float((torch.abs(g64[top, k] - fd[top, k]) / scale).max())
+        f32_err[nm] = float((torch.abs(g32[:, k] - g64[:, k]) / scale).max())
+    res = {'workload': '200 CAT facets, 1e6 photons at 0.65 keV, poses N(0,0.1 mm)/N(0,30 arcsec), '
+                       'loss = centroid + RMS width of order -6 (y on the CCDs)',
+           'photons': n, 'selected_photons': int((prob_.weight != 0).sum()),
+           'adjoints_per_s_fp32': 1. / out[32][0], 'adjoints_per_s_fp64': 1. / out[64][0],
+           'ms_per_adjoint_fp32': out[32][0] * 1e3, 'ms_per_adjoint_fp64': out[64][0] * 1e3,
+           'backward_kernel_ms_fp32': k_bw * 1e3,
+           'photons_per_s_fp32': n / out[32][0],
+           'gradients': '200 x 6 poses + 200 periods',
+           'fd_rel_err_fp64_top3_facets': fd_err, 'fd_steps': '1e-4 mm, 1e-6 rad, 1e-10 mm',
+           'fp32_vs_fp64_rel_err': f32_err}
+    log('adjoint', res)
+    return res
+
+
+def bench_batched(args, dev):
+    """64 design variants traced in one launch vs one launch per variant."""
+    import numpy as np
+    import torch
+    from marxs_amd import configs
+    from marxs_amd.batch import BatchedTrace, outputs_like
+    from marxs_amd.autotune import tune_batched
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    t0 = time.time()
+    variants = configs.design_variants(args.variants, np.random.default_rng(args.seed + 3))
+    cis = [CompiledInstrument(v[0], device=dev) for v in variants]
+    log(f'{len(cis)} variants ({sum(v[2]["n_facets"] for v in variants)} facets) built and '
+        f'compiled in {time.time() - t0:.1f}s')
+    n = args.variant_photons
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 4)
+    inputs, unis = [], []
+    for ci in cis:
+        u = torch.rand((ci.n_uniform_slots + 2, n), dtype=torch.float64, device=dev, generator=g)
+        phi = 2 * np.pi * u[-1]
+        dirs = torch.zeros((3, n), dtype=torch.float64, device=dev)
+        dirs[0] = -1.
+        pol = torch.stack([torch.zeros_like(phi), torch.sin(phi), torch.cos(phi)])
+        energy = 0.3 + 1.7 * u[-2]
+        inputs.append(DevicePhotons(dirs, pol, energy, torch.ones(n, dtype=torch.float64, device=dev),
+                                    device=dev))
+        unis.append(u[:ci.n_uniform_slots].contiguous())
+    total = n * len(cis)
+    res = {'workload': f'{len(cis)} design variants (R ~ U(5000,7000) mm, blaze ~ U(1,2) deg, '
+                       f'50-800 facets, 16 CCDs each), {n} photons each, 0.3-2.0 keV',
+           'variants': len(cis), 'photons': total,
+           'facets': [v[2]['n_facets'] for v in variants]}
+    for prec in (32, 64):
+        bt = BatchedTrace(cis, inputs, unis, precision=prec)
+        best, table = tune_batched(bt)
+        bt.run()
+        bt.capture()
+        for _ in range(args.warmup):
+            bt.replay()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            bt.replay()
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t) / args.steps
+        bt.run(use_graph=True)
+        # one launch per variant, same tables and photons
+        seq_out = [outputs_like(src) for src in inputs]
+
+        def seq():
+            for ci, src, u, o in zip(cis, inputs, unis, seq_out):
+                ci.launch(o, u, prec, src=src)
+        for _ in range(args.warmup):
+            seq()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            seq()
+        torch.cuda.synchronize()
+        ts = (time.perf_counter() - t) / args.steps
+        r = {'photons_per_s_batched': total / tb, 'ms_per_batch': tb * 1e3,
+             'photons_per_s_sequential': total / ts, 'ms_sequential': ts * 1e3,
+             'speedup_vs_sequential': ts / tb,
+             'plan': {k: (v if k != 'tile_photons' else v) for k, v in best.items()},
+             'autotune_ms': {f"{c['tile_photons']}/{'scaled' if c['scale_by_cost'] else 'flat'}/"
+                             f"{c['order'] if isinstance(c['order'], str) else 'perm'}/"
+                             f"wg{c['wg_per_cu']}": ms for c, ms in table}}
+        if prec == 64:
+            ident = True
+            for o, q in zip(bt.outputs, seq_out):
+                for a, b in [(o.pos, q.pos), (o.dir, q.dir), (o.pol, q.pol), (o.prob, q.prob)] + \
+                        [(o.fcols[k], q.fcols[k]) for k in o.fcols] + \
+                        [(o.icols[k], q.icols[k]) for k in o.icols]:
+                    if not torch.equal(torch.nan_to_num(a, nan=12345.), torch.nan_to_num(b, nan=12345.)):
+                        ident = False
+            r['fp64_batched_equals_sequential_bitwise'] = ident
+        res[f'fp{prec}'] = r
+        log(f'batched FP{prec}', r)
+        del bt, seq_out
+    return res
 
 
 if __name__ == '__main__':

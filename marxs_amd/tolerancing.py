@@ -15,7 +15,8 @@ import numpy as np
 
 from .mathutils import compose, euler2mat_sxyz
 
-__all__ = ['generate_facet_uncertainty', 'wiggle', 'varyperiod', 'run_tolerances']
+__all__ = ['generate_facet_uncertainty', 'wiggle', 'varyperiod', 'run_tolerances',
+           'run_tolerances_batched']
 
 
 def generate_facet_uncertainty(n, xyz, angle_xyz, rng=None, trans_offset=0, rot_offset=0):
@@ -51,6 +52,54 @@ def run_tolerances(photons_in, instrum, wigglefunc, wiggleparts, parameters, ana
     for pars in parameters:
         wigglefunc(wiggleparts, **pars)
         photons = instrum(photons_in.copy())
+        cpars = copy(pars)
+        cpars.update(analyzefunc(photons))
+        out.append(cpars)
+    return out
+
+
+def run_tolerances_batched(photons_in, instrum, wigglefunc, wiggleparts, parameters, analyzefunc,
+                           precision=64, uniforms=None, generator=None, tune=False):
+    """``run_tolerances`` with every variant traced in one batched launch.
+
+    Each parameter set modifies the instrument as in the sequential loop and
+    the modified tree is compiled at once (a snapshot of its poses and
+    periods); then all variants are traced together (``mx_trace_batched``)
+    and each variant's photon table is analysed in turn.  ``uniforms`` (slots,
+    N) fixes the random draws (the same for every variant: common random
+    numbers); otherwise every variant draws its own.  ``tune`` runs the tile
+    autotuner first.  Photons must not hold output columns of the
+    instrument yet.
+    """
+    import torch
+    from .tracer import CompiledInstrument, DevicePhotons, write_back
+    from .batch import BatchedTrace
+    cis = []
+    for pars in parameters:
+        wigglefunc(wiggleparts, **pars)
+        cis.append(CompiledInstrument(instrum))
+    if not cis:
+        return []
+    clash = [c for c in photons_in.colnames
+             if c in cis[0].tables.fcols or c in cis[0].tables.icols]
+    if clash:
+        raise ValueError(f'photons already hold output columns {clash}')
+    src = DevicePhotons.from_table(photons_in, device=cis[0].device)
+    if uniforms is not None:
+        u = torch.as_tensor(np.ascontiguousarray(uniforms), dtype=torch.float64,
+                            device=cis[0].device)
+        unis = [u] * len(cis)
+    else:
+        unis = [ci.draw_uniforms(src.n, generator) for ci in cis]
+    bt = BatchedTrace(cis, [src] * len(cis), unis, precision=precision)
+    if tune:
+        from .autotune import tune_batched
+        tune_batched(bt)
+    bt.run()
+    out = []
+    for pars, ci, dp in zip(parameters, cis, bt.outputs):
+        photons = photons_in.copy()
+        write_back(dp, photons, ci.tables)
         cpars = copy(pars)
         cpars.update(analyzefunc(photons))
         out.append(cpars)

@@ -152,3 +152,37 @@ def test_autotuner_picks_a_plan_with_the_same_result(study):
     dp = outputs_like(inputs[0])
     wpc, res = tune_trace(cis[0], dp, unis[0], 64, candidates=(2, 8), reps=2, src=inputs[0])
     assert cis[0].wg_per_cu[64] == wpc and set(res) == {2, 8}
+
+
+def test_run_tolerances_batched_matches_sequential_loop(gpu):
+    """run_tolerances semantics (marxs/design/tolerancing.py:224-283): the
+    batched run gives the per-variant analysis of the sequential loop."""
+    import marxs_amd as mx
+    from marxs_amd import configs, tolerancing
+    from marxs_amd.tracer import CompiledInstrument
+    inst, parts = configs.cat_rowland_instrument(120)
+    n = 20000
+    ph, _ = make_inputs(n, 7, 0)
+    table = mx.PhotonTable({'dir': np.c_[ph['dir'], np.zeros(n)],
+                            'polarization': np.c_[ph['pol'], np.zeros(n)],
+                            'energy': ph['energy'], 'probability': ph['prob']})
+    u = np.random.default_rng(8).random((CompiledInstrument(inst).n_uniform_slots, n))
+
+    def analyse(p):
+        sel = (np.asarray(p['order']) == -6) & (np.asarray(p['CCD']) >= 0)
+        return {'n': int(sel.sum()), 'y': float(np.mean(np.asarray(p['pos'])[sel, 1]))}
+
+    seq = []
+    params = [{'dx': s, 'rx': 1e-4 * s, 'seed': k} for k, s in enumerate([0., 0.2, 1.0])]
+
+    def wig(e, dx, rx, seed):
+        tolerancing.wiggle(e, dx=dx, rx=rx, rng=np.random.default_rng(seed))
+
+    for pars in params:
+        wig(parts['gratings'], **pars)
+        out = CompiledInstrument(inst)(table.copy(), uniforms=u)
+        seq.append({**pars, **analyse(out)})
+    bat = tolerancing.run_tolerances_batched(table, inst, wig, parts['gratings'], params, analyse,
+                                             uniforms=u)
+    assert seq == bat
+    assert seq[0]['n'] > 100 and seq[0]['y'] != seq[2]['y']
