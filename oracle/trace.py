@@ -10,7 +10,8 @@ out component by component in a fixed left-to-right order (the reference's
 ``np.dot``/``einsum`` leave the summation order to BLAS); numpy never fuses
 multiply-adds, so these are plain IEEE double operations.
 
-How each part is pinned (tests/test_oracle_*.py):
+How each part is pinned (tests/test_oracle.py, tests/test_oracle_driver.py,
+tests/test_setup.py; committed vectors in tests/golden):
 
 * ``intersect`` -- known answers of ``marxs/math/tests/test_geometry.py:11-55``
   and a brute-force check against the closed-form ray-plane solution.

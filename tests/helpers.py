@@ -95,3 +95,68 @@ def make_inputs(n, seed, n_slots, energies='uniform'):
         e = configs.line_spectrum_energies(u[0], u[-1])
     ph = configs.onaxis_photons(e, u[1])
     return ph, np.ascontiguousarray(u[2:2 + n_slots])
+
+
+# ------------------------------------------------------------ golden vectors
+GOLDEN = __import__('os').path.join(__import__('os').path.dirname(__file__), 'golden')
+
+
+def encode_description(els):
+    """Instrument description (oracle.describe output) as JSON-able data;
+    identical selector tables are stored once."""
+    sels, out = [], []
+
+    def arr(a):
+        a = np.asarray(a)
+        return {'__array__': a.tolist(), 'dtype': str(a.dtype)}
+
+    for el in els:
+        d = {}
+        for k, v in el.items():
+            if k == 'selector':
+                enc = {kk: (arr(vv) if isinstance(vv, np.ndarray) else vv) for kk, vv in v.items()}
+                if enc not in sels:
+                    sels.append(enc)
+                d[k] = {'__selector__': sels.index(enc)}
+            elif isinstance(v, np.ndarray):
+                d[k] = arr(v)
+            elif isinstance(v, (np.integer,)):
+                d[k] = int(v)
+            elif isinstance(v, (np.floating,)):
+                d[k] = float(v)
+            elif isinstance(v, (np.bool_,)):
+                d[k] = bool(v)
+            else:
+                d[k] = v
+        out.append(d)
+    return {'selectors': sels, 'elements': out}
+
+
+def decode_description(obj):
+    def dec(v):
+        if isinstance(v, dict) and '__array__' in v:
+            return np.array(v['__array__'], dtype=v['dtype'])
+        return v
+    sels = [{k: dec(v) for k, v in s.items()} for s in obj['selectors']]
+    els = []
+    for d in obj['elements']:
+        e = {}
+        for k, v in d.items():
+            if isinstance(v, dict) and '__selector__' in v:
+                e[k] = dict(sels[v['__selector__']])
+            else:
+                e[k] = dec(v)
+        els.append(e)
+    return els
+
+
+def load_golden(name):
+    """(description, photons, uniforms, outputs) of a committed case."""
+    import json
+    import os
+    with open(os.path.join(GOLDEN, name + '.json')) as f:
+        meta = json.load(f)
+    z = np.load(os.path.join(GOLDEN, name + '.npz'), allow_pickle=False)
+    ph = {k[3:]: z[k] for k in z.files if k.startswith('in_')}
+    out = {k[4:]: z[k] for k in z.files if k.startswith('out_')}
+    return meta, decode_description(meta['description']), ph, z['uniforms'], out

@@ -237,3 +237,15 @@ def test_cat_order_convention_table(gpu):
     m5 = mx.CATGrating(d=1. / 5000, order_selector=mx.OrderSelector([-5]), zoom=2)(mk())
     assert np.all(p5['dir'][1:3, 1] > 0) and np.all(p5['dir'][3:, 1] < 0)
     assert np.all(m5['dir'][1:3, 1] < 0) and np.all(m5['dir'][3:, 1] > 0)
+
+
+@pytest.mark.parametrize('name', ['flat_grid_1000', 'cat_rowland_200_2000'])
+def test_golden_vectors(gpu, name):
+    """The FP64 HIP path against the committed oracle vectors."""
+    from helpers import load_golden
+    from marxs_amd import configs
+    meta, els, ph, u, out = load_golden(name)
+    inst = getattr(configs, meta['builder'])(**meta['kwargs'])
+    inst = inst[0] if isinstance(inst, tuple) else inst
+    got, st = gpu_trace(inst, ph, u, precision=64)
+    compare(out, got, rtol=1e-9, near=near_pixel_boundary(out))
