@@ -33,14 +33,14 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 10
+#define MX_ABI_VERSION 11
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
 #define MX_NSTATUS 8
 
 /* record widths of the instrument tables */
-#define MX_SI 32   /* int32 per stage   */
+#define MX_SI 48   /* int32 per stage   */
 #define MX_SD 32   /* double per stage  */
 #define MX_ET 16   /* double per element, intersection data */
 #define MX_EP 16   /* double per element, interaction parameters */

@@ -53,7 +53,7 @@ def tune_trace(ci, dp, uniforms, precision=32, candidates=(2, 4, 8, 16), reps=3,
 
 
 def _batched_candidates(bt):
-    bs = 768 if bt.precision == 32 else 512
+    bs = 1024 if bt.precision == 32 else 768
     out = []
     for mult in (1, 2, 4, 8):
         for scaled in (False, True):

@@ -31,7 +31,7 @@ from .tracer import (CompiledInstrument, DevicePhotons, DeviceTables, PREC, rais
 
 __all__ = ['BatchedTrace', 'variant_cost', 'outputs_like']
 
-BLOCK = {32: 768, 64: 512}
+BLOCK = {32: 1024, 64: 768}
 
 
 def outputs_like(src):

@@ -35,7 +35,7 @@ STAGE_LENS_THIN = 4
 STAGE_GRATING = 5
 STAGE_DETECTOR = 6
 
-SI, SD, ET, EP, EI, SELI = 32, 32, 16, 16, 8, 12
+SI, SD, ET, EP, EI, SELI = 48, 32, 16, 16, 8, 12
 ORDER_NONE = -2 ** 31
 
 # output column fields of a stage record (stage_i[10:19]); bit = index
@@ -191,18 +191,58 @@ def _orthonormal_basis(a):
     return a, b1, b2
 
 
-def build_grid(frames, eps_abs=1e-3, eps_rel=1e-6, theta_max=0.35, cell_frac=0.5,
-               max_cells=512):
-    """Culling grid for an array stage.
+def _cell_lists(G1, G2, U0, W0, cu, cw, delta, umin, umax, wmin, wmax):
+    starts = [0]
+    items = []
+    for iu in range(G1):
+        lo_u = U0 + iu * cu - delta
+        hi_u = U0 + (iu + 1) * cu + delta
+        ok_u = (umax >= lo_u) & (umin <= hi_u)
+        for iw in range(G2):
+            lo_w = W0 + iw * cw - delta
+            hi_w = W0 + (iw + 1) * cw + delta
+            ok = ok_u & (wmax >= lo_w) & (wmin <= hi_w)
+            items.extend(np.nonzero(ok)[0].tolist())
+            starts.append(len(items))
+    return np.array(starts, dtype=np.int32), np.array(items, dtype=np.int32)
+
+
+def _successor_lists(tt, eps, s, umin, umax, wmin, wmax):
+    """After a hit on element j, a ray within atan(tt) of the axis can only
+    reach elements k > j whose box overlaps j's box grown by (largest depth
+    difference of their corners) * tt."""
+    n = len(umin)
+    slo, shi = s.min(axis=1), s.max(axis=1)
+    sstart = [0]
+    sitems = []
+    for j in range(n):
+        ks = np.arange(j + 1, n)
+        if len(ks):
+            depth = np.maximum(np.abs(shi[j] - slo[ks]), np.abs(shi[ks] - slo[j])) + 2 * eps
+            grow = depth * tt + 2 * eps
+            ok = ((umax[ks] >= umin[j] - grow) & (umin[ks] <= umax[j] + grow) &
+                  (wmax[ks] >= wmin[j] - grow) & (wmin[ks] <= wmax[j] + grow))
+            sitems.extend(ks[ok].tolist())
+        sstart.append(len(sitems))
+    return np.array(sstart, dtype=np.int32), np.array(sitems, dtype=np.int32)
+
+
+def build_grid(frames, eps_abs=1e-3, eps_rel=1e-6, theta_max=0.35, theta_tight=0.1,
+               cell_frac=0.5, max_cells=512):
+    """Two-tier culling grid for an array stage.
 
     ``frames``: list of :func:`plane_frame` dicts.  The stage axis ``a`` is
     the mean element normal; elements are projected onto the plane
-    perpendicular to ``a``.  Every element whose projected bounding box, grown
-    by ``eps``, overlaps a cell grown by ``delta`` is listed (ascending) for
-    that cell.  ``delta`` is half the slab thickness times
-    ``tan(theta_max)`` plus ``eps``: rays within ``theta_max`` of ``a`` have a
-    projected segment no longer than ``2 delta`` and are served from one
-    cell; the kernel tests all elements for more oblique rays.
+    perpendicular to ``a``.  For a cell, the tight list holds every element
+    whose projected bounding box, grown by ``eps``, overlaps the cell grown by
+    ``delta_t``, the loose list the same for ``delta_l``; ``delta`` is half the
+    slab thickness times ``tan(theta)`` plus ``eps`` (``theta_tight``,
+    ``theta_max``).  A ray whose segment through the slab projects into a
+    box no wider than ``2 delta_t`` (``2 delta_l``) is served by the tight
+    (loose) list of the cell holding the box centre; the kernel tests all
+    elements for more oblique rays.  Successor lists (the elements a ray can
+    reach after a hit) exist for both tiers.  Both tiers share one cell
+    layout; the tight lists are the ones staged in LDS.
     """
     n = len(frames)
     centers = np.array([f['center'] for f in frames])
@@ -228,7 +268,9 @@ def build_grid(frames, eps_abs=1e-3, eps_rel=1e-6, theta_max=0.35, cell_frac=0.5
     s_lo, s_hi = s.min() - eps, s.max() + eps
     umin, umax = u.min(axis=1) - eps, u.max(axis=1) + eps
     wmin, wmax = w.min(axis=1) - eps, w.max(axis=1) + eps
-    delta = eps + 0.5 * (s_hi - s_lo) * math.tan(theta_max)
+    theta_tight = min(theta_tight, theta_max)
+    delta_l = eps + 0.5 * (s_hi - s_lo) * math.tan(theta_max)
+    delta_t = eps + 0.5 * (s_hi - s_lo) * math.tan(theta_tight)
     U0, U1 = umin.min(), umax.max()
     W0, W1 = wmin.min(), wmax.max()
     size = np.median(np.maximum(umax - umin, wmax - wmin))
@@ -237,42 +279,18 @@ def build_grid(frames, eps_abs=1e-3, eps_rel=1e-6, theta_max=0.35, cell_frac=0.5
     G2 = int(min(max(1, math.ceil((W1 - W0) / cs)), max_cells))
     cu = (U1 - U0) / G1 if U1 > U0 else 1.0
     cw = (W1 - W0) / G2 if W1 > W0 else 1.0
-    starts = [0]
-    items = []
-    for iu in range(G1):
-        lo_u = U0 + iu * cu - delta
-        hi_u = U0 + (iu + 1) * cu + delta
-        ok_u = (umax >= lo_u) & (umin <= hi_u)
-        for iw in range(G2):
-            lo_w = W0 + iw * cw - delta
-            hi_w = W0 + (iw + 1) * cw + delta
-            ok = ok_u & (wmax >= lo_w) & (wmin <= hi_w)
-            idx = np.nonzero(ok)[0]
-            items.extend(idx.tolist())
-            starts.append(len(items))
-    # successor lists: after a hit on element j, a ray within theta_max of
-    # the axis can only reach elements k > j whose box overlaps j's box grown
-    # by (largest depth difference of their corners) * tan(theta_max)
-    tt = math.tan(theta_max)
-    slo, shi = s.min(axis=1), s.max(axis=1)
-    sstart = [0]
-    sitems = []
-    for j in range(n):
-        ks = np.arange(j + 1, n)
-        if len(ks):
-            depth = np.maximum(np.abs(shi[j] - slo[ks]), np.abs(shi[ks] - slo[j])) + 2 * eps
-            grow = depth * tt + 2 * eps
-            ok = ((umax[ks] >= umin[j] - grow) & (umin[ks] <= umax[j] + grow) &
-                  (wmax[ks] >= wmin[j] - grow) & (wmin[ks] <= wmax[j] + grow))
-            sitems.extend(ks[ok].tolist())
-        sstart.append(len(sitems))
+    starts_t, items_t = _cell_lists(G1, G2, U0, W0, cu, cw, delta_t, umin, umax, wmin, wmax)
+    starts_l, items_l = _cell_lists(G1, G2, U0, W0, cu, cw, delta_l, umin, umax, wmin, wmax)
+    tt, tl = math.tan(theta_tight), math.tan(theta_max)
+    sstart_t, sitems_t = _successor_lists(tt, eps, s, umin, umax, wmin, wmax)
+    sstart_l, sitems_l = _successor_lists(tl, eps, s, umin, umax, wmin, wmax)
     return {'origin': origin, 'a': a, 'b1': b1, 'b2': b2,
             's_lo': s_lo, 's_hi': s_hi, 'U0': U0, 'W0': W0, 'U1': U1, 'W1': W1,
-            'inv_cu': 1.0 / cu, 'inv_cw': 1.0 / cw, 'delta': delta,
-            'G1': G1, 'G2': G2, 'cell_start': np.array(starts, dtype=np.int32),
-            'items': np.array(items, dtype=np.int32), 'eps': eps, 'tan_theta': tt,
-            'succ_start': np.array(sstart, dtype=np.int32),
-            'succ_items': np.array(sitems, dtype=np.int32)}
+            'inv_cu': 1.0 / cu, 'inv_cw': 1.0 / cw, 'G1': G1, 'G2': G2, 'eps': eps,
+            'delta': delta_t, 'tan_theta': tt, 'cell_start': starts_t, 'items': items_t,
+            'succ_start': sstart_t, 'succ_items': sitems_t,
+            'delta_l': delta_l, 'tan_l': tl, 'cell_start_l': starts_l, 'items_l': items_l,
+            'succ_start_l': sstart_l, 'succ_items_l': sitems_l}
 
 
 def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
@@ -356,10 +374,21 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
             stage_d[s, 19] = g['W1']
             stage_d[s, 20] = g['delta']
             stage_d[s, 21] = g['tan_theta']
+            stage_d[s, 22] = g['delta_l']
+            stage_d[s, 23] = g['tan_l']
             rec[28] = sum(len(c) for c in succ_start)
             rec[29] = sum(len(c) for c in succ_items)
             succ_start.append(g['succ_start'])
             succ_items.append(g['succ_items'])
+            # loose tier (global memory only)
+            rec[32] = sum(len(c) for c in cell_start)
+            rec[33] = sum(len(c) for c in cell_items)
+            cell_start.append(g['cell_start_l'] + rec[33])
+            cell_items.append(g['items_l'])
+            rec[34] = sum(len(c) for c in succ_start)
+            rec[35] = sum(len(c) for c in succ_items)
+            succ_start.append(g['succ_start_l'])
+            succ_items.append(g['succ_items_l'])
         else:
             # FP32 ray advance needs an axis even without a grid
             stage_d[s, 3:6] = frames[0]['e_x']
@@ -386,13 +415,15 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
         rec[19] = 0
         rec[20:28] = -1     # LDS offsets, set by _build_lds
         rec[30:32] = -1
+        rec[36:42] = -1
         if g is None:
             rec[28:30] = -1
+            rec[32:36] = -1
     if len(fcols) > 16 or len(icols) > 8:
         raise CompileError('too many distinct output columns')
     tables = CompiledTables(
         stages=stages, stage_i=stage_i, stage_d=stage_d,
-        stage_f=np.concatenate([stage_d[:, 3:22], np.zeros((len(stages), SD - 19))],
+        stage_f=np.concatenate([stage_d[:, 3:24], np.zeros((len(stages), SD - 21))],
                                axis=1).astype(np.float32),
         elem_t=elem_t, elem_p=elem_p, elem_i=elem_i,
         elem_t32=elem_t32, elem_p32=elem_p32,
@@ -467,6 +498,7 @@ def _build_lds(tables, budget32=None, budget64=None):
     si = tables.stage_i
     si[:, 20:28] = -1
     si[:, 30:32] = -1
+    si[:, 36:42] = -1
     images = {}
     for prec, budget in ((32, budget32), (64, budget64)):
         parts = []
@@ -507,13 +539,14 @@ def _build_lds(tables, budget32=None, budget64=None):
             recs = (tables.elem_t32 if prec == 32 else tables.elem_t)[st.elem0:st.elem0 + n]
             params = _param_records32(tables, st) if (prec == 32 and st.kind == STAGE_GRATING) else None
             g = st.grid
-            if g is not None and (len(g['items']) >= 65536 or n >= 65536 or
-                                  len(g['succ_items']) >= 65536):
+            if g is not None and (max(len(g['items']), len(g['items_l']), n,
+                                      len(g['succ_items']), len(g['succ_items_l'])) >= 65535):
                 g = None
             size = _align(recs.nbytes) + (_align(params.nbytes) if params is not None else 0)
             if g is not None:
-                size += _align(2 * len(g['cell_start'])) + _align(2 * len(g['items']))
-                size += _align(2 * len(g['succ_start'])) + _align(2 * len(g['succ_items']) + 2)
+                for t in ('', '_l'):
+                    size += _align(2 * len(g['cell_start' + t])) + _align(2 * len(g['items' + t]))
+                    size += _align(2 * len(g['succ_start' + t])) + _align(2 * len(g['succ_items' + t]) + 2)
             if off + size > budget or (st.grid is not None and g is None):
                 continue
             c_rec = 20 if prec == 32 else 21
@@ -521,17 +554,20 @@ def _build_lds(tables, budget32=None, budget64=None):
             if params is not None:
                 si[s, 22] = add(params)
             if g is not None:
-                c0, c1 = (24, 25) if prec == 32 else (26, 27)
-                si[s, c0] = add(g['cell_start'].astype(np.uint16))
-                si[s, c1] = add(g['items'].astype(np.uint16))
-                # successor block: (n + 1) uint16 offsets, padded to 16 bytes,
-                # then the uint16 element indices
-                blk = np.zeros(_align(2 * len(g['succ_start'])) // 2 + len(g['succ_items']) + 1,
-                               dtype=np.uint16)
-                blk[:len(g['succ_start'])] = g['succ_start']
-                o0 = _align(2 * len(g['succ_start'])) // 2
-                blk[o0:o0 + len(g['succ_items'])] = g['succ_items']
-                si[s, 30 if prec == 32 else 31] = add(blk)
+                # both tiers: cell offsets and items (uint16), and successor
+                # blocks of (n + 1) uint16 offsets, padded to 16 bytes, then
+                # the uint16 element indices
+                cols = {'': ((24, 25, 30) if prec == 32 else (26, 27, 31)),
+                        '_l': ((36, 37, 38) if prec == 32 else (39, 40, 41))}
+                for t, (c0, c1, c2) in cols.items():
+                    si[s, c0] = add(g['cell_start' + t].astype(np.uint16))
+                    si[s, c1] = add(g['items' + t].astype(np.uint16))
+                    ss, sit = g['succ_start' + t], g['succ_items' + t]
+                    blk = np.zeros(_align(2 * len(ss)) // 2 + len(sit) + 1, dtype=np.uint16)
+                    blk[:len(ss)] = ss
+                    o0 = _align(2 * len(ss)) // 2
+                    blk[o0:o0 + len(sit)] = sit
+                    si[s, c2] = add(blk)
         img = np.zeros(_align(max(off, 16)), dtype=np.uint8)
         for o, b in parts:
             img[o:o + b.nbytes] = b
@@ -605,5 +641,6 @@ def _element_params(kind, x, f, ep, ei, sel_index, sel_rows, sel_d, sel_o):
         ep[6] = x.pixsize
         ep[7] = x.centerpix[0]
         ep[8] = x.centerpix[1]
+        ep[9] = 1.0 / x.pixsize
         ei[4] = x.npix[0]
         ei[5] = x.npix[1]

@@ -40,12 +40,12 @@
 #define MX_E2W_F 1.2398419292004202e-06f
 #define MX_ORDER_NONE (-2147483647 - 1)
 #ifndef MX_BLOCK32
-#define MX_BLOCK32 768          /* one workgroup per CU holds the LDS image   */
+#define MX_BLOCK32 1024         /* one workgroup per CU holds the LDS image   */
 #endif
 #ifndef MX_BLOCK64
-#define MX_BLOCK64 512
+#define MX_BLOCK64 768
 #endif
-#define MX_DEFAULT_WG_PER_CU 8  /* persistent grid: workgroups per CU         */
+#define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
 
 static thread_local char g_last_error[512] = "";
 
@@ -86,6 +86,24 @@ __device__ __forceinline__ bool plane_test64(const double *__restrict__ et, d3 p
     ly = dot3(v, ey);
     lz = dot3(v, ez);
     return (kd != 0.0) && (k >= 0.0) && (fabs(ly) <= et[12]) && (fabs(lz) <= et[13]);
+}
+
+// The same test for the FP32 path's final detector coordinate: the division
+// is a float reciprocal refined by one Newton step in double (relative error
+// ~1e-14, far below the FP32 path's error budget).
+__device__ __forceinline__ void plane_hit64_fast(const double *__restrict__ et, d3 p, d3 d,
+                                                 d3 &q, double &ly, double &lz) {
+    d3 c = ld3(et), ex = ld3(et + 3), ey = ld3(et + 6), ez = ld3(et + 9);
+    double kn = dot3(sub3(c, p), ex);
+    double kd = dot3(d, ex);
+    double r = (double)__builtin_amdgcn_rcpf((float)kd);
+    r = r * (2.0 - kd * r);
+    r = r * (2.0 - kd * r);
+    double k = kn * r;
+    q = mk(p.x + k * d.x, p.y + k * d.y, p.z + k * d.z);
+    d3 v = sub3(q, c);
+    ly = dot3(v, ey);
+    lz = dot3(v, ez);
 }
 
 // parallel_transport / paralleltransport_matrix with J = I,
@@ -173,9 +191,18 @@ __device__ __forceinline__ f3 ptrans32(f3 d1, f3 d2, f3 pol) {
 }
 
 // --------------------------------------------------------------- shared
+// Status counters of a workgroup, in LDS (indices MX_ST_*).  An event is
+// counted once per wave: the first active lane adds the number of active
+// lanes, so the counters take no vector registers.
 struct Counters {
-    unsigned prob_range, order_empty, slot_reuse, grid_fallback, multi_hit, bad;
+    unsigned *c;
 };
+#define MX_NCOUNT 6
+__device__ __forceinline__ void count(const Counters &cnt, int k) {
+    const unsigned long long m = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
+        atomicAdd(cnt.c + k, (unsigned)__popcll(m));
+}
 
 // Order selection (OrderSelector grating.py:51-57, EfficiencyFile 90-96):
 // row = nearest energy (first on ties); order index = first j with cdf > u.
@@ -243,12 +270,14 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
 
 // Candidate range of an array stage for one ray r + k d, k >= kmin, with r
 // relative to the stage origin.  Returns 0: no element can be hit, 1: test
-// the cell list [b, e), 2: test all elements.  g: a, b1, b2, S_lo, S_hi, U0,
-// W0, 1/cu, 1/cw, U1, W1, delta (stage_f / stage_d layout, see compiler).
-// cs: the stage's cell offsets (cell c lists items [cs[c], cs[c+1])).
+// the tight cell list [b, e), 4: the loose cell list [b, e), 2: test all
+// elements.  g: a, b1, b2, S_lo, S_hi, U0, W0, 1/cu, 1/cw, U1, W1, delta_t,
+// tan_t, delta_l, tan_l (stage_f / stage_d layout, see compiler).  cs: the
+// stage's tight cell offsets (cell c lists items [cs[c], cs[c+1])), csl the
+// loose ones (both in LDS or both in global memory).
 template <typename R, typename V, typename CS>
 __device__ __forceinline__ int cull_cell(const int *si, const R *g, V r, V d, R kmin, int &b, int &e,
-                                         CS cs) {
+                                         CS cs, CS csl) {
     R pa = r.x * g[0] + r.y * g[1] + r.z * g[2];
     R da = d.x * g[0] + d.y * g[1] + d.z * g[2];
     R pb1 = r.x * g[3] + r.y * g[4] + r.z * g[5];
@@ -269,17 +298,25 @@ __device__ __forceinline__ int cull_cell(const int *si, const R *g, V r, V d, R 
     R umin = u1 < u2 ? u1 : u2, umax = u1 < u2 ? u2 : u1;
     R wmin = w1 < w2 ? w1 : w2, wmax = w1 < w2 ? w2 : w1;
     if (umax < g[11] || umin > g[15] || wmax < g[12] || wmin > g[16]) return 0;
-    R delta2 = g[17] + g[17];
-    if ((umax - umin) > delta2 || (wmax - wmin) > delta2) return 2;
+    const R ext = (umax - umin) > (wmax - wmin) ? (umax - umin) : (wmax - wmin);
+    int tier;
+    if (ext <= g[17] + g[17]) tier = 1;          // tight list
+    else if (ext <= g[19] + g[19]) tier = 4;     // loose list (global memory)
+    else return 2;
     int G1 = si[6], G2 = si[7];
     int iu = (int)floor(((R)0.5 * (umin + umax) - g[11]) * g[13]);
     int iw = (int)floor(((R)0.5 * (wmin + wmax) - g[12]) * g[14]);
     iu = iu < 0 ? 0 : (iu >= G1 ? G1 - 1 : iu);
     iw = iw < 0 ? 0 : (iw >= G2 ? G2 - 1 : iw);
     const int c = iu * G2 + iw;
-    b = cs[c];
-    e = cs[c + 1];
-    return 1;
+    if (tier == 1) {
+        b = cs[c];
+        e = cs[c + 1];
+    } else {
+        b = csl[c];
+        e = csl[c + 1];
+    }
+    return tier;
 }
 
 __device__ __forceinline__ void put_f(const MxPhotons &P, int c, int64_t i, double v) {
@@ -322,10 +359,10 @@ struct Ray {
 
 // Uniform slot of the k-th interaction of a photon inside one stage.
 __device__ __forceinline__ double draw(const MxTables &T, const MxPhotons &P, const Ray &R, int us, int nus,
-                                       int k, int64_t i, Counters &cnt) {
-    if (nus <= 0) { cnt.bad++; return 0.0; }
+                                       int k, int64_t i, const Counters &cnt) {
+    if (nus <= 0) { count(cnt, MX_ST_BAD_ARGS); return 0.0; }
     int slot = k;
-    if (k >= nus) { slot = k % nus; cnt.slot_reuse++; }
+    if (k >= nus) { slot = k % nus; count(cnt, MX_ST_SLOT_REUSE); }
     if (us + slot == T.first_draw_slot) return R.u_first;
     return P.uniforms[(int64_t)(us + slot) * P.uniform_stride + i];
 }
@@ -476,7 +513,7 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
 // stage's records, parameters and culling lists are in LDS.
 template <int PREC, bool L>
 __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &P, const int s,
-                                            const int64_t i, const bool use_grid, Counters &cnt,
+                                            const int64_t i, const bool use_grid, const Counters &cnt,
                                             const unsigned char *lds, Ray &R) {
     const int *si = T.stage_i + (size_t)s * MX_SI;
     const double *sd = T.stage_d + (size_t)s * MX_SD;
@@ -490,14 +527,21 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
     const unsigned short *cs_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 26 : 24] : 0));
     const unsigned short *it_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 27 : 25] : 0));
     const unsigned short *su_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 31 : 30] : 0));
+    const unsigned short *csl_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 39 : 36] : 0));
+    const unsigned short *itl_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 40 : 37] : 0));
+    const unsigned short *sul_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 41 : 38] : 0));
     const int su_o = ((2 * (n_el + 1) + 15) & ~15) >> 1;   // items after the offsets
     const int *cs_g = T.cell_start + si[8];
     const int *ss_g = T.succ_start + si[28];
     const int *su_g = T.succ_items + si[29];
+    const int *csl_g = T.cell_start + si[32];
+    const int *ssl_g = T.succ_start + si[34];
+    const int *sul_g = T.succ_items + si[35];
     int last = -1, nhit = 0;
     for (;;) {
-        // mode 0: nothing can be hit; 1: culling-grid cell; 2: every element
-        // after `last`; 3: successors of the element hit last
+        // mode 0: nothing can be hit; 1: tight cell list; 2: every element
+        // after `last`; 3: tight successors of the element hit last; 4: loose
+        // cell list; 5: loose successors
         int cb = 0, ce = 0, mode = 2;
         // FP32: ray relative to the stage origin (one FP64 subtraction),
         // moved along itself to the stage mid-plane; k >= kmin32 is forward
@@ -518,31 +562,41 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         if (grid && last >= 0) {
             // after a hit: a ray within theta_max of the stage axis can only
             // reach the successors of the element it left
-            bool steep;
+            // |d . b1|, |d . b2| against |d . a| tan(theta) of each tier
+            bool tight, loose;
             if (PREC == 32) {
-                const float da = fabsf(dotf(R.df, ldf3(sf))) * sf[18];
-                steep = fabsf(dotf(R.df, ldf3(sf + 3))) <= da && fabsf(dotf(R.df, ldf3(sf + 6))) <= da;
+                const float da = fabsf(dotf(R.df, ldf3(sf)));
+                const float db = fmaxf(fabsf(dotf(R.df, ldf3(sf + 3))), fabsf(dotf(R.df, ldf3(sf + 6))));
+                tight = db <= da * sf[18];
+                loose = db <= da * sf[20];
             } else {
-                const double da = fabs(dot3(R.d, ld3(sd + 3))) * sd[21];
-                steep = fabs(dot3(R.d, ld3(sd + 6))) <= da && fabs(dot3(R.d, ld3(sd + 9))) <= da;
+                const double da = fabs(dot3(R.d, ld3(sd + 3)));
+                const double db = fmax(fabs(dot3(R.d, ld3(sd + 6))), fabs(dot3(R.d, ld3(sd + 9))));
+                tight = db <= da * sd[21];
+                loose = db <= da * sd[23];
             }
-            if (steep) {
+            if (tight) {
                 mode = 3;
                 if (lds_lists) { cb = su_l[last]; ce = su_l[last + 1]; }
                 else { cb = ss_g[last]; ce = ss_g[last + 1]; }
                 if (cb >= ce) break;
+            } else if (loose) {
+                mode = 5;
+                if (lds_lists) { cb = sul_l[last]; ce = sul_l[last + 1]; }
+                else { cb = ssl_g[last]; ce = ssl_g[last + 1]; }
+                if (cb >= ce) break;
             }
         } else if (grid) {
             if (PREC == 32) {
-                if (lds_lists) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l);
-                else mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_g);
+                if (lds_lists) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l, csl_l);
+                else mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_g, csl_g);
             } else {
-                if (lds_lists) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l);
-                else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g);
+                if (lds_lists) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l, csl_l);
+                else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g, csl_g);
             }
             if (mode == 0) break;
         }
-        if (mode == 2 && grid) cnt.grid_fallback++;
+        if (mode == 2 && grid) count(cnt, MX_ST_GRID_FALLBACK);
         // ---- next hit in ascending element order
         d3 q = mk(0, 0, 0);
         double ly = 0, lz = 0;
@@ -551,18 +605,22 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         const int tb = list ? cb : last + 1;
         const int te = list ? ce : n_el;
         int hit;
+        // global candidate lists: tight successors, loose successors, or
+        // cell_items (tight and loose cell lists hold absolute indices)
+        const int *items_g = mode == 3 ? su_g : (mode == 5 ? sul_g : T.cell_items);
         if (L) {
             if (lds_lists)
-                hit = scan_next<PREC>(rec_l, list, mode == 3 ? su_l + su_o : it_l, tb, te, last,
-                                      R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+                hit = scan_next<PREC>(rec_l, list,
+                                      mode == 3 ? su_l + su_o : (mode == 5 ? sul_l + su_o : (mode == 4 ? itl_l : it_l)),
+                                      tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
             else
-                hit = scan_next<PREC>(rec_l, list, mode == 3 ? su_g : T.cell_items, tb, te, last,
+                hit = scan_next<PREC>(rec_l, list, items_g, tb, te, last,
                                       R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
         } else if (PREC == 64) {
-            hit = scan_next<PREC>(T.elem_t + (size_t)e0 * MX_ET, list, mode == 3 ? su_g : T.cell_items,
+            hit = scan_next<PREC>(T.elem_t + (size_t)e0 * MX_ET, list, items_g,
                                   tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
         } else {
-            hit = scan_next<PREC>(T.elem_t32 + (size_t)e0 * MX_ET, list, mode == 3 ? su_g : T.cell_items,
+            hit = scan_next<PREC>(T.elem_t32 + (size_t)e0 * MX_ET, list, items_g,
                                   tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
         }
         if (hit < 0) break;
@@ -578,7 +636,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         const double *ep = T.elem_p + eg * MX_EP;
         const int *ei = T.elem_i + eg * MX_EI;
         int id;
-        if (nhit > 0) cnt.multi_hit++;
+        if (nhit > 0) count(cnt, MX_ST_MULTI_HIT);
 
         if (kind == MX_STAGE_GRATING) {
             const double u = draw(T, P, R, si[4], si[5], nhit, i, cnt);
@@ -586,7 +644,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             double blaze;
             if (PREC == 64) {
                 id = ei[0];
-                if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) cnt.order_empty++;
+                if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) count(cnt, MX_ST_ORDER_EMPTY);
                 blaze = diffract64(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
             } else {
                 const float wave = MX_E2W_F / (float)R.energy;
@@ -595,11 +653,11 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
                     const float *er = (const float *)rec_l + (size_t)hit * MX_ET;
                     id = __float_as_int(prm[9]);
                     if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds))
-                        cnt.order_empty++;
+                        count(cnt, MX_ST_ORDER_EMPTY);
                     blaze = (double)diffract32(ldf3(er + 3), prm, wave, (float)ly, m, R.df, R.polf);
                 } else {
                     id = ei[0];
-                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) cnt.order_empty++;
+                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) count(cnt, MX_ST_ORDER_EMPTY);
                     float prm[16];
                     for (int k = 0; k < 9; ++k) prm[k] = T.elem_p32[eg * MX_EP + k];
                     prm[11] = __int_as_float(ei[2]);
@@ -608,20 +666,25 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
                                                R.df, R.polf);
                 }
             }
-            if (ptot < 0.0 || ptot > 1.0) cnt.prob_range++;
+            if (ptot < 0.0 || ptot > 1.0) count(cnt, MX_ST_PROB_RANGE);
             R.prob = R.prob * ptot;
             put_i(P, si[13], i, m);
             put_f(P, si[14], i, blaze);
         } else if (kind == MX_STAGE_DETECTOR) {
             id = ei[0];
+            double dp0, dp1;
             if (PREC == 32) {
                 // final detector coordinate in FP64
                 double y64, z64;
-                plane_test64(et, R.p, tod(R.df), q, y64, z64);
+                plane_hit64_fast(et, R.p, tod(R.df), q, y64, z64);
                 ly = y64; lz = z64;
+                const double ip = ep[9];   // 1 / pixsize
+                dp0 = ly * ip + ep[7];
+                dp1 = lz * ip + ep[8];
+            } else {
+                dp0 = ly / ep[6] + ep[7];
+                dp1 = lz / ep[6] + ep[8];
             }
-            const double dp0 = ly / ep[6] + ep[7];
-            const double dp1 = lz / ep[6] + ep[8];
             put_f(P, si[15], i, dp0);
             put_f(P, si[16], i, dp1);
             put_i(P, si[17], i, pixel_index(dp0, ei[4]));
@@ -671,7 +734,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
 // One photon through all stages.
 template <int PREC>
 __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
-                                          const bool use_grid, Counters &cnt,
+                                          const bool use_grid, const Counters &cnt,
                                           const unsigned char *lds) {
     Ray R;
     R.p = mk(0.0, 0.0, 0.0);
@@ -730,7 +793,7 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             put_i(P, si[12], i, T.elem_i[(size_t)e0 * MX_EI]);
             continue;
         }
-        if (kind < MX_STAGE_LENS_PERFECT || kind > MX_STAGE_DETECTOR) { cnt.bad++; continue; }
+        if (kind < MX_STAGE_LENS_PERFECT || kind > MX_STAGE_DETECTOR) { count(cnt, MX_ST_BAD_ARGS); continue; }
         if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true>(T, P, s, i, use_grid, cnt, lds, R);
         else array_stage<PREC, false>(T, P, s, i, use_grid, cnt, lds, R);
     }
@@ -749,41 +812,24 @@ template <int PREC, int BS>
 __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, int flags,
                                                        unsigned long long *status) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned s_count[MX_NCOUNT];
     const int lds_bytes = PREC == 64 ? T.lds_bytes64 : T.lds_bytes32;
+    if (threadIdx.x < MX_NCOUNT) s_count[threadIdx.x] = 0;
     if (lds_bytes > 0) {
         const uint4 *src = (const uint4 *)(PREC == 64 ? T.lds_image64 : T.lds_image32);
         uint4 *dst = (uint4 *)lds;
         const int nvec = (lds_bytes + 15) >> 4;
         for (int v = threadIdx.x; v < nvec; v += BS) dst[v] = src[v];
-        __syncthreads();
     }
-    Counters cnt = {0, 0, 0, 0, 0, 0};
+    __syncthreads();
+    const Counters cnt = {s_count};
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
         trace_one<PREC>(T, P, i, use_grid, cnt, lds);
-
-    if (status) {
-        if (cnt.prob_range) atomicAdd(status + MX_ST_PROB_RANGE, (unsigned long long)cnt.prob_range);
-        if (cnt.order_empty) atomicAdd(status + MX_ST_ORDER_EMPTY, (unsigned long long)cnt.order_empty);
-        if (cnt.slot_reuse) atomicAdd(status + MX_ST_SLOT_REUSE, (unsigned long long)cnt.slot_reuse);
-        if (cnt.grid_fallback) atomicAdd(status + MX_ST_GRID_FALLBACK, (unsigned long long)cnt.grid_fallback);
-        if (cnt.multi_hit) atomicAdd(status + MX_ST_MULTI_HIT, (unsigned long long)cnt.multi_hit);
-        if (cnt.bad) atomicAdd(status + MX_ST_BAD_ARGS, (unsigned long long)cnt.bad);
-    }
-}
-
-// Wave-reduced status counters of one tile, added to one variant's block.
-__device__ __forceinline__ void flush_counters(unsigned long long *st, const Counters &c) {
-    unsigned v[6] = {c.prob_range, c.order_empty, c.slot_reuse, c.grid_fallback, c.multi_hit, c.bad};
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (v[k]) atomicAdd(st + k, (unsigned long long)v[k]);
-    }
+    __syncthreads();
+    if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
+        atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
 }
 
 // Batched persistent kernel: workgroups take tiles (variant, photon range)
@@ -800,7 +846,10 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ long long s_tile;
     __shared__ int s_var;
+    __shared__ unsigned s_count[MX_NCOUNT];
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
+    const Counters cnt = {s_count};
+    if (threadIdx.x < MX_NCOUNT) s_count[threadIdx.x] = 0;
     int cur = -1;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -842,10 +891,15 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         int64_t e = b + tp;
         if (e > P.n) e = P.n;
         if (b < 0 || tp <= 0) e = b;
-        Counters cnt = {0, 0, 0, 0, 0, 0};
         for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC>(T, P, i, use_grid, cnt, lds);
-        if (status) flush_counters(status + (size_t)v * MX_NSTATUS, cnt);
         __syncthreads();
+        // this tile's counters go to its variant; the next claim's barrier
+        // orders the reset before the next tile counts
+        if (threadIdx.x < MX_NCOUNT) {
+            const unsigned c = s_count[threadIdx.x];
+            if (c && status) atomicAdd(status + (size_t)v * MX_NSTATUS + threadIdx.x, (unsigned long long)c);
+            s_count[threadIdx.x] = 0;
+        }
     }
 }
 

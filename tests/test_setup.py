@@ -291,12 +291,31 @@ def _candidates(g, p, d):
     w = [r @ g['b2'] + k * (d @ g['b2']) for k in (kmin, kmax)]
     if max(u) < g['U0'] or min(u) > g['U1'] or max(w) < g['W0'] or min(w) > g['W1']:
         return []
-    if max(u) - min(u) > 2 * g['delta'] or max(w) - min(w) > 2 * g['delta']:
+    ext = max(max(u) - min(u), max(w) - min(w))
+    if ext <= 2 * g['delta']:
+        starts, items = g['cell_start'], g['items']
+    elif ext <= 2 * g['delta_l']:
+        starts, items = g['cell_start_l'], g['items_l']
+    else:
         return None
     iu = int(np.clip(np.floor((0.5 * sum(u) - g['U0']) * g['inv_cu']), 0, g['G1'] - 1))
     iw = int(np.clip(np.floor((0.5 * sum(w) - g['W0']) * g['inv_cw']), 0, g['G2'] - 1))
     c = iu * g['G2'] + iw
-    return list(g['items'][g['cell_start'][c]:g['cell_start'][c + 1]])
+    return list(items[starts[c]:starts[c + 1]])
+
+
+def _successors(g, j, d):
+    """numpy restatement of the kernel's successor-list choice after a hit
+    on element j (None: every later element is tested)."""
+    da = abs(d @ g['a'])
+    db = max(abs(d @ g['b1']), abs(d @ g['b2']))
+    if db <= da * g['tan_theta']:
+        st, it = g['succ_start'], g['succ_items']
+    elif db <= da * g['tan_l']:
+        st, it = g['succ_start_l'], g['succ_items_l']
+    else:
+        return None
+    return list(it[st[j]:st[j + 1]])
 
 
 @pytest.mark.parametrize('which', ['flat', 'cat_facets', 'cat_ccds'])
@@ -334,3 +353,43 @@ def test_grid_covers_every_hit(which):
                 continue
             assert j in c, (which, i, j)
     assert n_fall < 0.05 * len(p)
+
+
+@pytest.mark.parametrize('which', ['cat_facets', 'cat_ccds'])
+def test_successor_lists_cover_every_later_hit(which):
+    """After a ray leaves element j (any direction within the tier's angle),
+    every later element k > j it hits is in j's successor list, so the
+    kernel's exact successor scan equals testing all k > j in order."""
+    rng = np.random.default_rng(12)
+    inst, parts = configs.cat_rowland_instrument(200)
+    els = parts['gratings'].elements if which == 'cat_facets' else parts['ccds'].elements
+    frames = [plane_frame(e.geometry) for e in els]
+    ofr = [OT.frame(e.pos4d) for e in els]
+    g = build_grid(frames)
+    n_checked = n_all = 0
+    for j in range(len(els)):
+        f = frames[j]
+        # start points on element j, directions spread around the stage axis
+        m = 60
+        yz = rng.uniform(-1, 1, (m, 2))
+        p = f['center'] + yz[:, :1] * f['v_y'] + yz[:, 1:] * f['v_z']
+        ang = rng.uniform(0, 0.4, m)
+        phi = rng.uniform(0, 2 * np.pi, m)
+        a = -g['a'] if (f['e_x'] @ g['a']) > 0 else g['a']
+        d = (np.cos(ang)[:, None] * np.sign(rng.uniform(-1, 1, m))[:, None] * a +
+             np.sin(ang)[:, None] * (np.cos(phi)[:, None] * g['b1'] + np.sin(phi)[:, None] * g['b2']))
+        for k in range(j + 1, len(els)):
+            hit, q, ly, lz = OT.intersect(ofr[k], p, d)
+            for i in np.nonzero(hit)[0]:
+                n_all += 1
+                c = _successors(g, j, d[i])
+                if c is None:
+                    continue
+                n_checked += 1
+                assert k in c, (which, j, k, i)
+    if which == 'cat_facets':
+        # overlapping facets: many later hits, most of them within a tier
+        assert n_checked > 0.5 * n_all and n_checked > 20
+    else:
+        # the CCD row does not overlap along the beam
+        assert n_all == 0
