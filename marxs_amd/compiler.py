@@ -518,20 +518,27 @@ def _build_lds(tables, budget32=None, budget64=None):
         for k in range(tables.sel_i.shape[0]):
             nr, no = int(tables.sel_i[k, 0]), int(tables.sel_i[k, 1])
             need += _align(8 * (2 * nr + nr * no)) + _align(4 * no)
+        n_sel = tables.sel_i.shape[0]
+        need += _align(4 * SELI * n_sel)
         if need > budget:
             sel_ok = False
-        if sel_ok and tables.sel_i.shape[0] and tables.sel_i[0, 0] > 0:
-            for k in range(tables.sel_i.shape[0]):
+        col = 6 if prec == 32 else 7
+        tables.sel_i[:, col] = -1
+        if sel_ok and n_sel and tables.sel_i[0, 0] > 0:
+            # header rows (the kernel reads them from LDS), then the blocks
+            hdr = add(np.zeros(SELI * n_sel, dtype=np.int32))
+            assert hdr == 0
+            for k in range(n_sel):
                 nr, no, oe, oc, ot, oo = (int(x) for x in tables.sel_i[k, :6])
                 blk = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[oc:oc + nr * no],
                                       tables.sel_d[ot:ot + nr]])
                 o = add(blk)
                 add(tables.sel_o[oo:oo + no].astype(np.int32))
                 sel_offs.append(o)
-        col = 6 if prec == 32 else 7
-        tables.sel_i[:, col] = -1
-        for k, o in enumerate(sel_offs):
-            tables.sel_i[k, col] = o
+            for k, o in enumerate(sel_offs):
+                tables.sel_i[k, col] = o
+            # fill the header with this image's rows
+            parts[0] = (0, np.ascontiguousarray(tables.sel_i).astype(np.int32).view(np.uint8).ravel().copy())
         for s, st in enumerate(tables.stages):
             n = len(st.elements)
             if st.kind < STAGE_LENS_PERFECT or n < 2:
@@ -592,10 +599,20 @@ def _selector_row(sel, sel_index, sel_rows, sel_d, sel_o):
         # the reference raises only for photons that use such a row; the
         # kernel counts them and the host raises after the trace
         pass
-    increasing = bool(nr == 1 or np.all(np.diff(np.asarray(energy, dtype=float)) > 0))
+    e = np.asarray(energy, dtype=float)
+    increasing = bool(nr == 1 or np.all(np.diff(e) > 0))
     monotone = bool(np.all(np.diff(np.asarray(cdf, dtype=float), axis=1) >= 0))
-    row = [nr, no, len(sel_d), 0, 0, len(sel_o), -1, -1, int(increasing) | (int(monotone) << 1),
-           0, 0, 0]
+    # evenly spaced energies (within 10 % of the step): the kernel's nearest
+    # row search reads only the three rows around round((E - e0) / step)
+    even, e0_bits, inv_bits = False, 0, 0
+    if increasing and nr >= 2:
+        step = (e[-1] - e[0]) / (nr - 1)
+        if step > 0 and np.all(np.abs(e - (e[0] + step * np.arange(nr))) <= 0.1 * step):
+            even = True
+            e0_bits = int(np.array([e[0]], dtype=np.float32).view(np.int32)[0])
+            inv_bits = int(np.array([1.0 / step], dtype=np.float32).view(np.int32)[0])
+    row = [nr, no, len(sel_d), 0, 0, len(sel_o), -1, -1,
+           int(increasing) | (int(monotone) << 1) | (int(even) << 2), e0_bits, inv_bits, 0]
     sel_d.extend(np.asarray(energy, dtype=float).tolist())
     row[3] = len(sel_d)
     sel_d.extend(np.asarray(cdf, dtype=float).ravel().tolist())

@@ -46,6 +46,9 @@
 #define MX_BLOCK64 768
 #endif
 #define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
+#ifndef MX_CDF_COUNT
+#define MX_CDF_COUNT 0          /* 1: count cdf entries <= u instead of bisecting */
+#endif
 
 static thread_local char g_last_error[512] = "";
 
@@ -62,7 +65,7 @@ extern "C" int mx_abi_version(void) { return MX_ABI_VERSION; }
 struct d3 { double x, y, z; };
 
 __device__ __forceinline__ d3 mk(double x, double y, double z) { d3 r; r.x = x; r.y = y; r.z = z; return r; }
-__device__ __forceinline__ d3 ld3(const double *p) { return mk(p[0], p[1], p[2]); }
+template <typename PT> __device__ __forceinline__ d3 ld3(PT p) { return mk(p[0], p[1], p[2]); }
 __device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ d3 sub3(d3 a, d3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ d3 cross3(d3 a, d3 b) {
@@ -138,7 +141,7 @@ __device__ __forceinline__ d3 ptrans64(d3 d1, d3 d2, d3 pol) {
 // ------------------------------------------------------------------ FP32
 struct f3 { float x, y, z; };
 __device__ __forceinline__ f3 mkf(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
-__device__ __forceinline__ f3 ldf3(const float *p) { return mkf(p[0], p[1], p[2]); }
+template <typename PT> __device__ __forceinline__ f3 ldf3(PT p) { return mkf(p[0], p[1], p[2]); }
 __device__ __forceinline__ float dotf(f3 a, f3 b) {
 #pragma clang fp contract(fast)
     return a.x * b.x + a.y * b.y + a.z * b.z;
@@ -191,6 +194,13 @@ __device__ __forceinline__ f3 ptrans32(f3 d1, f3 d2, f3 pol) {
 }
 
 // --------------------------------------------------------------- shared
+// The instrument tables are never written by the trace: pointers into them
+// are typed in the constant address space, so that loads from wave-uniform
+// addresses (stage records, the element of a one-element stage) become scalar
+// loads even though the kernel writes other global memory.
+template <typename T> using cptr = const __attribute__((address_space(4))) T *;
+template <typename T> __device__ __forceinline__ cptr<T> cp(const T *p) { return (cptr<T>)p; }
+
 // Status counters of a workgroup, in LDS (indices MX_ST_*).  An event is
 // counted once per wave: the first active lane adds the number of active
 // lanes, so the counters take no vector registers.
@@ -207,15 +217,37 @@ __device__ __forceinline__ void count(const Counters &cnt, int k) {
 // Order selection (OrderSelector grating.py:51-57, EfficiencyFile 90-96):
 // row = nearest energy (first on ties); order index = first j with cdf > u.
 // Tables: energies eg[nr], cdf[nr*no], tot[nr] (float64), orders ord[no].
-// flags bit0: energies strictly increasing (binary search, equal to the
-// first-on-ties argmin); bit1: every cdf row non-decreasing (binary search
-// for the first entry > u).  Otherwise linear scans.
-template <typename DP, typename IP>
-__device__ __forceinline__ bool select_order_t(DP eg, DP cdf, DP tot, IP ord, int nr, int no, int fl,
+// Header hd (MX_SELI ints): nr, no, offsets, LDS offsets, flags (hd[8]):
+// bit0 energies strictly increasing, bit1 every cdf row non-decreasing,
+// bit2 energies evenly spaced within 10 % of the step, with hd[9], hd[10] the
+// float bits of the first energy and of 1/step.
+//  * evenly spaced: the nearest row is within one of round((E - e0)/step), so
+//    three independent loads and a first-on-ties comparison give argmin;
+//  * increasing: binary search;
+//  * otherwise a linear first-on-ties scan.
+//  * non-decreasing cdf row: the first j with cdf[j] > u is the number of
+//    entries <= u (independent loads, no dependency chain); otherwise a
+//    linear scan.
+template <typename HP, typename DP, typename IP>
+__device__ __forceinline__ bool select_order_t(HP hd, DP eg, DP cdf, DP tot, IP ord,
                                                double energy, double u, int &order, double &ptot) {
+    const int nr = hd[0], no = hd[1], fl = hd[8];
     int row = 0;
     if (nr > 1) {
-        if (fl & 1) {
+        if (fl & 4) {
+            const float x = ((float)energy - __int_as_float(hd[9])) * __int_as_float(hd[10]);
+            int r = (int)rintf(x);
+            r = r < 1 ? 1 : (r > nr - 2 ? nr - 2 : r);
+            if (nr == 2) r = 0;
+            // candidates r-1, r, r+1 (clamped), first minimum in index order
+            const int r0 = r > 0 ? r - 1 : 0;
+            const int r2 = r + 1 < nr ? r + 1 : nr - 1;
+            const double d0 = fabs(eg[r0] - energy), d1 = fabs(eg[r] - energy), d2 = fabs(eg[r2] - energy);
+            row = r0;
+            double best = d0;
+            if (d1 < best) { best = d1; row = r; }
+            if (d2 < best) { row = r2; }
+        } else if (fl & 1) {
             int lo = 0, hi = nr;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
@@ -236,6 +268,10 @@ __device__ __forceinline__ bool select_order_t(DP eg, DP cdf, DP tot, IP ord, in
     DP c = cdf + (size_t)row * no;
     int j;
     if (fl & 2) {
+#if MX_CDF_COUNT
+        j = 0;
+        for (int k = 0; k < no; ++k) j += c[k] <= u ? 1 : 0;
+#else
         int lo = 0, hi = no;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -243,6 +279,7 @@ __device__ __forceinline__ bool select_order_t(DP eg, DP cdf, DP tot, IP ord, in
             else lo = mid + 1;
         }
         j = lo;
+#endif
     } else {
         j = 0;
         while (j < no && !(c[j] > u)) ++j;
@@ -253,19 +290,22 @@ __device__ __forceinline__ bool select_order_t(DP eg, DP cdf, DP tot, IP ord, in
     return true;
 }
 
+// Selector sel of the tables.  When the selectors are staged in LDS (all or
+// none), their header rows sit at the start of the LDS image.
 template <int PREC>
 __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double energy, double u,
                                              int &order, double &ptot, const unsigned char *lds) {
-    const int *si = T.sel_i + (size_t)sel * MX_SELI;
-    const int nr = si[0], no = si[1], fl = si[8];
-    const int loff = si[PREC == 64 ? 7 : 6];
-    if (loff >= 0) {
+    if (cp(T.sel_i)[PREC == 64 ? 7 : 6] >= 0) {
+        const int *hd = (const int *)lds + (size_t)sel * MX_SELI;
+        const int nr = hd[0], no = hd[1];
+        const int loff = hd[PREC == 64 ? 7 : 6];
         const double *b = (const double *)(lds + loff);
         const int *o = (const int *)(lds + loff + ((8 * (2 * nr + nr * no) + 15) & ~15));
-        return select_order_t(b, b + nr, b + nr + nr * no, o, nr, no, fl, energy, u, order, ptot);
+        return select_order_t(hd, b, b + nr, b + nr + nr * no, o, energy, u, order, ptot);
     }
-    return select_order_t(T.sel_d + si[2], T.sel_d + si[3], T.sel_d + si[4], T.sel_o + si[5],
-                          nr, no, fl, energy, u, order, ptot);
+    const int *hd = T.sel_i + (size_t)sel * MX_SELI;
+    return select_order_t(hd, T.sel_d + hd[2], T.sel_d + hd[3], T.sel_d + hd[4], T.sel_o + hd[5],
+                          energy, u, order, ptot);
 }
 
 // Candidate range of an array stage for one ray r + k d, k >= kmin, with r
@@ -275,8 +315,8 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
 // tan_t, delta_l, tan_l (stage_f / stage_d layout, see compiler).  cs: the
 // stage's tight cell offsets (cell c lists items [cs[c], cs[c+1])), csl the
 // loose ones (both in LDS or both in global memory).
-template <typename R, typename V, typename CS>
-__device__ __forceinline__ int cull_cell(const int *si, const R *g, V r, V d, R kmin, int &b, int &e,
+template <typename R, typename V, typename CS, typename SIP, typename GP>
+__device__ __forceinline__ int cull_cell(SIP si, GP g, V r, V d, R kmin, int &b, int &e,
                                          CS cs, CS csl) {
     R pa = r.x * g[0] + r.y * g[1] + r.z * g[2];
     R da = d.x * g[0] + d.y * g[1] + d.z * g[2];
@@ -329,7 +369,8 @@ __device__ __forceinline__ void put_i(const MxPhotons &P, int c, int64_t i, int 
 // Default values (NaN / -1) for the columns this stage creates, for a photon
 // that did not interact with any element of the stage
 // (add_output_cols, marxs/base/base.py:271-286).
-__device__ __forceinline__ void put_defaults(const MxPhotons &P, const int *si, int64_t i) {
+template <typename SIP>
+__device__ __forceinline__ void put_defaults(const MxPhotons &P, SIP si, int64_t i) {
     const int always = si[19];
     if (!always) return;
     const double qnan = __builtin_nan("");
@@ -515,9 +556,9 @@ template <int PREC, bool L>
 __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &P, const int s,
                                             const int64_t i, const bool use_grid, const Counters &cnt,
                                             const unsigned char *lds, Ray &R) {
-    const int *si = T.stage_i + (size_t)s * MX_SI;
-    const double *sd = T.stage_d + (size_t)s * MX_SD;
-    const float *sf = T.stage_f + (size_t)s * MX_SD;
+    const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
+    const cptr<double> sd = cp(T.stage_d) + (size_t)s * MX_SD;
+    const cptr<float> sf = cp(T.stage_f) + (size_t)s * MX_SD;
     const int kind = si[0], n_el = si[1], e0 = si[2];
     const bool grid = use_grid && (si[3] & 1);
     const d3 O = ld3(sd);
@@ -753,14 +794,17 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
     R.u_first = (T.first_draw_slot >= 0 && T.first_draw_slot < P.n_uniform_slots)
                     ? P.uniforms[(int64_t)T.first_draw_slot * P.uniform_stride + i] : 0.0;
 
-    for (int s = 0; s < T.n_stages; ++s) {
-        const int *si = T.stage_i + (size_t)s * MX_SI;
+    for (int s_ = 0; s_ < T.n_stages; ++s_) {
+        // the stage index is the same in every lane: keep it (and all stage
+        // metadata derived from it) in scalar registers
+        const int s = __builtin_amdgcn_readfirstlane(s_);
+        const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
         const int kind = si[0], e0 = si[2];
 
         if (kind == MX_STAGE_APERTURE_CIRCLE || kind == MX_STAGE_APERTURE_RECT) {
             // aperture.py:68-78,92-95,138-146
-            const double *et = T.elem_t + (size_t)e0 * MX_ET;
-            const double *ep = T.elem_p + (size_t)e0 * MX_EP;
+            const cptr<double> et = cp(T.elem_t) + (size_t)e0 * MX_ET;
+            const cptr<double> ep = cp(T.elem_p) + (size_t)e0 * MX_EP;
             double u0 = draw(T, P, R, si[4], si[5], 0, i, cnt);
             double u1 = draw(T, P, R, si[4], si[5], 1, i, cnt);
             double x, y, f;
@@ -785,12 +829,12 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             d3 c = ld3(et), vy = ld3(ep), vz = ld3(ep + 3);
             R.p = mk(c.x + x * vy.x + y * vz.x, c.y + x * vy.y + y * vz.y, c.z + x * vy.z + y * vz.z);
             if (PREC == 64) f = dot3(R.d, ld3(ep + 10));
-            else f = (double)dotf(R.df, ldf3(T.elem_p32 + (size_t)e0 * MX_EP + 10));
+            else f = (double)dotf(R.df, ldf3(cp(T.elem_p32) + (size_t)e0 * MX_EP + 10));
             f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
             R.prob = R.prob * f;
             put_f(P, si[10], i, x);
             put_f(P, si[11], i, y);
-            put_i(P, si[12], i, T.elem_i[(size_t)e0 * MX_EI]);
+            put_i(P, si[12], i, cp(T.elem_i)[(size_t)e0 * MX_EI]);
             continue;
         }
         if (kind < MX_STAGE_LENS_PERFECT || kind > MX_STAGE_DETECTOR) { count(cnt, MX_ST_BAD_ARGS); continue; }

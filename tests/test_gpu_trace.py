@@ -249,3 +249,26 @@ def test_golden_vectors(gpu, name):
     inst = inst[0] if isinstance(inst, tuple) else inst
     got, st = gpu_trace(inst, ph, u, precision=64)
     compare(out, got, rtol=1e-9, near=near_pixel_boundary(out))
+
+
+@pytest.mark.parametrize('grid', ['even', 'uneven', 'two'])
+def test_efficiency_row_search_ties_and_ends(gpu, grid):
+    """Nearest energy row (first on ties, grating.py:93-94) for energies on
+    grid points, exactly half-way between rows (ties), off both ends, and for
+    evenly and unevenly spaced tables (different search paths in the kernel)."""
+    energies = {'even': np.round(np.arange(0.3, 2.0001, 0.1), 10),
+                'uneven': np.array([0.3, 0.35, 0.5, 0.9, 1.0, 1.6, 2.0]),
+                'two': np.array([0.5, 1.5])}[grid]
+    orders = np.array([-2, -1, 0, 1])
+    rng = np.random.default_rng(9)
+    prob = rng.uniform(0.05, 0.2, (len(energies), len(orders)))
+    eff = mx.EfficiencyFile.from_arrays(energies, prob, orders)
+    mids = 0.5 * (energies[1:] + energies[:-1])
+    e = np.concatenate([energies, mids, [0.01, 0.29, 2.5, 10.], rng.uniform(0, 3, 2000)])
+    n = len(e)
+    ph, u = _photons(n, 10, spread=0.0)
+    ph['energy'] = e
+    ph['pos'][:, 1:] = rng.uniform(-5, 5, (n, 2))
+    grat = mx.FlatGrating(d=1e-3, order_selector=eff, zoom=[1, 20, 20])
+    ref, got, st = _single([grat], ph, u[:2])
+    assert np.array_equal(ref['order'], got['order'])
