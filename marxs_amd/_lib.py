@@ -9,13 +9,13 @@ import os
 
 import torch  # noqa: F401  (loads the HIP runtime first)
 
-__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
+__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseArgs', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
            'MX_NSTATUS', 'LIB_PATH', 'check']
 
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 11
+MX_ABI_VERSION = 12
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -46,6 +46,7 @@ class MxAdjointArgs(ctypes.Structure):
     _fields_ = [('n', ctypes.c_int64), ('pos', P * 3), ('dir', P * 3), ('energy', P),
                 ('weight', P), ('facet', P), ('order', P), ('ccd', P), ('geom', P),
                 ('ccd_t', P), ('seed', P), ('y_out', P), ('grad', P), ('moments', P),
+                ('seed_moments', P), ('y_shift', ctypes.c_double),
                 ('n_facet', ctypes.c_int32), ('n_ccd', ctypes.c_int32),
                 ('coord', ctypes.c_int32), ('precision', ctypes.c_int32),
                 ('lds_accumulate', ctypes.c_int32), ('reserved', ctypes.c_int32)]
@@ -56,6 +57,13 @@ class MxBatch(ctypes.Structure):
                 ('n_tiles', ctypes.c_int64), ('tables', P), ('photons', P),
                 ('tables_dev', P), ('photons_dev', P), ('tile_first', P),
                 ('tile_photons', P), ('queue', P), ('status', P)]
+
+
+class MxPoseArgs(ctypes.Structure):
+    _fields_ = [('n_facet', ctypes.c_int32), ('mode', ctypes.c_int32), ('theta', P),
+                ('period', P), ('gu', P), ('t0', P), ('rpost', P), ('zoom', P), ('extra', P),
+                ('geom', P), ('grad13', P), ('grad7', P), ('moments', P), ('loss', P),
+                ('y_shift', ctypes.c_double)]
 
 
 MX_AJ_GEOM = 16
@@ -79,6 +87,8 @@ def _load():
     lib.mx_trace_batched.argtypes = [ctypes.POINTER(MxBatch), ctypes.c_int32, ctypes.c_int32, P]
     lib.mx_adjoint.restype = ctypes.c_int
     lib.mx_adjoint.argtypes = [ctypes.POINTER(MxAdjointArgs), P]
+    lib.mx_facet_pose.restype = ctypes.c_int
+    lib.mx_facet_pose.argtypes = [ctypes.POINTER(MxPoseArgs), P]
     v = lib.mx_abi_version()
     if v != MX_ABI_VERSION:
         raise ImportError(f'libmarxs_hip.so ABI {v}, expected {MX_ABI_VERSION}: rebuild it')

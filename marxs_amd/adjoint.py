@@ -148,17 +148,22 @@ class AdjointProblem:
                               self.transmission)
 
     # ----------------------------------------------------------- kernels
-    def _args(self, geom, precision, seed=None, y_out=None, grad=None, moments=None):
+    def _args(self, geom, precision, seed=None, y_out=None, grad=None, moments=None, src=None):
+        """Kernel arguments over all photons, or over the compacted photon
+        columns ``src`` (a dict from :meth:`_pose_buffers`)."""
+        c = src if src is not None else {
+            'n': self.n, 'pos': self.pos, 'dir': self.dir, 'energy': self.energy,
+            'weight': self.weight, 'facet': self.facet, 'order': self.order, 'ccd': self.ccd}
         a = _lib.MxAdjointArgs()
-        a.n = self.n
+        a.n = c['n']
         for k in range(3):
-            a.pos[k] = self.pos[k].data_ptr()
-            a.dir[k] = self.dir[k].data_ptr()
-        a.energy = self.energy.data_ptr()
-        a.weight = self.weight.data_ptr()
-        a.facet = self.facet.data_ptr()
-        a.order = self.order.data_ptr()
-        a.ccd = self.ccd.data_ptr()
+            a.pos[k] = c['pos'][k].data_ptr()
+            a.dir[k] = c['dir'][k].data_ptr()
+        a.energy = c['energy'].data_ptr()
+        a.weight = c['weight'].data_ptr()
+        a.facet = c['facet'].data_ptr()
+        a.order = c['order'].data_ptr()
+        a.ccd = c['ccd'].data_ptr()
         a.geom = geom.data_ptr()
         a.ccd_t = self.ccd_t.data_ptr()
         a.seed = seed.data_ptr() if seed is not None else None
@@ -186,20 +191,118 @@ class AdjointProblem:
                 torch.cuda.current_stream(self.device).cuda_stream)
         return grad
 
+    def loss(self, theta, period, precision=64):
+        """L(theta, period) = centroid + RMS width (differentiable)."""
+        geom = self.geometry(theta, period)
+        return FacetLoss.apply(geom, self, precision)
+
+    # -------------------------------------------------- fused fast path
+    def _pose_buffers(self):
+        """Device copies of the fixed pose-chain parts and the work buffers of
+        :meth:`loss_and_grad` (allocated once, so that the call can be
+        captured in a hipGraph)."""
+        if getattr(self, '_pb', None) is not None:
+            return self._pb
+        f64 = dict(dtype=torch.float64, device=self.device)
+        b = self.base
+        F = self.F
+        pb = {'gu': b['GU'].contiguous().reshape(16).clone(),
+              't0': b['t0'].contiguous().clone(),
+              'rpost': b['Rpost'].contiguous().reshape(F, 9).clone(),
+              'zoom': b['zoom'].contiguous().clone(),
+              'extra': torch.stack([self.groove, self.sign_mode, self.transmission], 1).contiguous(),
+              'theta': torch.zeros((F, 6), **f64), 'period': torch.zeros(F, **f64),
+              'geom': torch.zeros((F, 16), **f64), 'grad13': torch.zeros((F, 13), **f64),
+              'grad7': torch.zeros((F, 7), **f64), 'moments': torch.zeros(3, **f64),
+              'loss': torch.zeros(1, **f64)}
+        # shift of the moments: the weighted mean coordinate at the base pose
+        y = self.coordinates(self.geometry(self.theta0, self.period0))
+        ok = ~torch.isnan(y)
+        w = torch.where(ok, self.weight, torch.zeros_like(self.weight))
+        self.y_shift = float((w * torch.where(ok, y, torch.zeros_like(y))).sum() / w.sum())
+        # stream compaction: the fused kernels visit only the photons that
+        # carry weight on a facet and a detector
+        idx = torch.nonzero((self.weight != 0) & (self.facet >= 0) & (self.ccd >= 0)).flatten()
+        pb['src'] = {'n': int(idx.numel()), 'idx': idx,
+                     'pos': self.pos[:, idx].contiguous(), 'dir': self.dir[:, idx].contiguous(),
+                     'energy': self.energy[idx].contiguous(), 'weight': self.weight[idx].contiguous(),
+                     'facet': self.facet[idx].contiguous(), 'order': self.order[idx].contiguous(),
+                     'ccd': self.ccd[idx].contiguous()}
+        self._pb = pb
+        return pb
+
     def restrict_to_traced(self, y_traced, rtol=1e-10):
         """Zero the weight of photons whose traced detector coordinate the
         one-facet replay does not reproduce (photons that met more than one
         facet or detector in the trace).  Returns the number excluded."""
+        self._pb = None
         y = self.coordinates(self.geometry(self.theta0, self.period0))
         used = self.weight != 0
         bad = used & ~(torch.abs(y - y_traced) <= rtol * torch.abs(y_traced).clamp(min=1.0))
         self.weight = torch.where(bad, torch.zeros_like(self.weight), self.weight).contiguous()
         return int(bad.sum())
 
-    def loss(self, theta, period, precision=64):
-        """L(theta, period) = centroid + RMS width (differentiable)."""
-        geom = self.geometry(theta, period)
-        return FacetLoss.apply(geom, self, precision)
+    def _pose_args(self, mode):
+        pb = self._pb
+        a = _lib.MxPoseArgs()
+        a.n_facet = self.F
+        a.mode = mode
+        for k in ('theta', 'period', 'gu', 't0', 'rpost', 'zoom', 'extra', 'geom', 'grad13',
+                  'grad7', 'moments', 'loss'):
+            setattr(a, k, pb[k].data_ptr())
+        a.y_shift = self.y_shift
+        return a
+
+    def launch_loss_and_grad(self, precision=32, stream=None):
+        """Launch the fused evaluation on the parameters held in
+        ``buffers['theta']``/``['period']`` (asynchronous): pose chain, forward
+        replay with moments, adjoint with seeds from those moments, chain rule
+        back to the poses.  Results: ``buffers['loss']`` and
+        ``buffers['grad7']`` (d/d theta, d/d period)."""
+        pb = self._pose_buffers()
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        pb['moments'].zero_()
+        pb['grad13'].zero_()
+        _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(self._pose_args(0)), ctypes.c_void_p(stream)))
+        fwd = self._args(pb['geom'], precision, moments=pb['moments'], src=pb['src'])
+        fwd.y_shift = self.y_shift
+        _launch(fwd, stream)
+        bwd = self._args(pb['geom'], precision, grad=pb['grad13'], src=pb['src'])
+        bwd.seed_moments = pb['moments'].data_ptr()
+        bwd.y_shift = self.y_shift
+        _launch(bwd, stream)
+        _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(self._pose_args(1)), ctypes.c_void_p(stream)))
+
+    @property
+    def buffers(self):
+        return self._pose_buffers()
+
+    def loss_and_grad(self, theta, period, precision=32):
+        """L = centroid + RMS width and its gradient with respect to the 6-DOF
+        poses and periods, without autograd: returns (L, dL/dtheta (F, 6),
+        dL/dperiod (F,)), all on the device."""
+        pb = self._pose_buffers()
+        pb['theta'].copy_(theta)
+        pb['period'].copy_(period)
+        self.launch_loss_and_grad(precision)
+        return pb['loss'][0].clone(), pb['grad7'][:, :6].clone(), pb['grad7'][:, 6].clone()
+
+    def capture(self, precision=32):
+        """hipGraph of :meth:`launch_loss_and_grad` (replay after writing new
+        parameters into ``buffers['theta']``/``['period']``)."""
+        self._pose_buffers()
+        torch.cuda.synchronize(self.device)
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            self.launch_loss_and_grad(precision, s.cuda_stream)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.launch_loss_and_grad(precision, s.cuda_stream)
+        return g
 
 
 def _moments(y, w):

@@ -43,7 +43,8 @@ def test_struct_layout_matches_header(lib, tmp_path):
     fields = {'MxTables': [f[0] for f in lib.MxTables._fields_],
               'MxPhotons': [f[0] for f in lib.MxPhotons._fields_],
               'MxAdjointArgs': [f[0] for f in lib.MxAdjointArgs._fields_],
-              'MxBatch': [f[0] for f in lib.MxBatch._fields_]}
+              'MxBatch': [f[0] for f in lib.MxBatch._fields_],
+              'MxPoseArgs': [f[0] for f in lib.MxPoseArgs._fields_]}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "marxs_hip.h"',
            'int main(void) {']
     for st, fs in fields.items():

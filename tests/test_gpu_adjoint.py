@@ -112,3 +112,66 @@ def test_fp32_adjoint_matches_fp64(problem):
         err = float((torch.abs(grads[32][:, k] - grads[64][:, k]) / scale).max())
         print(k, float(scale), err, tol)
         assert err <= tol, (k, err, tol)
+
+
+def _autograd(prob, th0, pe0, prec):
+    import torch
+    th = th0.clone().requires_grad_(True)
+    pe = pe0.clone().requires_grad_(True)
+    L = prob.loss(th, pe, precision=prec)
+    L.backward()
+    return L.detach(), th.grad, pe.grad
+
+
+def test_fused_loss_and_grad_match_autograd(problem):
+    """The fused path (pose kernel, forward replay with moments, adjoint with
+    seeds from the moments, pose chain rule in HIP) equals the autograd path
+    in FP64, and its FP32 version stays within the FP32 seed tolerance."""
+    import torch
+    prob, full, sel = problem
+    rng = torch.Generator(device=prob.device).manual_seed(3)
+    th = prob.theta0 + 1e-3 * torch.randn(prob.theta0.shape, generator=rng, dtype=torch.float64,
+                                          device=prob.device) * torch.tensor([50., 50, 50, .1, .1, .1],
+                                                                            dtype=torch.float64,
+                                                                            device=prob.device)
+    pe = prob.period0.clone()
+    La, gta, gpa = _autograd(prob, th, pe, 64)
+    Lf, gtf, gpf = prob.loss_and_grad(th, pe, precision=64)
+    assert abs(float(Lf - La)) <= 1e-10 * abs(float(La))
+    for a, b in ((gta, gtf), (gpa[:, None], gpf[:, None])):
+        for k in range(a.shape[1]):
+            scale = float(torch.abs(a[:, k]).max())
+            assert float(torch.abs(a[:, k] - b[:, k]).max()) <= 1e-9 * scale, k
+    L32, gt32, gp32 = prob.loss_and_grad(th, pe, precision=32)
+    assert abs(float(L32 - La)) <= 1e-6 * abs(float(La))
+    for a, b in ((gta, gt32), (gpa[:, None], gp32[:, None])):
+        for k in range(a.shape[1]):
+            scale = float(torch.abs(a[:, k]).max())
+            assert float(torch.abs(a[:, k] - b[:, k]).max()) <= 2e-3 * scale, k
+
+
+def test_fused_graph_replays_follow_the_parameters(problem):
+    import torch
+    prob, full, sel = problem
+    g = prob.capture(precision=64)
+    buf = prob.buffers
+    for rep in range(3):
+        th = prob.theta0.clone()
+        th[:, 0] += 0.05 * rep
+        th[:, 5] += 1e-4 * rep
+        buf['theta'].copy_(th)
+        buf['period'].copy_(prob.period0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = (buf['loss'][0].item(), buf['grad7'].clone())
+        # eager evaluation in between replays
+        L, gt, gp = prob.loss_and_grad(th, prob.period0, precision=64)
+        assert got[0] == L.item()
+        # gradients are sums of atomic adds: equal up to summation order
+        ref = torch.cat([gt, gp[:, None]], 1)
+        scale = torch.abs(ref).max(0).values
+        assert bool((torch.abs(got[1] - ref) <= 1e-12 * scale).all())
+        if rep:
+            assert not torch.equal(got[1], prev)
+        prev = got[1]
