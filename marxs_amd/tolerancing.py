@@ -9,14 +9,35 @@ with the random generator passed in instead of numpy's global stream.
 (``tolerancing.py:224-283``); ``run_tolerances_batched`` traces all variants
 of the same instrument in one batched GPU launch (``marxs_amd.batch``).
 """
+import collections.abc
 from copy import copy
+from functools import wraps
 
 import numpy as np
 
 from .mathutils import compose, euler2mat_sxyz
 
-__all__ = ['generate_facet_uncertainty', 'wiggle', 'varyperiod', 'run_tolerances',
-           'run_tolerances_batched']
+__all__ = ['oneormoreelements', 'generate_facet_uncertainty', 'wiggle', 'moveglobal',
+           'moveindividual', 'moveelem', 'varyperiod', 'varyorderselector', 'varyattribute',
+           'run_tolerances', 'run_tolerances_batched', 'generate_6d_wigglelist', 'reset_6d',
+           'select_1dof_changed']
+
+
+def oneormoreelements(func):
+    """Let a function written for one element accept a list of elements
+    (tolerancing.py:26-40)."""
+    @wraps(func)
+    def wrapper(elements, *args, **kwargs):
+        if isinstance(elements, collections.abc.Iterable):
+            for e in elements:
+                func(e, *args, **kwargs)
+        else:
+            func(elements, *args, **kwargs)
+    return wrapper
+
+
+def _move(dx, dy, dz, rx, ry, rz):
+    return compose([dx, dy, dz], euler2mat_sxyz(rx, ry, rz), np.ones(3))
 
 
 def generate_facet_uncertainty(n, xyz, angle_xyz, rng=None, trans_offset=0, rot_offset=0):
@@ -27,22 +48,99 @@ def generate_facet_uncertainty(n, xyz, angle_xyz, rng=None, trans_offset=0, rot_
             for t, a in zip(translation, rotation)]
 
 
+@oneormoreelements
 def wiggle(e, dx=0, dy=0, dz=0, rx=0., ry=0., rz=0., rng=None):
-    """Random per-element misalignment of a Parallel container."""
+    """Random per-element misalignment of a Parallel container
+    (Gaussian sigmas in mm and rad; tolerancing.py:44-58)."""
     e.elem_uncertainty = generate_facet_uncertainty(len(e.elements), [dx, dy, dz],
                                                     [rx, ry, rz], rng=rng)
     e.generate_elements()
 
 
-def varyperiod(elements, period_mean, period_sigma, rng=None):
-    """Gaussian grating periods, one draw per element."""
+@oneormoreelements
+def moveglobal(e, dx=0, dy=0, dz=0, rx=0., ry=0., rz=0.):
+    """Move and rotate the whole Parallel container about its own origin
+    (tolerancing.py:62-78)."""
+    e.uncertainty = _move(dx, dy, dz, rx, ry, rz)
+    e.generate_elements()
+
+
+@oneormoreelements
+def moveindividual(e, dx=0, dy=0, dz=0, rx=0, ry=0, rz=0):
+    """Move and rotate every element of a Parallel container by the same
+    amount, each about its own centre (tolerancing.py:81-103)."""
+    e.elem_uncertainty = [_move(dx, dy, dz, rx, ry, rz)] * len(e.elements)
+    e.generate_elements()
+
+
+@oneormoreelements
+def moveelem(e, dx=0, dy=0, dz=0, rx=0., ry=0., rz=0.):
+    """Move and rotate one element; its original pose is kept as
+    ``geometry.pos4d_orig`` (tolerancing.py:106-132)."""
+    if not hasattr(e.geometry, 'pos4d_orig'):
+        e.geometry.pos4d_orig = e.geometry.pos4d.copy()
+    e.geometry.pos4d = _move(dx, dy, dz, rx, ry, rz) @ e.geometry.pos4d_orig
+
+
+@oneormoreelements
+def varyattribute(element, **kwargs):
+    """Set existing attributes of an element (tolerancing.py:135-170)."""
+    for key, val in kwargs.items():
+        if not hasattr(element, key):
+            raise ValueError(f'Object {element} does not have {key} attribute.')
+        setattr(element, key, val)
+
+
+@oneormoreelements
+def varyperiod(el, period_mean, period_sigma, rng=None):
+    """Gaussian grating period, one draw per element
+    (tolerancing.py:175-201)."""
     rng = np.random.default_rng() if rng is None else rng
-    if not isinstance(elements, (list, tuple)):
-        elements = [elements]
-    for el in elements:
-        if not hasattr(el, '_d'):
-            raise ValueError(f'Object {el} does not have grating period `_d` attribute.')
-        el._d = rng.normal(period_mean, period_sigma)
+    if not hasattr(el, '_d'):
+        raise ValueError(f'Object {el} does not have grating period `_d` attribute.')
+    el._d = rng.normal(period_mean, period_sigma)
+
+
+@oneormoreelements
+def varyorderselector(element, order_selector, *args, **kwargs):
+    """Replace the order selector of a grating (tolerancing.py:204-221)."""
+    if not hasattr(element, 'order_selector'):
+        raise ValueError(f'Object {element} does not have an order_selector attribute.')
+    element.order_selector = order_selector(*args, **kwargs)
+
+
+def generate_6d_wigglelist(trans, rot, names=('dx', 'dy', 'dz', 'rx', 'ry', 'rz')):
+    """Parameter lists changing one degree of freedom at a time
+    (tolerancing.py:454-516).  ``trans`` in mm and ``rot`` in rad (plain
+    numbers; the first entry of each should be 0).  Returns (changes in both
+    directions for the movers, one-sided changes for ``wiggle``)."""
+    trans = np.asarray(trans, dtype=float)
+    rot = np.asarray(rot, dtype=float)
+    if trans[0] != 0 or rot[0] != 0:
+        import warnings
+        warnings.warn('First element of trans and rot should be 0.')
+    n_t, n_r = len(trans), len(rot)
+    n = 3 * n_t
+    g = np.zeros((2 * 3 * n_t + 2 * 3 * n_r, 6))
+    for i in range(3):
+        g[i * n_t:(i + 1) * n_t, i] = trans
+        g[n + i * n_r:n + (i + 1) * n_r, i + 3] = rot
+    half = g.shape[0] // 2
+    g[half:, :] = -g[:half, :]
+    ind = g[:half, :]
+    g = np.unique(g, axis=0)
+    ind = np.unique(ind, axis=0)
+    return ([dict(zip(names, row)) for row in g], [dict(zip(names, row)) for row in ind])
+
+
+reset_6d = {'dx': 0., 'dy': 0., 'dz': 0., 'rx': 0., 'ry': 0., 'rz': 0.}
+
+
+def select_1dof_changed(rows, par, parlist=('dx', 'dy', 'dz', 'rx', 'ry', 'rz')):
+    """Rows (dicts, e.g. the output of run_tolerances) in which every
+    parameter of ``parlist`` except ``par`` is zero (tolerancing.py:523-548)."""
+    others = set(parlist) - {par}
+    return [r for r in rows if all(r[p] == 0 for p in others)]
 
 
 def run_tolerances(photons_in, instrum, wigglefunc, wiggleparts, parameters, analyzefunc):
