@@ -12,7 +12,8 @@ energy2wave = 1.2398419292004202e-06   # keV -> mm (marxs/__init__.py:14)
 from .table import PhotonTable, generate_test_photons  # noqa: E402
 from .base import (GeometryError, SimulationSetupError,  # noqa: E402
                    SimulationSetupWarning)
-from .optics import (OrderSelector, EfficiencyFile, FlatGrating, CATGrating,  # noqa: E402
+from .optics import (OrderSelector, EfficiencyFile, InterpolateEfficiencyTable,  # noqa: E402
+                     FlatGrating, CATGrating,
                      NonParallelCATGrating, CircleAperture, RectangleAperture,
                      PerfectLens, ThinLens, FlatDetector, FlatOpticalElement,
                      OpticalElement)

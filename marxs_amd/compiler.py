@@ -79,9 +79,11 @@ def leaf_kind(e):
     if isinstance(e, O.FlatGrating):
         if callable(e._d):
             raise CompileError('gratings with a callable period d are not compiled')
-        if not isinstance(e.order_selector, (O.OrderSelector, O.EfficiencyFile)):
-            raise CompileError('order_selector must be an OrderSelector or EfficiencyFile '
-                               'to run on the GPU, got {0!r}'.format(e.order_selector))
+        if not isinstance(e.order_selector, (O.OrderSelector, O.EfficiencyFile,
+                                             O.InterpolateEfficiencyTable)):
+            raise CompileError('order_selector must be an OrderSelector, EfficiencyFile or '
+                               'InterpolateEfficiencyTable to run on the GPU, got '
+                               '{0!r}'.format(e.order_selector))
         return STAGE_GRATING
     if isinstance(e, O.FlatDetector):
         return STAGE_DETECTOR
@@ -520,7 +522,8 @@ def _build_lds(tables, budget32=None, budget64=None):
             need += _align(8 * (2 * nr + nr * no)) + _align(4 * no)
         n_sel = tables.sel_i.shape[0]
         need += _align(4 * SELI * n_sel)
-        if need > budget:
+        if need > budget or np.any(tables.sel_i[:, 8] & 8):
+            # interpolated tables stay in global memory (and with them all)
             sel_ok = False
         col = 6 if prec == 32 else 7
         tables.sel_i[:, col] = -1
@@ -587,9 +590,39 @@ def _build_lds(tables, budget32=None, budget64=None):
     tables.lds_image64 = images[64]
 
 
+def _interp_row(sel, sel_d, sel_o):
+    """Selector record of an interpolated efficiency table: grid sizes,
+    offsets of wavelength (nm), blaze-angle (rad) and probability
+    ([order][wave][angle]) blocks in sel_d, of the orders in sel_o; flag 8."""
+    if getattr(sel, 'k', 1) != 1:
+        raise CompileError('interpolated efficiency tables are traced with degree-1 '
+                           '(bilinear) interpolation only; got k=%r' % sel.k)
+    wave, theta, orders, data = sel.interp_table()
+    wave = np.asarray(wave, dtype=float)
+    theta = np.asarray(theta, dtype=float)
+    data = np.asarray(data, dtype=float)
+    nw, nt, no = len(wave), len(theta), len(orders)
+    if nw < 2 or nt < 2 or np.any(np.diff(wave) <= 0) or np.any(np.diff(theta) <= 0):
+        raise CompileError('interpolation grids need >= 2 strictly increasing points')
+    if np.any(np.abs(np.asarray(orders)) >= 2 ** 31 - 1):
+        raise CompileError('order numbers must fit in int32')
+    row = [nw, no, len(sel_d), 0, 0, len(sel_o), -1, -1, 8, nt, 0, 0]
+    sel_d.extend(wave.tolist())
+    row[4] = len(sel_d)
+    sel_d.extend(theta.tolist())
+    row[3] = len(sel_d)
+    sel_d.extend(data.ravel().tolist())
+    sel_o.extend(np.asarray(orders, dtype=np.int64).tolist())
+    return row
+
+
 def _selector_row(sel, sel_index, sel_rows, sel_d, sel_o):
     key = id(sel)
     if key in sel_index:
+        return sel_index[key]
+    if hasattr(sel, 'interp_table'):
+        sel_rows.append(_interp_row(sel, sel_d, sel_o))
+        sel_index[key] = len(sel_rows) - 1
         return sel_index[key]
     energy, cdf, tot, orders = sel.table()
     nr, no = cdf.shape

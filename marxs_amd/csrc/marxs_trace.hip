@@ -290,11 +290,74 @@ __device__ __forceinline__ bool select_order_t(HP hd, DP eg, DP cdf, DP tot, IP 
     return true;
 }
 
+// Interpolated efficiency table (InterpolateEfficiencyTable,
+// marxs/missions/mitsnl/catgrating.py:97-144, degree-1 spline): per order the
+// bilinear value at (h c / E in nm, blaze), arguments clamped to the grid, the
+// four terms added in FITPACK's order; the first order whose cumulative sum
+// divided by the total exceeds u, the first order when none does; factor =
+// total.  hd[0] = n_wave, hd[1] = n_orders, hd[9] = n_angle; offsets hd[2]
+// wavelengths, hd[4] angles, hd[3] probabilities [order][wave][angle], hd[5]
+// orders.
+#define MX_KEV_NM 1.2398419843320026
+__device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, double &h0, double &h1) {
+    x = x < g[0] ? g[0] : (x > g[n - 1] ? g[n - 1] : x);
+    int lo = 0, hi = n - 1;               // largest i <= n - 2 with g[i] <= x
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (g[mid] <= x) lo = mid; else hi = mid;
+    }
+    i = lo;
+    const double f = 1.0 / (g[i + 1] - g[i]);
+    h0 = f * (g[i + 1] - x);
+    h1 = f * (x - g[i]);
+}
+
+template <typename HP>
+__device__ __forceinline__ bool select_interp(const MxTables &T, HP hd, double energy, double blaze,
+                                              double u, int &order, double &ptot) {
+    const int nw = hd[0], no = hd[1], nt = hd[9];
+    const double *wg = T.sel_d + hd[2], *tg = T.sel_d + hd[4], *dat = T.sel_d + hd[3];
+    int iw, it;
+    double hx0, hx1, hy0, hy1;
+    hat1(wg, nw, MX_KEV_NM / energy, iw, hx0, hx1);
+    hat1(tg, nt, blaze, it, hy0, hy1);
+    const size_t o00 = (size_t)iw * nt + it, o01 = o00 + 1, o10 = o00 + nt, o11 = o10 + 1;
+    const size_t stride = (size_t)nw * nt;
+    double total = 0.0;
+    for (int m = 0; m < no; ++m) {
+        const double *d = dat + (size_t)m * stride;
+        double sp = 0.0;
+        sp = sp + d[o00] * hx0 * hy0;
+        sp = sp + d[o01] * hx0 * hy1;
+        sp = sp + d[o10] * hx1 * hy0;
+        sp = sp + d[o11] * hx1 * hy1;
+        total = total + sp;
+    }
+    int j = 0;
+    double run = 0.0;
+    for (int m = 0; m < no; ++m) {
+        const double *d = dat + (size_t)m * stride;
+        double sp = 0.0;
+        sp = sp + d[o00] * hx0 * hy0;
+        sp = sp + d[o01] * hx0 * hy1;
+        sp = sp + d[o10] * hx1 * hy0;
+        sp = sp + d[o11] * hx1 * hy1;
+        run = run + sp;
+        if (run / total > u) { j = m; break; }
+    }
+    order = T.sel_o[hd[5] + j];
+    ptot = total;
+    return true;
+}
+
 // Selector sel of the tables.  When the selectors are staged in LDS (all or
-// none), their header rows sit at the start of the LDS image.
-template <int PREC>
+// none; never with an interpolated table), their header rows sit at the start
+// of the LDS image.  blaze_fn() gives the photon's blaze angle (only the
+// interpolated tables need it).
+template <int PREC, typename BF>
 __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double energy, double u,
-                                             int &order, double &ptot, const unsigned char *lds) {
+                                             int &order, double &ptot, const unsigned char *lds,
+                                             BF blaze_fn) {
     if (cp(T.sel_i)[PREC == 64 ? 7 : 6] >= 0) {
         const int *hd = (const int *)lds + (size_t)sel * MX_SELI;
         const int nr = hd[0], no = hd[1];
@@ -304,6 +367,7 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
         return select_order_t(hd, b, b + nr, b + nr + nr * no, o, energy, u, order, ptot);
     }
     const int *hd = T.sel_i + (size_t)sel * MX_SELI;
+    if (hd[8] & 8) return select_interp(T, hd, energy, blaze_fn(), u, order, ptot);
     return select_order_t(hd, T.sel_d + hd[2], T.sel_d + hd[3], T.sel_d + hd[4], T.sel_o + hd[5],
                           energy, u, order, ptot);
 }
@@ -434,6 +498,31 @@ __device__ __forceinline__ void thin_lens64(d3 c, double f, d3 q, d3 &d, d3 &pol
                 m20 * pol.x + m21 * pol.y + m22 * pol.z);
     d = nd;
     pol = np_;
+}
+
+// The blaze angle alone (grating.py:240-249), as diffract64 / diffract32
+// compute it, for order selectors that depend on it.
+__device__ __forceinline__ double blaze64(const double *et, const double *ep, double ly, d3 d) {
+    d3 pp = unit3(d);
+    d3 l = ld3(ep), n = ld3(et + 3);
+    double pl = dot3(pp, l);
+    d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
+    double cb = fabs(dot3(pperp, n));
+    cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
+    double blaze = acos(cb);
+    return blaze + (ep[7] + ly * ep[8]);
+}
+
+template <typename FP>
+__device__ __forceinline__ float blaze32(f3 n, FP pr, float ly, f3 d) {
+#pragma clang fp contract(fast)
+    f3 pp = unitf(d);
+    f3 l = mkf(pr[0], pr[1], pr[2]);
+    float pl = dotf(pp, l);
+    f3 pperp = unitf(mkf(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
+    float cb = fabsf(dotf(pperp, n));
+    cb = cb > 1.0f ? 1.0f : cb;
+    return acosf(cb) + (pr[7] + ly * pr[8]);
 }
 
 // FlatGrating.diffract_photons (marxs/optics/grating.py:233-266), FP64 in
@@ -685,7 +774,9 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             double blaze;
             if (PREC == 64) {
                 id = ei[0];
-                if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) count(cnt, MX_ST_ORDER_EMPTY);
+                if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds,
+                                        [&]() { return blaze64(et, ep, ly, R.d); }))
+                    count(cnt, MX_ST_ORDER_EMPTY);
                 blaze = diffract64(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
             } else {
                 const float wave = MX_E2W_F / (float)R.energy;
@@ -693,14 +784,18 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
                     const float *prm = (const float *)(lds + si[22]) + (size_t)hit * 16;
                     const float *er = (const float *)rec_l + (size_t)hit * MX_ET;
                     id = __float_as_int(prm[9]);
-                    if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds))
+                    if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds,
+                                            [&]() { return (double)blaze32(ldf3(er + 3), prm, (float)ly, R.df); }))
                         count(cnt, MX_ST_ORDER_EMPTY);
                     blaze = (double)diffract32(ldf3(er + 3), prm, wave, (float)ly, m, R.df, R.polf);
                 } else {
                     id = ei[0];
-                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds)) count(cnt, MX_ST_ORDER_EMPTY);
                     float prm[16];
                     for (int k = 0; k < 9; ++k) prm[k] = T.elem_p32[eg * MX_EP + k];
+                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds,
+                                            [&]() { return (double)blaze32(ldf3(T.elem_t32 + eg * MX_ET + 3), prm,
+                                                                           (float)ly, R.df); }))
+                        count(cnt, MX_ST_ORDER_EMPTY);
                     prm[11] = __int_as_float(ei[2]);
                     prm[12] = __int_as_float(ei[3]);
                     blaze = (double)diffract32(ldf3(T.elem_t32 + eg * MX_ET + 3), prm, wave, (float)ly, m,

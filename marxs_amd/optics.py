@@ -138,6 +138,100 @@ class EfficiencyFile:
                 self.totalprob.astype(float), self.orders.astype(np.int64))
 
 
+# keV -> nm, h c / keV with the SI-exact h, c and e (astropy's spectral
+# equivalency, used by marxs/missions/mitsnl/catgrating.py:131)
+KEV_NM = 1.2398419843320026
+
+
+class DataFileFormatException(Exception):
+    """Efficiency table not on a regular, sorted grid (marxs/utils.py:54-56)."""
+
+
+def tablerows_to_2d(x, y, columns):
+    """Regular (x, y) grid from flattened table rows (marxs/utils.py:59-100):
+    x like [1,1,1,2,2,2,...], y like [1,2,3,1,2,3,...].  Returns the unique
+    x, y and each column reshaped to (n_x, n_y)."""
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    n_x, n_y = len(set(x.tolist())), len(set(y.tolist()))
+    if len(x) != n_x * n_y:
+        raise DataFileFormatException('Data is not on regular grid.')
+    xa, ya = x.reshape(n_x, n_y), y.reshape(n_x, n_y)
+    if not (np.allclose(xa, xa[:, :1]) and np.allclose(ya, ya[:1, :])):
+        raise DataFileFormatException('Input table x, y not sorted as expected.')
+    return x[::n_y], y[:n_y], [np.asarray(c, dtype=float).reshape(n_x, n_y) for c in columns]
+
+
+def read_efficiency_table(path):
+    """Read an efficiency table in the ECSV layout of
+    ``marxs/missions/mitsnl/tests/grating_efficiency.csv`` (comment header
+    with units, a row of column names: wavelength, angle, then one column per
+    order; whitespace-separated rows).  Returns (column names, units dict,
+    2-d float array)."""
+    units, names, rows = {}, None, []
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if not t:
+                continue
+            if t.startswith('#'):
+                if 'name:' in t and 'unit:' in t:
+                    nm = t.split('name:')[1].split(',')[0].strip().strip("'")
+                    units[nm] = t.split('unit:')[1].split(',')[0].strip()
+                continue
+            if names is None:
+                names = t.split()
+                continue
+            rows.append([float(v) for v in t.split()])
+    return names, units, np.array(rows, dtype=float)
+
+
+class InterpolateEfficiencyTable:
+    """Order probabilities interpolated in (wavelength, blaze angle)
+    (``marxs/missions/mitsnl/catgrating.py:63-144``).
+
+    ``tab``: either a path to a table file (see :func:`read_efficiency_table`)
+    or ``(names, units, data)`` as returned by it.  Column 0: wavelength in
+    nm, column 1: blaze angle (deg or rad per its unit), then one column per
+    order.  Each order's probability is a tensor-product spline of degree
+    ``k`` through the grid values (``RectBivariateSpline``); for ``k = 1``
+    (the default, the only degree the GPU path implements) that is bilinear
+    interpolation, constant beyond the grid edges.  Order selection: first
+    order whose cumulative probability (normalised to 1) exceeds the draw,
+    the first order when none does; probability factor = sum over orders.
+    """
+
+    def __init__(self, tab, k=1):
+        if isinstance(tab, str):
+            tab = read_efficiency_table(tab)
+        names, units, data = tab
+        wave, theta, cols = tablerows_to_2d(data[:, 0], data[:, 1], data[:, 2:].T)
+        if units.get(names[1], 'deg') == 'deg':
+            theta = np.deg2rad(theta)
+        if units.get(names[0], 'nm') != 'nm':
+            raise ValueError('wavelength column must be in nm')
+        self.k = int(k)
+        self.wave = np.asarray(wave, dtype=float)
+        self.theta = np.asarray(theta, dtype=float)
+        self.orders = np.array([int(n) for n in names[2:]])
+        self.data = np.stack(cols)          # (n_orders, n_wave, n_theta)
+
+    @classmethod
+    def from_arrays(cls, wave_nm, theta_rad, orders, data, k=1):
+        self = cls.__new__(cls)
+        self.k = int(k)
+        self.wave = np.asarray(wave_nm, dtype=float)
+        self.theta = np.asarray(theta_rad, dtype=float)
+        self.orders = np.asarray(orders).astype(int)
+        self.data = np.asarray(data, dtype=float)
+        if self.data.shape != (len(self.orders), len(self.wave), len(self.theta)):
+            raise ValueError('data must have shape (n_orders, n_wave, n_theta)')
+        return self
+
+    def interp_table(self):
+        return self.wave, self.theta, self.orders.astype(np.int64), self.data
+
+
 # ----------------------------------------------------------------- gratings
 class FlatGrating(FlatOpticalElement):
     """Thin flat grating (``grating.py:99-277``).

@@ -32,6 +32,10 @@ def _selector(sel):
     if name == 'EfficiencyFile':
         return {'type': 'efficiency', 'energy': np.asarray(sel.energy, dtype=float),
                 'prob': np.asarray(sel.prob, dtype=float), 'orders': np.asarray(sel.orders)}
+    if name == 'InterpolateEfficiencyTable':
+        return {'type': 'interp', 'wave': np.asarray(sel.wave, dtype=float),
+                'theta': np.asarray(sel.theta, dtype=float), 'orders': np.asarray(sel.orders),
+                'data': np.asarray(sel.data, dtype=float), 'k': int(sel.k)}
     raise TypeError(name)
 
 

@@ -130,6 +130,13 @@ def selector_table(sel):
     renormalised so that the last entry is 1).  ``{'type': 'efficiency',
     'energy', 'prob', 'orders'}`` follows EfficiencyFile (grating.py:78-96).
     """
+    if sel['type'] == 'interp':
+        if int(sel.get('k', 1)) != 1:
+            raise ValueError('only degree-1 interpolation tables are restated')
+        return {'type': 'interp', 'wave': np.asarray(sel['wave'], dtype=float),
+                'theta': np.asarray(sel['theta'], dtype=float),
+                'orders': np.asarray(sel['orders'], dtype=np.int64),
+                'data': np.asarray(sel['data'], dtype=float)}
     if sel['type'] == 'orderselector':
         p = np.asarray(sel['p'], dtype=float)
         p_sum = p.sum()
@@ -144,9 +151,57 @@ def selector_table(sel):
             'orders': np.asarray(sel['orders'], dtype=np.int64)}
 
 
-def select_order(tab, energy, u):
+KEV_NM = 1.2398419843320026   # h c / keV in nm (SI-exact h, c, e)
+
+
+def _hat(grid, x):
+    """Degree-1 B-spline basis on ``grid`` (knots = grid points, doubled at
+    the ends) at x clamped to the grid: interval index i and the two basis
+    values f (g[i+1] - x), f (x - g[i]) with f = 1 / (g[i+1] - g[i]), as
+    FITPACK's fpbspl forms them."""
+    g = np.asarray(grid, dtype=float)
+    x = np.minimum(np.maximum(x, g[0]), g[-1])
+    i = np.clip(np.searchsorted(g, x, side='right') - 1, 0, len(g) - 2)
+    f = 1.0 / (g[i + 1] - g[i])
+    return i, f * (g[i + 1] - x), f * (x - g[i])
+
+
+def interp_probabilities(tab, energy, blaze):
+    """InterpolateEfficiencyTable.probabilities (catgrating.py:97-130) for
+    k = 1: per order the tensor-product degree-1 spline through the grid
+    values at (wavelength, blaze), constant beyond the edges; terms added in
+    FITPACK's fpbisp order (c00 hx0 hy0, c01 hx0 hy1, c10 hx1 hy0, c11 hx1 hy1).
+    Returns (n_orders, n) probabilities."""
+    wave = KEV_NM / np.asarray(energy, dtype=float)
+    i, hx0, hx1 = _hat(tab['wave'], wave)
+    j, hy0, hy1 = _hat(tab['theta'], np.asarray(blaze, dtype=float))
+    d = tab['data']
+    out = np.empty((d.shape[0], len(wave)))
+    for m in range(d.shape[0]):
+        sp = np.zeros(len(wave))
+        sp = sp + d[m, i, j] * hx0 * hy0
+        sp = sp + d[m, i, j + 1] * hx0 * hy1
+        sp = sp + d[m, i + 1, j] * hx1 * hy0
+        sp = sp + d[m, i + 1, j + 1] * hx1 * hy1
+        out[m] = sp
+    return out
+
+
+def select_order(tab, energy, u, blaze=None):
     """Nearest energy row (first on ties), then the first order whose
-    cumulative probability exceeds u.  Returns (order, prob, ok)."""
+    cumulative probability exceeds u.  Returns (order, prob, ok).
+
+    For an interpolated table (catgrating.py:132-144) the probabilities come
+    from :func:`interp_probabilities`, the cumulative sum is divided by the
+    total, and the first order is taken when no entry exceeds u (argmax of
+    an all-false row); ``ok`` is then always true."""
+    if tab.get('type') == 'interp':
+        p = interp_probabilities(tab, energy, blaze)
+        total = np.sum(p, axis=0)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            cum = np.cumsum(p, axis=0) / total
+        j = np.argmax(cum > u[None, :], axis=0)
+        return tab['orders'][j], total, np.ones(len(u), dtype=bool)
     if tab['energy'] is None or len(tab['energy']) == 1:
         row = np.zeros(len(energy), dtype=int)
     else:
@@ -166,6 +221,17 @@ def groove_frame(fr, groove_angle):
     e_groove = math.sin(-groove_angle) * ex + math.cos(-groove_angle) * ey
     e_perp = math.cos(-groove_angle) * ex - math.sin(-groove_angle) * ey
     return e_groove, e_perp, en
+
+
+def blaze_angle(el, fr, dir, ly):
+    """The blaze angle of grating.py:240-249 (as computed in :func:`diffract`)."""
+    p = unit(dir)
+    l, e_perp, n = groove_frame(fr, el.get('groove_angle', 0.))
+    with np.errstate(invalid='ignore', divide='ignore'):
+        pl = dot(p, l)
+        pperp = unit(p - pl[:, None] * l[None, :])
+        blaze = np.arccos(np.clip(np.abs(dot(pperp, n)), 0, 1))
+        return blaze + (el.get('blaze_center', 0.) + ly * el.get('d_blaze_mm', 0.))
 
 
 def diffract(el, fr, dir, energy, ly, order, ptot):
@@ -361,7 +427,8 @@ def trace(elements, photons, uniforms, n_draws=2):
                 slot = np.where(k < nslot, k, k % nslot)
                 u = uniforms[base + slot, np.nonzero(hit)[0]]
                 tab = selector_table(el['selector'])
-                order, ptot, ok = select_order(tab, ph['energy'][hit], u)
+                blz = blaze_angle(el, fr, d_old, ly) if tab.get('type') == 'interp' else None
+                order, ptot, ok = select_order(tab, ph['energy'][hit], u, blz)
                 if not ok.all():
                     raise ValueError('no order selected')
                 new, blaze = diffract(el, fr, d_old, ph['energy'][hit], ly, order, ptot)

@@ -10,6 +10,7 @@ import pytest
 
 from marxs_amd import configs
 import marxs_amd as mx
+from marxs_amd.mathutils import axangle2mat
 from tests.helpers import (gpu_trace, oracle_trace, compare, make_inputs,
                            near_pixel_boundary)
 
@@ -272,3 +273,34 @@ def test_efficiency_row_search_ties_and_ends(gpu, grid):
     grat = mx.FlatGrating(d=1e-3, order_selector=eff, zoom=[1, 20, 20])
     ref, got, st = _single([grat], ph, u[:2])
     assert np.array_equal(ref['order'], got['order'])
+
+
+def _interp_table():
+    import os
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'grating_efficiency.csv')
+    return mx.InterpolateEfficiencyTable(path)
+
+
+def test_interpolated_efficiency_table_parity(gpu):
+    """InterpolateEfficiencyTable (catgrating.py:63-144, degree 1) through the
+    FP64 kernel: orders, probabilities and blaze angles against the oracle,
+    for wavelengths and blaze angles inside and beyond the table."""
+    ph, u = _photons(6000, 11, spread=0.04)
+    ph['energy'] = np.random.default_rng(12).uniform(0.3, 2.0, len(ph['energy']))
+    # the table scaled to totals below 1 (its rows summing to exactly 1 can
+    # interpolate to 1 + 1 ulp, which the reference rejects as a probability)
+    t = _interp_table()
+    t = mx.InterpolateEfficiencyTable.from_arrays(t.wave, t.theta, t.orders, 0.9 * t.data)
+    grat = mx.CATGrating(d=2e-4, order_selector=t, zoom=[1, 40, 40],
+                         orientation=axangle2mat([0, 0, 1], np.deg2rad(1.5)))
+    ref, got, st = _single([grat], ph, u[:2])
+    assert np.array_equal(ref['order'], got['order'])
+    assert len(set(got['order'][got['order'] > -100].tolist())) == 3
+
+
+def test_interpolated_table_in_use(gpu):
+    """test_cat.py:108-117: test photons (blaze 0, clamped to the 1 deg row)
+    at 1 keV go to order 0 with probability 0.5."""
+    cat = mx.CATGrating(order_selector=_interp_table(), d=0.001)
+    p = cat(mx.generate_test_photons(5000))
+    assert np.isclose((np.asarray(p['order']) == 0).sum(), 2500, rtol=.05)

@@ -401,3 +401,71 @@ def test_placement_compose_equals_product():
         rot = PL.euler_sxyz(*rng.uniform(-1, 1, 3))
         exp = PL.translation(t) @ PL.linear(rot) @ PL.linear(np.diag(z))
         assert np.allclose(PL.compose(t, rot, z), exp)
+
+
+# ------------------------------------------- interpolated efficiency table
+def _interp_table(k=1):
+    import os
+    from marxs_amd.optics import InterpolateEfficiencyTable
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'grating_efficiency.csv')
+    return InterpolateEfficiencyTable(path, k=k)
+
+
+def _interp_desc(t):
+    return OT.selector_table({'type': 'interp', 'wave': t.wave, 'theta': t.theta,
+                              'orders': t.orders, 'data': t.data, 'k': t.k})
+
+
+def test_interp_table_known_answers():
+    """marxs/missions/mitsnl/tests/test_cat.py:98-106 (table data file of the
+    reference's tests), at the grid nodes where every spline degree agrees:
+    totals [.8, .8, 1, 1], order-0 probabilities [.3, .3, .2] (the first
+    column there is the first three orders at 0.5 keV, 1 deg)."""
+    t = _interp_table()
+    assert list(t.orders) == [0, -1, -2]
+    p = OT.interp_probabilities(_interp_desc(t), np.array([0.5, 0.5, 1., 1.]),
+                                np.deg2rad([1., 2., 1., 2.]))
+    assert np.allclose(p.sum(axis=0), [.8, .8, 1., 1.], rtol=1e-4)
+    assert np.allclose(p[:, 0], [.3, .3, .2], rtol=1e-4)
+
+
+def test_interp_matches_scipy_rectbivariatespline():
+    """Between and beyond the nodes: the restated degree-1 evaluation against
+    scipy's RectBivariateSpline(kx=1, ky=1).ev (FITPACK), the reference's
+    interpolator (catgrating.py:102,134)."""
+    from scipy.interpolate import RectBivariateSpline
+    rng = np.random.default_rng(4)
+    wave = np.array([0.6, 0.9, 1.3, 2.0, 2.5, 4.1])
+    theta = np.deg2rad([0.5, 1.0, 1.7, 2.2])
+    data = rng.uniform(0, 0.3, (5, len(wave), len(theta)))
+    tab = OT.selector_table({'type': 'interp', 'wave': wave, 'theta': theta,
+                             'orders': np.arange(5), 'data': data, 'k': 1})
+    e = OT.KEV_NM / rng.uniform(0.3, 5., 3000)        # includes both edges
+    b = rng.uniform(0., np.deg2rad(3.), 3000)
+    got = OT.interp_probabilities(tab, OT.KEV_NM / e, b)
+    for m in range(5):
+        ref = RectBivariateSpline(wave, theta, data[m], kx=1, ky=1).ev(e, b)
+        assert np.allclose(got[m], ref, rtol=1e-13, atol=1e-15)
+
+
+def test_interp_order_rule_and_fallback():
+    """catgrating.py:132-144: first order whose normalised cumulative
+    probability exceeds u; the first order when none does (u = 1 can only
+    happen with rounding), probability factor = total."""
+    t = _interp_table()
+    tab = _interp_desc(t)
+    e = np.full(4, 1.0)
+    b = np.deg2rad(np.full(4, 1.))
+    o, prob, ok = OT.select_order(tab, e, np.array([0.1, 0.55, 0.85, 1.0]), b)
+    # probabilities at (1.2398 nm ~ 1 keV, 1 deg): .5, .3, .2
+    assert list(o) == [0, -1, -2, 0]
+    assert ok.all() and np.allclose(prob, 1.0, rtol=1e-4)
+
+
+def test_interp_table_wrong_format():
+    """test_cat.py:120-124: a table with missing rows is rejected."""
+    import os
+    from marxs_amd.optics import InterpolateEfficiencyTable, DataFileFormatException
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'grating_efficiency_broken.csv')
+    with pytest.raises(DataFileFormatException):
+        InterpolateEfficiencyTable(path)
