@@ -455,7 +455,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
 
 
 # LDS image budgets per workgroup (160 KiB of LDS per CU; one workgroup of
-# 768 (FP32) or 512 (FP64) work-items per CU).
+# 1024 (FP32) or 768 (FP64) work-items per CU).
 LDS_BUDGET32 = 150 * 1024
 LDS_BUDGET64 = 150 * 1024
 
