@@ -80,7 +80,14 @@ __device__ __forceinline__ d3 unit3(d3 v) {
 // FinitePlane.intersect, marxs/math/geometry.py:233-253
 __device__ __forceinline__ bool plane_test64(const double *__restrict__ et, d3 p, d3 d,
                                              d3 &q, double &ly, double &lz) {
-    d3 c = ld3(et), ex = ld3(et + 3), ey = ld3(et + 6), ez = ld3(et + 9);
+    // records are 16-byte aligned: 16-byte loads
+    const double2 a0 = ((const double2 *)et)[0], a1 = ((const double2 *)et)[1];
+    const double2 a2 = ((const double2 *)et)[2], a3 = ((const double2 *)et)[3];
+    const double2 a4 = ((const double2 *)et)[4], a5 = ((const double2 *)et)[5];
+    const double2 a6 = ((const double2 *)et)[6];
+    d3 c = mk(a0.x, a0.y, a1.x), ex = mk(a1.y, a2.x, a2.y);
+    d3 ey = mk(a3.x, a3.y, a4.x), ez = mk(a4.y, a5.x, a5.y);
+    const double hy = a6.x, hz = a6.y;
     double kn = dot3(sub3(c, p), ex);
     double kd = dot3(d, ex);
     double k = kn / kd;
@@ -88,7 +95,7 @@ __device__ __forceinline__ bool plane_test64(const double *__restrict__ et, d3 p
     d3 v = sub3(q, c);
     ly = dot3(v, ey);
     lz = dot3(v, ez);
-    return (kd != 0.0) && (k >= 0.0) && (fabs(ly) <= et[12]) && (fabs(lz) <= et[13]);
+    return (kd != 0.0) && (k >= 0.0) && (fabs(ly) <= hy) && (fabs(lz) <= hz);
 }
 
 // The same test for the FP32 path's final detector coordinate: the division
@@ -165,14 +172,19 @@ __device__ __forceinline__ d3 tod(f3 v) { return mk((double)v.x, (double)v.y, (d
 __device__ __forceinline__ bool plane_test32(const float *__restrict__ et, f3 p, f3 d, float kmin,
                                              float &k, float &ly, float &lz) {
 #pragma clang fp contract(fast)
-    f3 c = ldf3(et), ex = ldf3(et + 3), ey = ldf3(et + 6), ez = ldf3(et + 9);
+    // records are 64-byte aligned: four 16-byte loads
+    const float4 r0 = ((const float4 *)et)[0], r1 = ((const float4 *)et)[1];
+    const float4 r2 = ((const float4 *)et)[2], r3 = ((const float4 *)et)[3];
+    f3 c = mkf(r0.x, r0.y, r0.z), ex = mkf(r0.w, r1.x, r1.y);
+    f3 ey = mkf(r1.z, r1.w, r2.x), ez = mkf(r2.y, r2.z, r2.w);
+    const float hy = r3.x, hz = r3.y;
     f3 dp = subf(c, p);
     float kn = dotf(dp, ex);
     float kd = dotf(d, ex);
     k = __fdividef(kn, kd);
     ly = k * dotf(d, ey) - dotf(dp, ey);
     lz = k * dotf(d, ez) - dotf(dp, ez);
-    return (kd != 0.0f) && (k >= kmin) && (fabsf(ly) <= et[12]) && (fabsf(lz) <= et[13]);
+    return (kd != 0.0f) && (k >= kmin) && (fabsf(ly) <= hy) && (fabsf(lz) <= hz);
 }
 
 __device__ __forceinline__ f3 ptrans32(f3 d1, f3 d2, f3 pol) {
