@@ -29,7 +29,9 @@ from .mathutils import decompose44
 __all__ = ['OpticalElement', 'FlatOpticalElement', 'OrderSelector',
            'EfficiencyFile', 'FlatGrating', 'CATGrating',
            'NonParallelCATGrating', 'FlatAperture', 'CircleAperture',
-           'RectangleAperture', 'PerfectLens', 'ThinLens', 'FlatDetector']
+           'RectangleAperture', 'PerfectLens', 'ThinLens', 'FlatDetector', 'FlatStack',
+           'InterpolateEfficiencyTable', 'read_efficiency_table', 'tablerows_to_2d',
+           'DataFileFormatException']
 
 
 class OpticalElement(SimulationSequenceElement):
@@ -230,6 +232,24 @@ class InterpolateEfficiencyTable:
 
     def interp_table(self):
         return self.wave, self.theta, self.orders.astype(np.int64), self.data
+
+
+# ------------------------------------------------------------------ stacks
+class FlatStack(FlatOpticalElement):
+    """Several flat layers acting at one place (``optics/base.py:218-281``):
+    one intersection with the stack's plane, then every layer processes the
+    photons that met it, in order, with the same intersection point and local
+    coordinates.  The stack writes its own local-coordinate and id columns.
+    Layers supported by the GPU path: a grating membrane
+    (``FlatGrating``/``CATGrating``/``NonParallelCATGrating``) first, then
+    ``QualityFactor``, ``L1``, ``L2Abs``, ``RandomGaussianScatter`` /
+    ``L2Diffraction`` (``marxs_amd.catgrating``, ``marxs_amd.scatter``)."""
+
+    def __init__(self, **kwargs):
+        elements = kwargs.pop('elements')
+        keywords = kwargs.pop('keywords')
+        super().__init__(**kwargs)
+        self.elements = [elem(pos4d=self.pos4d, **k) for elem, k in zip(elements, keywords)]
 
 
 # ----------------------------------------------------------------- gratings

@@ -10,8 +10,12 @@ product's compile step.
 import numpy as np
 
 
+def _is_stack(e):
+    return any(c.__name__ == 'FlatStack' for c in type(e).__mro__)
+
+
 def _leaves(tree):
-    if hasattr(tree, 'elements'):
+    if hasattr(tree, 'elements') and not _is_stack(tree):
         out = []
         for e in tree.elements:
             out += _leaves(e)
@@ -39,10 +43,55 @@ def _selector(sel):
     raise TypeError(name)
 
 
+LAYER_TYPES = ('QualityFactor', 'L1', 'L2Abs', 'L2Diffraction', 'RandomGaussianScatter')
+
+
+def _layer(e):
+    """Plain description of one stack layer (or a stand-alone layer)."""
+    name = type(e).__name__
+    loc = e.loc_coos_name
+    if name in ('FlatGrating', 'CATGrating', 'NonParallelCATGrating', 'L1'):
+        g = e.geometry
+        d = {'op': 'l1' if name == 'L1' else 'grating', 'd': e._d,
+             'groove_angle': g['groove_angle'],
+             'sign': 'flat' if name == 'FlatGrating' else 'cat',
+             'transmission': e.transmission, 'selector': _selector(e.order_selector),
+             'order_name': e.order_name, 'blaze_name': e.blaze_name, 'loc': loc}
+        if name == 'NonParallelCATGrating':
+            d['blaze_center'] = e.blaze_center
+            d['d_blaze_mm'] = e.d_blaze_mm
+        if name == 'L1':
+            d['openfraction'] = e.openfraction
+            d['trans_energy'] = np.asarray(e.trans_energy, dtype=float)
+            d['trans_value'] = np.asarray(e.trans_value, dtype=float)
+        return d
+    if name == 'QualityFactor':
+        return {'op': 'qf', 'factor': e.factor, 'loc': loc}
+    if name == 'L2Abs':
+        openfraction = (e.innerfree / e.period) ** 2
+        totalarea = e.period ** 2 / 2 * np.sqrt(3)
+        return {'op': 'l2abs', 'openfraction': openfraction,
+                'coef': e.bardepth * e.innerfree / totalarea, 'loc': loc}
+    if name in ('L2Diffraction', 'RandomGaussianScatter'):
+        if name == 'L2Diffraction':
+            return {'op': 'scatter', 'mode': 'l2', 'innerfree': e.innerfree,
+                    'name': e.scattername, 'loc': loc}
+        return {'op': 'scatter', 'mode': 'const', 'sigma': float(e.scatter),
+                'name': e.scattername, 'loc': loc}
+    raise TypeError(f'oracle has no model of stack layer {name}')
+
+
 def describe(tree):
     out = []
     for e in _leaves(tree):
         name = type(e).__name__
+        if _is_stack(e) or name in LAYER_TYPES:
+            stack = _is_stack(e)
+            out.append({'kind': 'stack', 'pos4d': np.array(e.pos4d, dtype=float),
+                        'id_col': e.id_col, 'id_num': e.id_num,
+                        'loc': e.loc_coos_name if stack else None,
+                        'layers': [_layer(x) for x in (e.elements if stack else [e])]})
+            continue
         g = e.geometry
         d = {'pos4d': np.array(e.pos4d, dtype=float), 'id_col': e.id_col,
              'id_num': e.id_num, 'loc': e.loc_coos_name}

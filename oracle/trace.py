@@ -336,13 +336,44 @@ def pixel_index(detpix, npix):
 
 
 # ------------------------------------------------------------------ driver
+STACK_DRAWS = 5   # draws per stack interaction: order, L1 order, L1 open, normal, angle
+
+
+def stack_loc_pairs(el):
+    """Distinct local-coordinate column pairs of a stack (its own first,
+    then those of its layers in order)."""
+    pairs = []
+    for loc in [el.get('loc')] + [ly.get('loc') for ly in el['layers']]:
+        if loc is not None and tuple(loc) not in pairs:
+            pairs.append(tuple(loc))
+    return pairs
+
+
 def columns_of(el):
     kind = el['kind']
     cols = {'loc0': None, 'loc1': None, 'id': el.get('id_col'), 'order': None,
             'blaze': None, 'detpix0': None, 'detpix1': None, 'pix0': None, 'pix1': None}
+    if kind == 'stack':
+        cols.update({'loc0b': None, 'loc1b': None, 'order2': None, 'blaze2': None, 'scat': None})
+        pairs = stack_loc_pairs(el)
+        if len(pairs) > 2:
+            raise ValueError('more than two local-coordinate column pairs in a stack')
+        if pairs:
+            cols['loc0'], cols['loc1'] = pairs[0]
+        if len(pairs) > 1:
+            cols['loc0b'], cols['loc1b'] = pairs[1]
+        for ly in el['layers']:
+            if ly['op'] == 'grating':
+                cols['order'], cols['blaze'] = ly['order_name'], ly['blaze_name']
+            elif ly['op'] == 'l1':
+                cols['order2'], cols['blaze2'] = ly['order_name'], ly['blaze_name']
+            elif ly['op'] == 'scatter' and not (ly['mode'] == 'const' and ly['sigma'] == 0):
+                cols['scat'] = ly['name']
+        return cols
     default_loc = {'aperture_circle': ['y', 'z'], 'aperture_rect': ['y', 'z'],
                    'lens_perfect': ['mirror_x', 'mirror_y'], 'lens_thin': ['y', 'z'],
-                   'grating': ['grat_y', 'grat_z'], 'detector': ['det_x', 'det_y']}[kind]
+                   'grating': ['grat_y', 'grat_z'], 'detector': ['det_x', 'det_y'],
+                   }[kind]
     loc = el.get('loc', default_loc)
     if loc is not None:
         cols['loc0'], cols['loc1'] = loc
@@ -370,7 +401,9 @@ def uniform_slots(elements, n_draws=2):
         key = (el['kind'], tuple(sorted((k, v) for k, v in columns_of(el).items() if v)))
         new_stage = (prev is None or key != prev or el['kind'] in APERTURES)
         if new_stage:
-            n = 2 if el['kind'] in APERTURES else (n_draws if el['kind'] == 'grating' else 0)
+            n = (2 if el['kind'] in APERTURES else
+                 (n_draws if el['kind'] == 'grating' else
+                  (STACK_DRAWS * n_draws if el['kind'] == 'stack' else 0)))
             base = slot
             slot += n
             stage_id += 1
@@ -381,7 +414,125 @@ def uniform_slots(elements, n_draws=2):
     return out, slot
 
 
-INT_COLS = ('id', 'order', 'pix0', 'pix1')
+INT_COLS = ('id', 'order', 'pix0', 'pix1', 'order2')
+
+
+def axangle_t_apply(axes, angles, v):
+    """einsum('...ij,...i->...j', axangle2mat(axes, angles), v) of the
+    reference's scatter (rotations.py:54-87, scatter.py:133-139): the
+    transposed rotation matrix (axis normalised) applied to v."""
+    axes = axes / np.linalg.norm(axes, axis=1)[:, None]
+    c = np.cos(angles)
+    s = np.sin(angles)
+    C = 1 - c
+    x, y, z = axes[:, 0], axes[:, 1], axes[:, 2]
+    xs, ys, zs = x * s, y * s, z * s
+    xC, yC, zC = x * C, y * C, z * C
+    xyC, yzC, zxC = x * yC, y * zC, z * xC
+    m = [[x * xC + c, xyC - zs, zxC + ys],
+         [xyC + zs, y * yC + c, yzC - xs],
+         [zxC - ys, yzC + xs, z * zC + c]]
+    return np.stack([(m[0][j] * v[:, 0] + m[1][j] * v[:, 1]) + m[2][j] * v[:, 2]
+                     for j in range(3)], axis=1)
+
+
+def transmission(ly, energy):
+    """interp1d(energy, trans) of the L1 layer (catgrating.py:189-192),
+    scipy's linear arithmetic; raises outside the table like interp1d."""
+    x, y = ly['trans_energy'], ly['trans_value']
+    if np.any(energy < x[0]) or np.any(energy > x[-1]):
+        raise ValueError('A value in x_new is outside the interpolation range.')
+    hi = np.clip(np.searchsorted(x, energy), 1, len(x) - 1)
+    lo = hi - 1
+    slope = (y[hi] - y[lo]) / (x[hi] - x[lo])
+    return slope * (energy - x[lo]) + y[lo]
+
+
+def _check_prob(v):
+    if np.any(v < 0) or np.any(v > 1):
+        raise ValueError('Found probability outside of the 0..1 arange.')
+
+
+def stack_layers(el, fr, ph, hit, ly, uniforms, base, nslot, k, cols_out):
+    """FlatStack.process_photons (optics/base.py:258-281): every layer in
+    order on the photons ``hit``, each a process_photons of its own
+    (probability factor checked and multiplied, columns written).  Layer
+    draws: slot 5 k + (0 order, 1 L1 order, 2 L1 open, 3 normal, 4 angle).
+    Returns {column key: values} for the bookkeeping."""
+    from scipy.special import ndtri
+    idx = np.nonzero(hit)[0]
+    energy = ph['energy'][hit]
+
+    def u(j):
+        sl = STACK_DRAWS * k + j
+        sl = np.where(sl < nslot, sl, sl % nslot)
+        return uniforms[base + sl, idx]
+    order_last = None
+    for layer in el['layers']:
+        op = layer['op']
+        d_old, pol_old = ph['dir'][hit].copy(), ph['pol'][hit].copy()
+        if op in ('grating', 'l1'):
+            lel = dict(layer, kind='grating')
+            tab = selector_table(layer['selector'])
+            blz = blaze_angle(lel, fr, d_old, ly) if tab.get('type') == 'interp' else None
+            order, ptot, ok = select_order(tab, energy, u(0 if op == 'grating' else 1), blz)
+            if not ok.all():
+                raise ValueError('no order selected')
+            new, blaze = diffract(lel, fr, d_old, energy, ly, order, ptot)
+            pol = parallel_transport(d_old, new, pol_old)
+            if op == 'l1':
+                through = u(2) > layer['openfraction']
+                new[through] = d_old[through]
+                pol[through] = pol_old[through]
+                order = np.where(through, 0, order)
+                ptot = np.where(through, 0., ptot)
+                if through.any():
+                    ptot[through] = transmission(layer, energy[through])
+                cols_out['order2'] = order
+                cols_out['blaze2'] = blaze
+            else:
+                order_last = order
+                cols_out['order'] = order
+                cols_out['blaze'] = blaze
+            _check_prob(ptot)
+            ph['prob'][hit] = ph['prob'][hit] * ptot
+            ph['dir'][hit] = new
+            ph['pol'][hit] = pol
+        elif op == 'qf':
+            if order_last is None:
+                raise ValueError('QualityFactor needs a grating layer before it')
+            f = layer['factor'] ** (order_last.astype(float) ** 2)
+            _check_prob(f)
+            ph['prob'][hit] = ph['prob'][hit] * f
+        elif op == 'l2abs':
+            p3 = unit(d_old)
+            angle = np.arccos(np.abs(dot(p3, fr['e_x'])))
+            f = layer['openfraction'] - layer['coef'] * np.sin(angle)
+            _check_prob(f)
+            ph['prob'][hit] = ph['prob'][hit] * f
+        elif op == 'scatter':
+            if layer['mode'] == 'const' and layer['sigma'] == 0:
+                continue
+            if layer['mode'] == 'l2':
+                wave = KEV_NM * 1e-6 / energy
+                sigma = 1.22 * 0.4 * np.arcsin(wave / layer['innerfree'])
+            else:
+                sigma = layer['sigma']
+            angle = ndtri(u(3)) * sigma
+            pdir = unit(d_old)
+            guess = np.zeros_like(pdir)
+            ind = np.abs(pdir[:, 0]) < 0.99999
+            guess[ind, 0] = 1
+            guess[~ind, 1] = 1
+            perp = np.cross(pdir, guess)
+            out = axangle_t_apply(perp, angle, pdir)
+            angle2 = u(4) * 2 * np.pi
+            out = axangle_t_apply(pdir, angle2, out)
+            ph['pol'][hit] = parallel_transport(d_old, out, pol_old)
+            ph['dir'][hit] = out
+            cols_out['scat'] = angle
+        else:
+            raise ValueError(op)
 
 
 def trace(elements, photons, uniforms, n_draws=2):
@@ -445,6 +596,21 @@ def trace(elements, photons, uniforms, n_draws=2):
                 new, pol = thin_lens(el, fr, d_old, q, pol_old)
                 out['dir'] = new
                 out['polarization'] = pol
+            elif kind == 'stack':
+                # photons meeting the stack for the k-th time in this stage use
+                # draws 5 k ... 5 k + 4: process each multiplicity in turn
+                k = count[hit]
+                stack_out = {}
+                for kk in np.unique(k):
+                    sub = hit.copy()
+                    sub[hit] = k == kk
+                    sub_out = {}
+                    stack_layers(el, fr, ph, sub, ly[k == kk], uniforms, base, nslot, int(kk), sub_out)
+                    for key, v in sub_out.items():
+                        arr = stack_out.setdefault(key, np.zeros(int(hit.sum()),
+                                                                 dtype=np.asarray(v).dtype))
+                        arr[k == kk] = v
+                out.update(stack_out)
             elif kind == 'detector':
                 npix, cpix = detector_params(el)
                 dp0 = ly / el['pixsize'] + cpix[0]
@@ -467,7 +633,10 @@ def trace(elements, photons, uniforms, n_draws=2):
         if 'polarization' in out:
             ph['pol'][hit] = out['polarization']
         extra = {'loc0': ly, 'loc1': lz, 'id': el.get('id_num', -9)}
-        for key in ('order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1'):
+        if kind == 'stack':
+            extra['loc0b'], extra['loc1b'] = ly, lz
+        for key in ('order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1', 'order2',
+                    'blaze2', 'scat'):
             if key in out:
                 extra[key] = out[key]
         for key, val in extra.items():
@@ -476,7 +645,8 @@ def trace(elements, photons, uniforms, n_draws=2):
                 continue
             if name not in cols:
                 if key in INT_COLS:
-                    cols[name] = np.full(n, ORDER_NONE if key == 'order' else -1, dtype=np.int64)
+                    cols[name] = np.full(n, ORDER_NONE if key in ('order', 'order2') else -1,
+                                         dtype=np.int64)
                     int_names[name] = key
                 else:
                     cols[name] = np.full(n, np.nan)

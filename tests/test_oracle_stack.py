@@ -1,0 +1,133 @@
+"""Oracle restatement of stacked layers (CAT grating support, scatter) on
+the CPU, against the reference's known answers
+(marxs/missions/mitsnl/tests/test_cat.py, marxs/optics/tests/test_scatter.py)
+and closed forms."""
+import os
+
+import numpy as np
+import pytest
+
+import marxs_amd as mx
+from marxs_amd import catgrating as CG
+from marxs_amd.scatter import RandomGaussianScatter
+from marxs_amd.mathutils import euler2mat_sxyz
+from oracle import trace as OT
+from oracle.describe import describe
+
+SI = os.path.join(os.path.dirname(__file__), 'golden', 'SiTransmission.csv')
+
+
+def _ph(t):
+    return {'pos': np.asarray(t['pos'], dtype=float)[:, :3].copy(),
+            'dir': np.asarray(t['dir'], dtype=float)[:, :3].copy(),
+            'pol': np.asarray(t['polarization'], dtype=float)[:, :3].copy(),
+            'energy': np.asarray(t['energy'], dtype=float),
+            'prob': np.asarray(t['probability'], dtype=float)}
+
+
+def run(elements, photons, seed=0, n_draws=2):
+    els = describe(elements)
+    n = len(photons['dir'])
+    n_slots = OT.uniform_slots(els, n_draws)[1]
+    u = np.random.default_rng(seed).random((n_slots, n))
+    return OT.trace(els, photons, u, n_draws=n_draws)
+
+
+def test_check_lx_dims():
+    """test_cat.py:31-35"""
+    with pytest.raises(ValueError):
+        CG.check_lx_dims({'barwidth': 1, 'period': 1})
+
+
+def test_si_transparency_scaling():
+    """test_cat.py:38-48: 4 um of Si at 1 keV transmits 0.22458 (CXRO); with
+    an open fraction of 0.1 most test photons go through the bars."""
+    cat = CG.L1(order_selector=mx.OrderSelector([-5]), l1_transmission=SI,
+                l1_dims={'bardepth': 0.004, 'period': 0.0001, 'barwidth': 0.00009})
+    out = run(cat, _ph(mx.generate_test_photons(100)))
+    assert np.isclose(np.median(out['prob']), 0.22458, rtol=1e-4)
+    assert set(out['order_L1'].tolist()) <= {0, -5}
+
+
+def test_transmission_matches_scipy_interp1d():
+    """The L1 table interpolation is scipy's interp1d arithmetic."""
+    from scipy.interpolate import interp1d
+    cat = CG.L1(order_selector=mx.OrderSelector([1]), l1_transmission=SI)
+    e = np.random.default_rng(1).uniform(0.05, 20., 5000)
+    ref = interp1d(cat.trans_energy, cat.trans_value)(e)
+    lay = describe(cat)[0]['layers'][0]
+    assert np.array_equal(OT.transmission(lay, e), ref)
+    assert np.array_equal(cat.transfunc(e), ref)
+
+
+def test_l2_abs_known_answer_and_angle():
+    """test_cat.py:63-90: 0.81 on axis; the shadow grows with the angle."""
+    dims = {'period': 0.966, 'bardepth': 0.5, 'barwidth': 0.1}
+    p1 = run(CG.L2Abs(l2_dims=dims), _ph(mx.generate_test_photons(1)))
+    assert np.isclose(p1['prob'][0], 0.81, rtol=0.02)
+    rot = euler2mat_sxyz(0, 0, np.deg2rad(1.8))
+    p2 = run(CG.L2Abs(l2_dims=dims, orientation=rot), _ph(mx.generate_test_photons(1)))
+    assert p2['prob'][0] < p1['prob'][0]
+
+
+def test_l2_broadening():
+    """test_cat.py:51-61: L2 diffraction at 0.25 keV broadens by ~0.5 arcsec."""
+    t = mx.generate_test_photons(1000)
+    t['energy'] = np.full(1000, 0.25)
+    l2 = CG.L2Diffraction(l2_dims={'period': 0.966, 'barwidth': 0.1})
+    det = mx.FlatDetector(position=[-1, 0, 0])
+    out = run(mx.Sequence(elements=[l2, det]), _ph(t))
+    assert np.isclose(np.std(out['det_x']), .5 * 4.8e-6, rtol=.5)
+    assert 'L2Diffraction' in out
+
+
+def test_gaussian_scatter_distribution_and_zero():
+    """test_scatter.py:77-110: sigma 0 changes nothing and adds no column; a
+    small sigma gives the Gaussian width on a detector 1 mm away."""
+    t = mx.generate_test_photons(2000)
+    t['pos'] = np.tile(np.array([0., 1., 0., 1.]), (2000, 1))
+    det = mx.FlatDetector(position=[-1, 0, 0], zoom=1000)
+    out0 = run(mx.Sequence(elements=[RandomGaussianScatter(scatter=0.), det]), _ph(t))
+    assert np.all(out0['det_x'] == 1.0) and np.all(out0['det_y'] == 0)
+    assert 'scatter' not in out0
+    out = run(mx.Sequence(elements=[RandomGaussianScatter(scatter=1e-5), det]), _ph(t))
+    r = np.hypot(out['det_x'] - 1., out['det_y'])
+    # radial rms of a 2-d Gaussian of sigma 1e-5 rad at 1 mm: sqrt(2) 1e-5 ... the
+    # angle itself is Gaussian with sigma 1e-5 about a random azimuth
+    assert np.isclose(np.sqrt(np.mean(r ** 2)), 1e-5, rtol=0.1)
+    assert np.isclose(np.std(out['scatter']), 1e-5, rtol=0.1)
+    # unit directions, polarization perpendicular
+    assert np.allclose(np.linalg.norm(out['dir'], axis=1), 1, atol=1e-12)
+
+
+def test_nonparallel_stack_layer_equals_cat():
+    """test_cat.py:15-28: with no blaze modifier NonParallelCATGrating is a
+    CATGrating bit for bit."""
+    t = _ph(mx.generate_test_photons(50))
+    t['dir'] = t['dir'] + np.random.default_rng(2).normal(0, 0.01, (50, 3))
+    sel = mx.OrderSelector([2])
+    p1 = run(mx.CATGrating(order_selector=sel, d=0.001), t)
+    p2 = run(mx.NonParallelCATGrating(order_selector=sel, d=0.001), t)
+    assert np.array_equal(p1['dir'], p2['dir'])
+
+
+def test_cat_l1l2_stack_columns_and_invariants():
+    """The full stack: order, L1 order, scatter columns; probabilities in
+    [0, 1]; unit directions with perpendicular polarization."""
+    eff = mx.OrderSelector([-1, 0, 1], [0.2, 0.5, 0.2])
+    st = CG.CATL1L2Stack(order_selector=eff, l1_transmission=SI, zoom=[1, 20, 20], id_col='facet',
+                         id_num=7)
+    t = _ph(mx.generate_test_photons(3000))
+    t['pos'][:, 1:] = np.random.default_rng(3).uniform(-15, 15, (3000, 2))
+    out = run(st, t)
+    for c in ('y', 'z', 'grat_y', 'grat_z', 'order', 'blaze', 'order_L1', 'blaze_L1',
+              'L2Diffraction', 'facet'):
+        assert c in out, c
+    assert np.all(out['facet'] == 7)
+    assert set(out['order'].tolist()) <= {-1, 0, 1}
+    assert np.all((out['prob'] > 0) & (out['prob'] <= 0.9 * 0.82))
+    assert np.allclose(np.linalg.norm(out['dir'], axis=1), 1, atol=1e-12)
+    # parallel transport keeps the polarization for turns with
+    # |d1 x d2| < 1e-8 (polarization.py:151-170): perpendicular to that level
+    assert np.allclose(np.sum(out['dir'] * out['pol'], axis=1), 0, atol=1e-8)
+    assert np.array_equal(out['grat_y'], out['y'])
