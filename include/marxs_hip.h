@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 12
+#define MX_ABI_VERSION 13
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -54,6 +54,18 @@ extern "C" {
 #define MX_STAGE_LENS_THIN       4
 #define MX_STAGE_GRATING         5
 #define MX_STAGE_DETECTOR        6
+#define MX_STAGE_STACK           7   /* FlatStack: one intersection, a program of layers */
+
+/* layer operations of a stack stage (stage_i[47]: bits 0-3 count, op k in
+ * bits 4+3k): membrane grating, quality factor, L1 cross-support, L2
+ * absorption, L2 diffraction scatter, Gaussian scatter of fixed sigma */
+#define MX_OP_GRATING    1
+#define MX_OP_QF         2
+#define MX_OP_L1         3
+#define MX_OP_L2ABS      4
+#define MX_OP_SCAT_L2    5
+#define MX_OP_SCAT_CONST 6
+#define MX_STACK_DRAWS   5   /* uniforms per stack interaction */
 
 /* precision of mx_trace */
 #define MX_PREC_FP64 64   /* reference op order, no FMA contraction      */
@@ -72,6 +84,7 @@ extern "C" {
 #define MX_ST_GRID_FALLBACK 3  /* ray too oblique for the culling grid       */
 #define MX_ST_MULTI_HIT     4  /* >1 interaction of one photon in one stage  */
 #define MX_ST_BAD_ARGS      5
+#define MX_ST_TRANS_RANGE   6  /* energy outside the L1 transmission table     */
 
 typedef struct MxTables {
     int32_t n_stages;
@@ -96,7 +109,7 @@ typedef struct MxTables {
     const unsigned char *lds_image32;  /* bytes copied to LDS by each workgroup, FP32 */
     const unsigned char *lds_image64;  /* the same for FP64 */
     int32_t first_draw_slot;   /* uniform slot of the first grating draw (-1: none), prefetched */
-    int32_t reserved;
+    int32_t has_stack;         /* 1: some stage is MX_STAGE_STACK (selects the kernel that runs them) */
 } MxTables;
 
 typedef struct MxPhotons {

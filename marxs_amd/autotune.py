@@ -53,7 +53,8 @@ def tune_trace(ci, dp, uniforms, precision=32, candidates=(2, 4, 8, 16), reps=3,
 
 
 def _batched_candidates(bt):
-    bs = 1024 if bt.precision == 32 else 768
+    from .batch import BLOCK, BLOCK_STK
+    bs = BLOCK_STK if bt.has_stack else BLOCK[bt.precision]
     out = []
     for mult in (1, 2, 4, 8):
         for scaled in (False, True):

@@ -27,11 +27,12 @@ from . import _lib
 from .compiler import _build_lds
 from .tracer import (CompiledInstrument, DevicePhotons, DeviceTables, PREC, raise_for_status,
                      TraceStatus, ST_PROB_RANGE, ST_ORDER_EMPTY, ST_SLOT_REUSE,
-                     ST_GRID_FALLBACK, ST_MULTI_HIT)
+                     ST_GRID_FALLBACK, ST_MULTI_HIT, ST_TRANS_RANGE)
 
 __all__ = ['BatchedTrace', 'variant_cost', 'outputs_like']
 
 BLOCK = {32: 1024, 64: 768}
+BLOCK_STK = 512      # work-items per workgroup of the kernels with stack stages
 
 
 def outputs_like(src):
@@ -110,6 +111,7 @@ class BatchedTrace:
         self._photon_structs = [ci.photons_struct(dp, u, src) for ci, dp, u, src in
                                 zip(self.instruments, self.outputs, self.uniforms, self.inputs)]
         self.costs = np.array([variant_cost(ci) for ci in self.instruments])
+        self.has_stack = any(ci.dev_tables.struct.has_stack for ci in self.instruments)
         self.queue = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.graph = None
         self.plan = None
@@ -126,7 +128,7 @@ class BatchedTrace:
         variants first), 'given', or a permutation.  lds_budget: bytes of
         tables staged in LDS per workgroup (None = compile default).
         """
-        bs = BLOCK[self.precision]
+        bs = BLOCK_STK if self.has_stack else BLOCK[self.precision]
         nv = self.n_variants
         if tile_photons is None:
             tile_photons = bs
@@ -233,7 +235,8 @@ class BatchedTrace:
             dp.has_pos = True
         return [TraceStatus(prob_range=int(r[ST_PROB_RANGE]), order_empty=int(r[ST_ORDER_EMPTY]),
                             slot_reuse=int(r[ST_SLOT_REUSE]), grid_fallback=int(r[ST_GRID_FALLBACK]),
-                            multi_hit=int(r[ST_MULTI_HIT])) for r in raw]
+                            multi_hit=int(r[ST_MULTI_HIT]), trans_range=int(r[ST_TRANS_RANGE]))
+                for r in raw]
 
     def raw_status(self):
         """(n_variants, MX_NSTATUS) counters of the last launch, input order."""

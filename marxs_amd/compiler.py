@@ -20,13 +20,16 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import optics as O
+from . import catgrating as CG
+from . import scatter as SC
 from .base import SimulationSetupError
 from .geometry import plane_frame
 from .simulator import BaseContainer
 
 __all__ = ['CompiledTables', 'compile_tree', 'flatten', 'build_grid',
            'STAGE_APERTURE_CIRCLE', 'STAGE_APERTURE_RECT', 'STAGE_LENS_PERFECT',
-           'STAGE_LENS_THIN', 'STAGE_GRATING', 'STAGE_DETECTOR', 'ORDER_NONE']
+           'STAGE_LENS_THIN', 'STAGE_GRATING', 'STAGE_DETECTOR', 'STAGE_STACK',
+           'ORDER_NONE', 'STACK_DRAWS']
 
 STAGE_APERTURE_CIRCLE = 1
 STAGE_APERTURE_RECT = 2
@@ -34,13 +37,24 @@ STAGE_LENS_PERFECT = 3
 STAGE_LENS_THIN = 4
 STAGE_GRATING = 5
 STAGE_DETECTOR = 6
+STAGE_STACK = 7
+
+# layer operations of a stack stage, 3 bits each (stage_i[47]: bits 0-3 the
+# number of operations, operation k in bits 4 + 3k)
+OP_GRATING, OP_QF, OP_L1, OP_L2ABS, OP_SCAT_L2, OP_SCAT_CONST = 1, 2, 3, 4, 5, 6
+STACK_DRAWS = 5     # draws per stack interaction: order, L1 order, L1 open, normal, angle
+SR = 16             # float64 per stack record
 
 SI, SD, ET, EP, EI, SELI = 48, 32, 16, 16, 8, 12
 ORDER_NONE = -2 ** 31
 
-# output column fields of a stage record (stage_i[10:19]); bit = index
-COL_FIELDS = ['loc0', 'loc1', 'id', 'order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1']
-INT_FIELDS = {'id', 'order', 'pix0', 'pix1'}
+# output column fields of a stage record, by bit (stage_i[10:19], then the
+# stack's second local-coordinate pair, L1 order and blaze angle and scatter
+# angle in stage_i[42:47])
+COL_FIELDS = ['loc0', 'loc1', 'id', 'order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1',
+              'loc0b', 'loc1b', 'order2', 'blaze2', 'scat']
+COL_SLOT = [10, 11, 12, 13, 14, 15, 16, 17, 18, 42, 43, 44, 45, 46]
+INT_FIELDS = {'id', 'order', 'pix0', 'pix1', 'order2'}
 
 
 class CompileError(SimulationSetupError, NotImplementedError):
@@ -65,7 +79,74 @@ def flatten(tree):
     return [tree]
 
 
+def _is_stack_layer(e):
+    return isinstance(e, (CG.QualityFactor, CG.L1, CG.L2Abs, SC.RandomGaussianScatter))
+
+
+def stack_layers(e):
+    """Layers of a stack stage element: a FlatStack's elements, or a
+    stand-alone layer element as a stack of one."""
+    return list(e.elements) if isinstance(e, O.FlatStack) else [e]
+
+
+def _check_grating(e):
+    if callable(e._d):
+        raise CompileError('gratings with a callable period d are not compiled')
+    if not isinstance(e.order_selector, (O.OrderSelector, O.EfficiencyFile,
+                                         O.InterpolateEfficiencyTable)):
+        raise CompileError('order_selector must be an OrderSelector, EfficiencyFile or '
+                           'InterpolateEfficiencyTable to run on the GPU, got '
+                           '{0!r}'.format(e.order_selector))
+
+
+def stack_program(e):
+    """Layer operations of a stack (``FlatStack.process_photons``,
+    ``optics/base.py:258-281``, runs its layers in order) as
+    ``[(op, layer)]``.  A scatter layer with sigma 0 changes nothing and is
+    left out (``scatter.py:124-129``)."""
+    prog = []
+    for ly in stack_layers(e):
+        if isinstance(ly, CG.L1):
+            _check_grating(ly)
+            prog.append((OP_L1, ly))
+        elif isinstance(ly, O.FlatGrating):
+            _check_grating(ly)
+            prog.append((OP_GRATING, ly))
+        elif isinstance(ly, CG.QualityFactor):
+            if not any(op == OP_GRATING for op, _ in prog):
+                raise CompileError('a QualityFactor layer needs a grating layer before it '
+                                   'in the same stack')
+            prog.append((OP_QF, ly))
+        elif isinstance(ly, CG.L2Abs):
+            prog.append((OP_L2ABS, ly))
+        elif isinstance(ly, CG.L2Diffraction):
+            prog.append((OP_SCAT_L2, ly))
+        elif isinstance(ly, SC.RandomGaussianScatter):
+            if callable(ly.scatter):
+                raise CompileError('a callable scatter has no GPU implementation')
+            if float(ly.scatter) != 0.:
+                prog.append((OP_SCAT_CONST, ly))
+        else:
+            raise CompileError('{0} has no GPU implementation as a stack layer'.format(
+                type(ly).__name__))
+    ops = [op for op, _ in prog]
+    for group in ((OP_GRATING,), (OP_QF,), (OP_L1,), (OP_L2ABS,), (OP_SCAT_L2, OP_SCAT_CONST)):
+        if sum(op in group for op in ops) > 1:
+            raise CompileError('a stack is compiled with at most one layer of each type')
+    return prog
+
+
+def encode_program(prog):
+    code = len(prog)
+    for k, (op, _) in enumerate(prog):
+        code |= op << (4 + 3 * k)
+    return code
+
+
 def leaf_kind(e):
+    if isinstance(e, O.FlatStack) or _is_stack_layer(e):
+        stack_program(e)
+        return STAGE_STACK
     if isinstance(e, O.CircleAperture):
         return STAGE_APERTURE_CIRCLE
     if isinstance(e, O.RectangleAperture):
@@ -77,13 +158,7 @@ def leaf_kind(e):
     if isinstance(e, O.ThinLens):
         return STAGE_LENS_THIN
     if isinstance(e, O.FlatGrating):
-        if callable(e._d):
-            raise CompileError('gratings with a callable period d are not compiled')
-        if not isinstance(e.order_selector, (O.OrderSelector, O.EfficiencyFile,
-                                             O.InterpolateEfficiencyTable)):
-            raise CompileError('order_selector must be an OrderSelector, EfficiencyFile or '
-                               'InterpolateEfficiencyTable to run on the GPU, got '
-                               '{0!r}'.format(e.order_selector))
+        _check_grating(e)
         return STAGE_GRATING
     if isinstance(e, O.FlatDetector):
         return STAGE_DETECTOR
@@ -97,6 +172,27 @@ def leaf_columns(e, kind):
     if loc is not None:
         cols['loc0'], cols['loc1'] = loc[0], loc[1]
     cols['id'] = e.id_col
+    if kind == STAGE_STACK:
+        # the stack's own local-coordinate pair, then those of its layers
+        # (each layer's process_photons writes its own, optics/base.py:258-281)
+        pairs = []
+        for x in ([e] if isinstance(e, O.FlatStack) else []) + stack_layers(e):
+            lc = getattr(x, 'loc_coos_name', None)
+            if lc is not None and (lc[0], lc[1]) not in pairs:
+                pairs.append((lc[0], lc[1]))
+        if len(pairs) > 2:
+            raise CompileError('a stack is compiled with at most two local-coordinate '
+                               'column pairs')
+        cols['loc0'], cols['loc1'] = pairs[0] if pairs else (None, None)
+        if len(pairs) > 1:
+            cols['loc0b'], cols['loc1b'] = pairs[1]
+        for op, ly in stack_program(e):
+            if op == OP_GRATING:
+                cols['order'], cols['blaze'] = ly.order_name, ly.blaze_name
+            elif op == OP_L1:
+                cols['order2'], cols['blaze2'] = ly.order_name, ly.blaze_name
+            elif op in (OP_SCAT_L2, OP_SCAT_CONST):
+                cols['scat'] = ly.scattername
     if kind == STAGE_GRATING:
         cols['order'] = e.order_name
         cols['blaze'] = e.blaze_name
@@ -116,6 +212,7 @@ class StageInfo:
     uslot: int = 0
     n_uslot: int = 0
     grid: dict = None
+    program: int = 0
 
 
 @dataclass
@@ -160,7 +257,7 @@ class CompiledTables:
         for s, st in enumerate(self.stages):
             mask = 0
             for b, f in enumerate(COL_FIELDS):
-                name = st.columns[f]
+                name = st.columns.get(f)
                 if name is None:
                     continue
                 if name not in existing and name not in first:
@@ -175,12 +272,13 @@ def _group(leaves):
     for e in leaves:
         kind = leaf_kind(e)
         cols = leaf_columns(e, kind)
+        prog = encode_program(stack_program(e)) if kind == STAGE_STACK else 0
         aperture = kind in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT)
         if (stages and not aperture and stages[-1].kind == kind
-                and stages[-1].columns == cols):
+                and stages[-1].columns == cols and stages[-1].program == prog):
             stages[-1].elements.append(e)
         else:
-            stages.append(StageInfo(kind=kind, elements=[e], columns=cols))
+            stages.append(StageInfo(kind=kind, elements=[e], columns=cols, program=prog))
     return stages
 
 
@@ -322,6 +420,8 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
             st.uslot, st.n_uslot = uslot, 2
         elif st.kind == STAGE_GRATING:
             st.uslot, st.n_uslot = uslot, int(n_draws)
+        elif st.kind == STAGE_STACK:
+            st.uslot, st.n_uslot = uslot, STACK_DRAWS * int(n_draws)
         uslot += st.n_uslot
         # stage origin: mean element centre (FP32 tables are relative to it)
         origin = np.mean([f['center'] for f in frames], axis=0)
@@ -356,6 +456,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
         rec[3] = 1 if g is not None else 0
         rec[4] = st.uslot
         rec[5] = st.n_uslot
+        rec[47] = st.program
         stage_d[s, 0:3] = origin
         if g is not None:
             rec[6], rec[7] = g['G1'], g['G2']
@@ -396,24 +497,26 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
             stage_d[s, 3:6] = frames[0]['e_x']
         # output columns
         for b, fname in enumerate(COL_FIELDS):
-            name = st.columns[fname]
+            name = st.columns.get(fname)
+            slot = COL_SLOT[b]
             if name is None:
-                rec[10 + b] = -1
+                rec[slot] = -1
                 continue
             if fname in INT_FIELDS:
-                kindname = 'order' if fname == 'order' else ('pix' if fname.startswith('pix') else 'id')
+                kindname = ('order' if fname in ('order', 'order2') else
+                            ('pix' if fname.startswith('pix') else 'id'))
                 if name not in icols:
                     icols.append(name)
                     icol_kind[name] = kindname
                 elif icol_kind[name] != kindname:
                     raise CompileError(f'column {name} used with two meanings')
-                rec[10 + b] = icols.index(name)
+                rec[slot] = icols.index(name)
             else:
                 if name in icols:
                     raise CompileError(f'column {name} used as int and float')
                 if name not in fcols:
                     fcols.append(name)
-                rec[10 + b] = fcols.index(name)
+                rec[slot] = fcols.index(name)
         rec[19] = 0
         rec[20:28] = -1     # LDS offsets, set by _build_lds
         rec[30:32] = -1
@@ -679,14 +782,9 @@ def _element_params(kind, x, f, ep, ei, sel_index, sel_rows, sel_d, sel_o):
     elif kind == STAGE_LENS_THIN:
         ep[6] = x.focallength
     elif kind == STAGE_GRATING:
-        l, e_perp, n = x.groove_frame()
-        ep[0:3] = l
-        ep[3:6] = -e_perp
-        ep[6] = x.d()
-        ep[7], ep[8] = x.blaze_modifier_params()
-        ei[1] = _selector_row(x.order_selector, sel_index, sel_rows, sel_d, sel_o)
-        ei[2] = 1 if x.sign_convention == 'cat' else 0
-        ei[3] = 1 if x.transmission else 0
+        _grating_params(x, ep, ei, 1, sel_index, sel_rows, sel_d, sel_o)
+    elif kind == STAGE_STACK:
+        _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o)
     elif kind == STAGE_DETECTOR:
         ep[6] = x.pixsize
         ep[7] = x.centerpix[0]
@@ -694,3 +792,60 @@ def _element_params(kind, x, f, ep, ei, sel_index, sel_rows, sel_d, sel_o):
         ep[9] = 1.0 / x.pixsize
         ei[4] = x.npix[0]
         ei[5] = x.npix[1]
+
+
+def _grating_params(x, ep, ei, isel, sel_index, sel_rows, sel_d, sel_o):
+    """Groove frame l, d = -e_perp (grating.py:186-207), period, blaze
+    modifier (centre, slope) into ep[0:9]; selector row, sign mode (1: CAT,
+    sign of p.d) and transmission flag into ei[isel:isel+3]."""
+    l, e_perp, n = x.groove_frame()
+    ep[0:3] = l
+    ep[3:6] = -e_perp
+    ep[6] = x.d()
+    ep[7], ep[8] = x.blaze_modifier_params()
+    ei[isel] = _selector_row(x.order_selector, sel_index, sel_rows, sel_d, sel_o)
+    ei[isel + 1] = 1 if x.sign_convention == 'cat' else 0
+    ei[isel + 2] = 1 if x.transmission else 0
+
+
+def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
+    """Parameters of a stack element.  The membrane grating as a grating
+    element (ep[0:9], ei[1:4]); the rest in a record of SR float64 in sel_d
+    at ei[6]: L1 groove frame, period and blaze modifier (0:9, the ep layout),
+    quality factor (9), L1 open fraction (10), L2 open fraction and
+    shadowing coefficient (11, 12), scatter parameter (13: sigma in rad, or
+    the L2 inner free width in mm), offset and length of the L1 transmission
+    table in sel_d (14, 15: energies in keV, then transmissions).  L1
+    selector row, sign mode and transmission flag in ei[7], ei[4], ei[5]."""
+    rec = np.zeros(SR)
+    ei[4] = ei[5] = ei[7] = -1
+    for op, ly in stack_program(x):
+        if op == OP_GRATING:
+            _grating_params(ly, ep, ei, 1, sel_index, sel_rows, sel_d, sel_o)
+        elif op == OP_L1:
+            tmp_i = np.zeros(4, dtype=np.int32)
+            _grating_params(ly, rec, tmp_i, 1, sel_index, sel_rows, sel_d, sel_o)
+            ei[7], ei[4], ei[5] = tmp_i[1], tmp_i[2], tmp_i[3]
+            rec[10] = ly.openfraction
+            te = np.asarray(ly.trans_energy, dtype=float)
+            tv = np.asarray(ly.trans_value, dtype=float)
+            if len(te) < 2 or len(tv) != len(te) or np.any(np.diff(te) <= 0):
+                raise CompileError('the L1 transmission table needs >= 2 strictly '
+                                   'increasing energies')
+            key = ('transmission', te.tobytes(), tv.tobytes())
+            if key not in sel_index:
+                sel_index[key] = len(sel_d)
+                sel_d.extend(te.tolist())
+                sel_d.extend(tv.tolist())
+            rec[14] = sel_index[key]
+            rec[15] = len(te)
+        elif op == OP_QF:
+            rec[9] = ly.factor
+        elif op == OP_L2ABS:
+            rec[11], rec[12] = ly.coefficients()
+        elif op == OP_SCAT_L2:
+            rec[13] = ly.innerfree
+        elif op == OP_SCAT_CONST:
+            rec[13] = float(ly.scatter)
+    ei[6] = len(sel_d)
+    sel_d.extend(rec.tolist())

@@ -74,7 +74,7 @@ def _gas_count(radius, d_element, phi):
     return len(probe.elemposyz()[0])
 
 
-def _gas_with_n(rowland, n, radius, d_element, elem_args):
+def _gas_with_n(rowland, n, radius, d_element, elem_args, elem_class=CATGrating):
     """GratingArrayStructure over a wedge around +z, widened until it holds at
     least ``n`` facets, then cut to the first ``n``."""
     lo, hi = 0.01, math.pi
@@ -92,7 +92,7 @@ def _gas_with_n(rowland, n, radius, d_element, elem_args):
             break
     half = best if best is not None else hi
     gas = GratingArrayStructure(rowland=rowland, d_element=d_element, radius=radius,
-                                phi=[-half, half], elem_class=CATGrating,
+                                phi=[-half, half], elem_class=elem_class,
                                 elem_args=dict(elem_args), id_col='facet')
     if len(gas.elem_pos) > n:
         gas.elem_pos = gas.elem_pos[:n]
@@ -103,9 +103,11 @@ def _gas_with_n(rowland, n, radius, d_element, elem_args):
 
 def cat_rowland_instrument(n_facets=200, R=5900., r=6000., blaze_deg=1.5,
                            n_ccd=16, ccd_y=(320., 720.), efficiency=None,
-                           focal=12500., d=2e-4):
+                           focal=12500., d=2e-4, facet_class=CATGrating, facet_kwargs=None):
     """Aperture, PerfectLens, a CAT facet array on a Rowland torus and a row of
-    CCDs on the Rowland circle.  Returns (Sequence, parts dict)."""
+    CCDs on the Rowland circle.  Returns (Sequence, parts dict).
+    ``facet_class``/``facet_kwargs``: e.g. ``catgrating.CATL1L2Stack`` with
+    ``{'l1_transmission': ...}`` for facets with their support structure."""
     aper = CircleAperture(position=[focal, 0, 0], zoom=[1, 600, 600], r_inner=300)
     lens = PerfectLens(focallength=focal, position=[focal, 0, 0], zoom=[1, 700, 700])
     # Rowland circle through the focus (origin) and x = 2 r on axis
@@ -117,7 +119,9 @@ def cat_rowland_instrument(n_facets=200, R=5900., r=6000., blaze_deg=1.5,
     r_out = 600. * x_on / focal
     facet_args = {'zoom': [1, 12.5, 12.5], 'd': d, 'order_selector': eff,
                   'orientation': blaze}
-    gas = _gas_with_n(rowland, n_facets, [r_in + 13.5, r_out - 13.5], [27., 27.], facet_args)
+    facet_args.update(facet_kwargs or {})
+    gas = _gas_with_n(rowland, n_facets, [r_in + 13.5, r_out - 13.5], [27., 27.], facet_args,
+                      elem_class=facet_class)
     pitch = 24.576 + 0.5
     ccd = RectangularGrid(rowland=rowland, elem_class=FlatDetector,
                           elem_args={'pixsize': 0.024, 'zoom': [1, 12.288, 12.288]},

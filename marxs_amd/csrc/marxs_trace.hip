@@ -45,6 +45,9 @@
 #ifndef MX_BLOCK64
 #define MX_BLOCK64 768
 #endif
+#ifndef MX_BLOCK_STK
+#define MX_BLOCK_STK 512        /* kernels with stack stages: 256 VGPRs, no spills */
+#endif
 #define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
 #ifndef MX_CDF_COUNT
 #define MX_CDF_COUNT 0          /* 1: count cdf entries <= u instead of bisecting */
@@ -219,7 +222,7 @@ template <typename T> __device__ __forceinline__ cptr<T> cp(const T *p) { return
 struct Counters {
     unsigned *c;
 };
-#define MX_NCOUNT 6
+#define MX_NCOUNT 7
 __device__ __forceinline__ void count(const Counters &cnt, int k) {
     const unsigned long long m = __ballot(1);
     if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
@@ -459,6 +462,11 @@ __device__ __forceinline__ void put_defaults(const MxPhotons &P, SIP si, int64_t
     if (always & 64) put_f(P, si[16], i, qnan);
     if (always & 128) put_i(P, si[17], i, -1);
     if (always & 256) put_i(P, si[18], i, -1);
+    if (always & 512) put_f(P, si[42], i, qnan);
+    if (always & 1024) put_f(P, si[43], i, qnan);
+    if (always & 2048) put_i(P, si[44], i, MX_ORDER_NONE);
+    if (always & 4096) put_f(P, si[45], i, qnan);
+    if (always & 8192) put_f(P, si[46], i, qnan);
 }
 
 __device__ __forceinline__ int pixel_index(double detpix, int npix) {
@@ -600,6 +608,152 @@ __device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, i
     return blaze;
 }
 
+// ------------------------------------------------------------ stack layers
+// L1 bar transmission at energy e: linear interpolation with scipy
+// interp1d's arithmetic (marxs/missions/mitsnl/catgrating.py:189-192):
+// interval [lo, lo+1] with lo + 1 the first index whose energy is >= e,
+// clamped to [1, n-1]; slope * (e - x[lo]) + y[lo].  out: e outside the
+// table (interp1d raises).
+__device__ __forceinline__ double trans_interp(const double *x, const double *y, int n, double e, bool &out) {
+    out = (e < x[0]) || (e > x[n - 1]);
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] < e) lo = mid + 1;
+        else hi = mid;
+    }
+    int h = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+    const int l = h - 1;
+    const double slope = (y[h] - y[l]) / (x[h] - x[l]);
+    return slope * (e - x[l]) + y[l];
+}
+
+// Inverse of the standard normal CDF (the scatter's normal deviate from a
+// uniform draw).  Not inlined: inlined into the trace loop its register
+// demand makes the kernel spill.
+__device__ __noinline__ double inv_normal_cdf(double u) { return normcdfinv(u); }
+
+// The reference's scatter applies transforms3d's axangle2mat(axis, angle)
+// as einsum('...ij,...i->...j', M, v) (marxs/optics/scatter.py:133-139,
+// marxs/math/rotations.py:54-87): the transposed matrix, axis normalised.
+__device__ __forceinline__ d3 axangle_t(d3 ax, double ang, d3 v) {
+    ax = unit3(ax);
+    const double x = ax.x, y = ax.y, z = ax.z;
+    const double c = cos(ang), s = sin(ang), C = 1 - c;
+    const double xs = x * s, ys = y * s, zs = z * s;
+    const double xC = x * C, yC = y * C, zC = z * C;
+    const double xyC = x * yC, yzC = y * zC, zxC = z * xC;
+    const double m00 = x * xC + c, m01 = xyC - zs, m02 = zxC + ys;
+    const double m10 = xyC + zs, m11 = y * yC + c, m12 = yzC - xs;
+    const double m20 = zxC - ys, m21 = yzC + xs, m22 = z * zC + c;
+    return mk(m00 * v.x + m10 * v.y + m20 * v.z,
+              m01 * v.x + m11 * v.y + m21 * v.z,
+              m02 * v.x + m12 * v.y + m22 * v.z);
+}
+
+// FlatStack.process_photons (marxs/optics/base.py:258-281) for one photon
+// at its intersection with stack element (et, ep, ei): the layer program
+// stage_i[47] runs in order, in float64 for both precisions, each layer with
+// the direction and polarization left by the one before it.
+//  grating  membrane (grating.py:233-266): draw 0, order, blaze, probability
+//  qf       exp(-(2 pi sigma/d)^2) ** (order^2) (catgrating.py:147-160)
+//  l1       L1 as a CAT grating (catgrating.py:169-212): draw 1; with
+//           draw 2 > open fraction the photon passed a bar instead:
+//           direction and polarization unchanged, order 0, probability =
+//           bar transmission at its energy
+//  l2abs    open fraction - coef sin(angle to normal) (catgrating.py:215-249)
+//  scatter  (scatter.py:88-145) angle = Phi^-1(draw 3) sigma about
+//           dir x guess, then 2 pi draw 4 about dir, polarization
+//           parallel-transported; sigma = 1.22 0.4 asin(lambda / inner
+//           free width) for L2 diffraction (catgrating.py:252-271)
+// Draw j of the k-th interaction in the stage: slot 5 k + j (cyclic).
+// Stack record sr (float64, sel_d + ei[6]): L1 l (0-2), d (3-5), period (6),
+// blaze centre and slope (7, 8), quality factor (9), L1 open fraction (10),
+// L2 open fraction and coefficient (11, 12), scatter parameter (13),
+// transmission table offset and length (14, 15).
+template <int PREC>
+__device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons &P, Ray &R, const cptr<int> si,
+                                             const double *et, const double *ep, const int *ei, double ly,
+                                             int nhit, int64_t i, const Counters &cnt,
+                                             const unsigned char *lds) {
+    const int prog = si[47];
+    const double *sr = T.sel_d + ei[6];
+    d3 d = PREC == 64 ? R.d : tod(R.df);
+    d3 pol = PREC == 64 ? R.pol : tod(R.polf);
+    const double energy = R.energy;
+    const int k0 = MX_STACK_DRAWS * nhit;
+    int m_last = MX_ORDER_NONE;
+    const int nops = prog & 15;
+    for (int k = 0; k < nops; ++k) {
+        const int op = (prog >> (4 + 3 * k)) & 7;
+        if (op == MX_OP_GRATING || op == MX_OP_L1) {
+            const bool l1 = op == MX_OP_L1;
+            const double *gp = l1 ? sr : ep;
+            const double u = draw(T, P, R, si[4], si[5], k0 + (l1 ? 1 : 0), i, cnt);
+            int m; double ptot;
+            const d3 d_old = d, pol_old = pol;
+            if (!select_order<PREC>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds,
+                                    [&]() { return blaze64(et, gp, ly, d_old); }))
+                count(cnt, MX_ST_ORDER_EMPTY);
+            const double blaze = diffract64(et, gp, l1 ? ei[4] : ei[2], (l1 ? ei[5] : ei[3]) ? 1.0 : -1.0,
+                                            energy, ly, m, d, pol);
+            if (l1) {
+                const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
+                if (u2 > sr[10]) {
+                    d = d_old;
+                    pol = pol_old;
+                    m = 0;
+                    const int64_t off = (int64_t)sr[14];
+                    const int nt = (int)sr[15];
+                    bool out;
+                    ptot = trans_interp(T.sel_d + off, T.sel_d + off + nt, nt, energy, out);
+                    if (out) count(cnt, MX_ST_TRANS_RANGE);
+                }
+                put_i(P, si[44], i, m);
+                put_f(P, si[45], i, blaze);
+            } else {
+                m_last = m;
+                put_i(P, si[13], i, m);
+                put_f(P, si[14], i, blaze);
+            }
+            if (ptot < 0.0 || ptot > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * ptot;
+        } else if (op == MX_OP_QF) {
+            const double fm = (double)m_last;
+            const double f = pow(sr[9], fm * fm);
+            if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * f;
+        } else if (op == MX_OP_L2ABS) {
+            const double ang = acos(fabs(dot3(unit3(d), ld3(et + 3))));
+            const double f = sr[11] - sr[12] * sin(ang);
+            if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * f;
+        } else if (op == MX_OP_SCAT_L2 || op == MX_OP_SCAT_CONST) {
+            double sigma;
+            if (op == MX_OP_SCAT_L2) {
+                const double wave = (MX_KEV_NM * 1e-6) / energy;
+                sigma = (1.22 * 0.4) * asin(wave / sr[13]);
+            } else {
+                sigma = sr[13];
+            }
+            const double u3 = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
+            const double u4 = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
+            const double ang = inv_normal_cdf(u3) * sigma;
+            const d3 pd = unit3(d);
+            const d3 guess = fabs(pd.x) < 0.99999 ? mk(1.0, 0.0, 0.0) : mk(0.0, 1.0, 0.0);
+            d3 nd = axangle_t(cross3(pd, guess), ang, pd);
+            nd = axangle_t(pd, u4 * 2 * M_PI, nd);
+            pol = ptrans64(d, nd, pol);
+            d = nd;
+            put_f(P, si[46], i, ang);
+        } else {
+            count(cnt, MX_ST_BAD_ARGS);
+        }
+    }
+    if (PREC == 64) { R.d = d; R.pol = pol; }
+    else { R.df = tof(d); R.polf = tof(pol); }
+}
+
 // Next hit of the ray among candidates [tb, te) (cell list when `list`,
 // element indices otherwise) with index > last, tested in ascending order.
 // ET points at the stage's element records (LDS or global).  FP64: q, ly,
@@ -653,7 +807,7 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
 
 // One array stage (lens, grating facets, detectors) for one photon.  L: the
 // stage's records, parameters and culling lists are in LDS.
-template <int PREC, bool L>
+template <int PREC, bool L, bool STK>
 __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &P, const int s,
                                             const int64_t i, const bool use_grid, const Counters &cnt,
                                             const unsigned char *lds, Ray &R) {
@@ -818,6 +972,11 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             R.prob = R.prob * ptot;
             put_i(P, si[13], i, m);
             put_f(P, si[14], i, blaze);
+        } else if (STK && kind == MX_STAGE_STACK) {
+            id = ei[0];
+            stack_layers<PREC>(T, P, R, si, et, ep, ei, ly, nhit, i, cnt, lds);
+            put_f(P, si[42], i, ly);
+            put_f(P, si[43], i, lz);
         } else if (kind == MX_STAGE_DETECTOR) {
             id = ei[0];
             double dp0, dp1;
@@ -880,7 +1039,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
 
 // ======================================================= the trace kernel
 // One photon through all stages.
-template <int PREC>
+template <int PREC, bool STK>
 __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
                                           const bool use_grid, const Counters &cnt,
                                           const unsigned char *lds) {
@@ -944,9 +1103,12 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             put_i(P, si[12], i, cp(T.elem_i)[(size_t)e0 * MX_EI]);
             continue;
         }
-        if (kind < MX_STAGE_LENS_PERFECT || kind > MX_STAGE_DETECTOR) { count(cnt, MX_ST_BAD_ARGS); continue; }
-        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true>(T, P, s, i, use_grid, cnt, lds, R);
-        else array_stage<PREC, false>(T, P, s, i, use_grid, cnt, lds, R);
+        if (kind < MX_STAGE_LENS_PERFECT || kind > (STK ? MX_STAGE_STACK : MX_STAGE_DETECTOR)) {
+            count(cnt, MX_ST_BAD_ARGS);
+            continue;
+        }
+        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, STK>(T, P, s, i, use_grid, cnt, lds, R);
+        else array_stage<PREC, false, STK>(T, P, s, i, use_grid, cnt, lds, R);
     }
 
     if (PREC == 32) { R.d = tod(R.df); R.pol = tod(R.polf); }
@@ -959,7 +1121,9 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
 // Persistent kernel: each workgroup copies the LDS image (element records,
 // parameters, culling lists and selector tables of the stages marked in
 // stage_i/sel_i) once, then walks photons with a grid stride.
-template <int PREC, int BS>
+// STK: the tables hold stack stages (a separate instantiation, so that the
+// layer code does not weigh on the register allocation of the others).
+template <int PREC, int BS, bool STK>
 __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, int flags,
                                                        unsigned long long *status) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -977,7 +1141,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, i
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
-        trace_one<PREC>(T, P, i, use_grid, cnt, lds);
+        trace_one<PREC, STK>(T, P, i, use_grid, cnt, lds);
     __syncthreads();
     if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
@@ -988,7 +1152,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, i
 // tile and finds its variant (binary search over tile_first); the LDS image
 // is reloaded only when the variant changes.  Every exit is uniform over the
 // workgroup (the tile index is read from LDS after a barrier).
-template <int PREC, int BS>
+template <int PREC, int BS, bool STK>
 __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         const MxTables *__restrict__ tabs, const MxPhotons *__restrict__ phs,
         const int64_t *__restrict__ tile_first, const int32_t *__restrict__ tile_photons,
@@ -1042,7 +1206,7 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         int64_t e = b + tp;
         if (e > P.n) e = P.n;
         if (b < 0 || tp <= 0) e = b;
-        for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC>(T, P, i, use_grid, cnt, lds);
+        for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC, STK>(T, P, i, use_grid, cnt, lds);
         __syncthreads();
         // this tile's counters go to its variant; the next claim's barrier
         // orders the reset before the next tile counts
@@ -1090,7 +1254,8 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
         return 1;
     }
     if (photons->n == 0 || tables->n_stages == 0) return 0;
-    const int block = precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32;
+    const bool stk = tables->has_stack != 0;
+    const int block = stk ? MX_BLOCK_STK : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
     const int64_t nblk = (photons->n + block - 1) / block;
     hipStream_t st = (hipStream_t)stream;
     const int lds = precision == MX_PREC_FP64 ? tables->lds_bytes64 : tables->lds_bytes32;
@@ -1107,11 +1272,17 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
     int64_t grid = (int64_t)cus * per_cu;
     if (grid > nblk) grid = nblk;
-    if (precision == MX_PREC_FP64)
-        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64>), dim3((unsigned)grid), dim3(block), lds, st,
+    if (precision == MX_PREC_FP64 && stk)
+        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK_STK, true>), dim3((unsigned)grid), dim3(block), lds, st,
+                           *tables, *photons, (int)flags, status);
+    else if (precision == MX_PREC_FP64)
+        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64, false>), dim3((unsigned)grid), dim3(block), lds, st,
+                           *tables, *photons, (int)flags, status);
+    else if (stk)
+        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK_STK, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else
-        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32>), dim3((unsigned)grid), dim3(block), lds, st,
+        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32, false>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -1137,6 +1308,7 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
         return 1;
     }
     int lds = 0;
+    bool stk = false;
     for (int v = 0; v < b->n_variants; ++v) {
         if (check_tables(b->tables + v) || check_photons(b->photons + v)) {
             char buf[512];
@@ -1152,9 +1324,10 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
             return 1;
         }
         if (l > lds) lds = l;
+        if (t.has_stack) stk = true;
     }
     if (b->n_tiles == 0) return 0;
-    const int block = precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32;
+    const int block = stk ? MX_BLOCK_STK : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
     int64_t grid = b->grid;
     if (grid <= 0) {
         int dev = 0, cus = 0;
@@ -1171,14 +1344,15 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
         set_error("queue reset failed");
         return 2;
     }
-    if (precision == MX_PREC_FP64)
-        hipLaunchKernelGGL((mx_trace_batched_kernel<64, MX_BLOCK64>), dim3((unsigned)grid), dim3(block), lds, st,
-                           b->tables_dev, b->photons_dev, b->tile_first, b->tile_photons, b->n_variants,
-                           (long long)b->n_tiles, (int)flags, b->queue, b->status);
-    else
-        hipLaunchKernelGGL((mx_trace_batched_kernel<32, MX_BLOCK32>), dim3((unsigned)grid), dim3(block), lds, st,
-                           b->tables_dev, b->photons_dev, b->tile_first, b->tile_photons, b->n_variants,
-                           (long long)b->n_tiles, (int)flags, b->queue, b->status);
+#define MX_LAUNCH_BATCHED(PR, BSZ, S)                                                                  \
+    hipLaunchKernelGGL((mx_trace_batched_kernel<PR, BSZ, S>), dim3((unsigned)grid), dim3(block), lds, st,  \
+                       b->tables_dev, b->photons_dev, b->tile_first, b->tile_photons, b->n_variants,      \
+                       (long long)b->n_tiles, (int)flags, b->queue, b->status)
+    if (precision == MX_PREC_FP64 && stk) MX_LAUNCH_BATCHED(64, MX_BLOCK_STK, true);
+    else if (precision == MX_PREC_FP64) MX_LAUNCH_BATCHED(64, MX_BLOCK64, false);
+    else if (stk) MX_LAUNCH_BATCHED(32, MX_BLOCK_STK, true);
+    else MX_LAUNCH_BATCHED(32, MX_BLOCK32, false);
+#undef MX_LAUNCH_BATCHED
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         char buf[256];

@@ -23,14 +23,15 @@ import numpy as np
 import torch
 
 from . import _lib
-from .compiler import compile_tree, flatten, CompileError, ORDER_NONE
+from .compiler import compile_tree, flatten, CompileError, ORDER_NONE, STAGE_STACK
 from .simulator import BaseContainer, Sequence
 from . import random as mxrandom
 
 __all__ = ['DeviceTables', 'DevicePhotons', 'CompiledInstrument', 'trace',
            'TraceStatus']
 
-ST_PROB_RANGE, ST_ORDER_EMPTY, ST_SLOT_REUSE, ST_GRID_FALLBACK, ST_MULTI_HIT, ST_BAD = range(6)
+(ST_PROB_RANGE, ST_ORDER_EMPTY, ST_SLOT_REUSE, ST_GRID_FALLBACK, ST_MULTI_HIT, ST_BAD,
+ ST_TRANS_RANGE) = range(7)
 
 PREC = {64: 64, 32: 32, 'fp64': 64, 'fp32': 32}
 
@@ -72,6 +73,7 @@ class DeviceTables:
                      'succ_items', 'lds_image32', 'lds_image64'):
             setattr(s, name, k[name].data_ptr())
         s.first_draw_slot = self.host.first_draw_slot
+        s.has_stack = int(any(st.kind == STAGE_STACK for st in self.host.stages))
         return s
 
     def struct_for(self, existing):
@@ -274,7 +276,7 @@ class CompiledInstrument:
         raise_for_status(st)
         return TraceStatus(prob_range=int(st[ST_PROB_RANGE]), order_empty=int(st[ST_ORDER_EMPTY]),
                            slot_reuse=int(st[ST_SLOT_REUSE]), grid_fallback=int(st[ST_GRID_FALLBACK]),
-                           multi_hit=int(st[ST_MULTI_HIT]))
+                           multi_hit=int(st[ST_MULTI_HIT]), trans_range=int(st[ST_TRANS_RANGE]))
 
     # ------------------------------------------------------------- table
     def __call__(self, photons, uniforms=None, precision=64, generator=None, grid=None):
@@ -306,6 +308,10 @@ class CompiledInstrument:
 
 
 def raise_for_status(st):
+    if st[ST_TRANS_RANGE]:
+        # interp1d of the L1 bar transmission (catgrating.py:189-192) raises
+        # before the probability check of that layer
+        raise ValueError('A value in x_new is outside the interpolation range.')
     if st[ST_PROB_RANGE]:
         raise ValueError('Found probability outside of the 0..1 arange.')
     if st[ST_ORDER_EMPTY]:

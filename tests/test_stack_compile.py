@@ -1,0 +1,102 @@
+"""Compile step of stack stages (CPU): layer program, parameter records,
+output columns and the cases that are not compiled."""
+import os
+
+import numpy as np
+import pytest
+
+import marxs_amd as mx
+from marxs_amd import catgrating as CG, compiler as C, configs
+from marxs_amd.scatter import RandomGaussianScatter
+
+SI = os.path.join(os.path.dirname(__file__), 'golden', 'SiTransmission.csv')
+
+
+def _decode(code):
+    return [(code >> (4 + 3 * k)) & 7 for k in range(code & 15)]
+
+
+def test_cat_stack_program_and_records():
+    inst, parts = configs.cat_rowland_instrument(
+        50, facet_class=CG.CATL1L2Stack, facet_kwargs={'l1_transmission': SI})
+    t = C.compile_tree(inst, n_draws=3)
+    st = t.stages[2]
+    assert st.kind == C.STAGE_STACK and len(st.elements) == 50
+    assert _decode(t.stage_i[2, 47]) == [C.OP_GRATING, C.OP_QF, C.OP_L1, C.OP_L2ABS,
+                                         C.OP_SCAT_L2]
+    assert t.stage_i[2, 5] == C.STACK_DRAWS * 3
+    # columns: y/z of the stack, grat_y/grat_z of its layers, both orders
+    names = {f: st.columns.get(f) for f in C.COL_FIELDS}
+    assert (names['loc0'], names['loc1'], names['loc0b'], names['loc1b']) == ('y', 'z', 'grat_y',
+                                                                              'grat_z')
+    assert (names['order'], names['order2'], names['scat']) == ('order', 'order_L1',
+                                                                'L2Diffraction')
+    assert t.fcols[t.stage_i[2, 42]] == 'grat_y' and t.icols[t.stage_i[2, 44]] == 'order_L1'
+    assert t.fcols[t.stage_i[2, 46]] == 'L2Diffraction'
+    # per-facet record: L1 groove direction perpendicular to the membrane's
+    facet = st.elements[7]
+    k = st.elem0 + 7
+    rec = t.sel_d[t.elem_i[k, 6]:t.elem_i[k, 6] + C.SR]
+    l_mem = t.elem_p[k, 0:3]
+    assert abs(np.dot(rec[0:3], l_mem)) < 1e-12
+    assert rec[6] == CG.l1_dims['period']
+    assert np.isclose(rec[9], np.exp(-(2 * np.pi * 0.00175 / 0.2) ** 2))
+    assert rec[10] == facet.elements[2].openfraction
+    assert (rec[11], rec[12]) == facet.elements[3].coefficients()
+    assert rec[13] == facet.elements[4].innerfree
+    off, n = int(rec[14]), int(rec[15])
+    assert np.array_equal(t.sel_d[off:off + n], facet.elements[2].trans_energy)
+    assert np.array_equal(t.sel_d[off + n:off + 2 * n], facet.elements[2].trans_value)
+    # the transmission table is stored once for all facets
+    offs = {int(t.sel_d[t.elem_i[st.elem0 + j, 6] + 14]) for j in range(50)}
+    assert len(offs) == 1
+    # L1 selector, CAT sign, transmission flag
+    assert t.elem_i[k, 4] == 1 and t.elem_i[k, 5] == 1 and t.elem_i[k, 7] >= 0
+
+
+def test_stack_defaults_mask_and_stage_grouping():
+    sel = mx.OrderSelector([-1, 0, 1], [0.3, 0.4, 0.3])
+    a = CG.CATL1L2Stack(order_selector=sel, l1_transmission=SI, position=[60, 0, 0])
+    b = CG.CATL1L2Stack(order_selector=sel, l1_transmission=SI, position=[50, 0, 0])
+    s = RandomGaussianScatter(scatter=1e-4, position=[40, 0, 0])
+    t = C.compile_tree(mx.Sequence(elements=[a, b, s]))
+    assert [len(x.elements) for x in t.stages] == [2, 1]
+    # new columns of the first stack stage get defaults (bits 9-13 too)
+    assert t.stage_i[0, 19] & (1 << 11) and t.stage_i[0, 19] & (1 << 13)
+    # the scatter stage writes y/z again: present already, written on hit only
+    assert not (t.stage_i[1, 19] & 3)
+
+
+def test_sigma_zero_scatter_has_no_operation():
+    t = C.compile_tree(RandomGaussianScatter(scatter=0.))
+    assert t.stages[0].kind == C.STAGE_STACK and t.stage_i[0, 47] == 0
+    assert 'scatter' not in t.fcols
+
+
+@pytest.mark.parametrize('layers, match', [
+    ([CG.QualityFactor], 'grating layer before'),
+    ([RandomGaussianScatter, RandomGaussianScatter], 'at most one layer'),
+])
+def test_rejected_stacks(layers, match):
+    kw = {CG.QualityFactor: {}, RandomGaussianScatter: {'scatter': 1e-3}}
+    st = mx.FlatStack(elements=layers, keywords=[kw[x] for x in layers])
+    with pytest.raises(C.CompileError, match=match):
+        C.compile_tree(st)
+
+
+def test_callable_scatter_and_unknown_layer_rejected():
+    with pytest.raises(NotImplementedError):
+        RandomGaussianScatter(scatter=lambda ph: 1e-3)
+    st = mx.FlatStack(elements=[mx.PerfectLens], keywords=[{'focallength': 10.}])
+    with pytest.raises(C.CompileError, match='stack layer'):
+        C.compile_tree(st)
+
+
+def test_l1_transmission_table_validated():
+    with pytest.raises(ValueError, match='increase'):
+        CG.L1(order_selector=mx.OrderSelector([1]),
+              l1_transmission=(np.array([1., 0.5]), np.array([0.1, 0.2])))
+    l1 = CG.L1(order_selector=mx.OrderSelector([1]),
+               l1_transmission=(np.array([1.]), np.array([0.1])))
+    with pytest.raises(C.CompileError, match='transmission table'):
+        C.compile_tree(l1)
