@@ -23,6 +23,11 @@ Two more sections run after it (``--sections``):
              torch autograd over the pose chain).  Reports adjoints/s, the
              agreement with central finite differences and with the FP64
              adjoint.
+``stack``    the trace workload with every facet a CATL1L2Stack (membrane,
+             quality factor, L1 cross-support with the Si bar transmission,
+             L2 absorption and L2 diffraction), 1e7 photons: FP32 and FP64
+             rates, FP64 indices against the oracle on a sample, the
+             oracle's rate on that sample.
 ``batched``  64 design variants (Rowland R ~ U(5000, 7000) mm, blaze ~
              U(1, 2) deg, 50-800 facets, own CCD rows), 2e5 photons each at
              0.3-2.0 keV, traced in one autotuned, hipGraph-replayed launch,
@@ -30,7 +35,7 @@ Two more sections run after it (``--sections``):
              batched results are compared with the sequential FP64 traces.
 
     python bench.py [--steps K] [--warmup W] [--photons N] [--cpu-sample S]
-                    [--sections trace,adjoint,batched]
+                    [--sections trace,adjoint,stack,batched]
 """
 import argparse
 import json
@@ -69,7 +74,8 @@ def parse():
                     help='profiling: time one precision only, skip comparisons')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='profiling: run one device copy of this size first (counter calibration)')
-    ap.add_argument('--sections', default='trace,adjoint,batched')
+    ap.add_argument('--sections', default='trace,adjoint,stack,batched')
+    ap.add_argument('--stack-cpu-sample', type=int, default=100_000)
     ap.add_argument('--cpu-procs', type=int, default=0,
                     help='processes for the multi-process CPU baseline (0 = min(16, cpu count))')
     ap.add_argument('--variants', type=int, default=64)
@@ -328,6 +334,8 @@ def main():
     torch.cuda.empty_cache()
     if 'adjoint' in sections:
         res['adjoint'] = bench_adjoint(args, dev)
+    if 'stack' in sections:
+        res['stack'] = bench_stack(args, dev)
     if 'batched' in sections:
         res['batched'] = bench_batched(args, dev)
     print(json.dumps(res), flush=True)
@@ -441,6 +449,94 @@ def bench_adjoint(args, dev):
            'fd_rel_err_fp64_top3_facets': fd_err, 'fd_steps': '1e-4 mm, 1e-6 rad, 1e-10 mm',
            'fp32_vs_fp64_rel_err': f32_err}
     log('adjoint', res)
+    return res
+
+
+def gpu_inputs(ci, n, seed, dev):
+    """The trace workload's synthetic photons, generated on the GPU: line
+    spectrum, on-axis directions, polarization angle 2 pi u; returns
+    (DevicePhotons, uniforms)."""
+    import numpy as np
+    import torch
+    from marxs_amd import configs
+    from marxs_amd.tracer import DevicePhotons
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    u = torch.rand((ci.n_uniform_slots + 3, n), dtype=torch.float64, device=dev, generator=g)
+    lines = torch.tensor(configs.LINES, dtype=torch.float64, device=dev)
+    cont = u[0] < 0.2
+    idx = torch.clamp(((u[0] - 0.2) / 0.2).long(), 0, 3)
+    energy = torch.where(cont, 0.3 + 1.7 * u[1], lines[idx])
+    phi = 2 * np.pi * u[2]
+    dirs = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    dirs[0] = -1.
+    pol = torch.stack([torch.zeros_like(phi), torch.sin(phi), torch.cos(phi)])
+    src = DevicePhotons(dirs, pol, energy, torch.ones(n, dtype=torch.float64, device=dev),
+                        device=dev)
+    return src, u[3:].contiguous()
+
+
+def bench_stack(args, dev):
+    """600 CATL1L2Stack facets + 16 CCDs, 1e7 photons."""
+    import numpy as np
+    import torch
+    from marxs_amd import configs, catgrating
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons, raise_for_status
+    from marxs_amd.batch import outputs_like
+    si = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tests', 'golden',
+                      'SiTransmission.csv')
+    inst, parts = configs.cat_rowland_instrument(
+        args.facets, facet_class=catgrating.CATL1L2Stack, facet_kwargs={'l1_transmission': si})
+    ci = CompiledInstrument(inst, grid=bool(args.grid), device=dev)
+    n = args.photons
+    src, uni = gpu_inputs(ci, n, args.seed + 5, dev)
+    status = torch.zeros(8, dtype=torch.int64, device=dev)
+    outs = {p: outputs_like(src) for p in (32, 64)}
+    ms = {}
+    for prec in (32, 64):
+        for _ in range(args.warmup):
+            ci.launch(outs[prec], uni, precision=prec, src=src, status=status)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            ci.launch(outs[prec], uni, precision=prec, src=src, status=status)
+        b.record()
+        torch.cuda.synchronize()
+        ms[prec] = a.elapsed_time(b) / args.steps
+    raise_for_status(status.cpu().numpy())
+    o32, o64 = outs[32], outs[64]
+    same = ((o32.icols['CCD'] == o64.icols['CCD']) & (o64.icols['CCD'] >= 0)
+            & (o32.icols['facet'] == o64.icols['facet'])
+            & (o32.icols['order_L1'] == o64.icols['order_L1']))
+    dx = (o32.fcols['det_x'] - o64.fcols['det_x'])[same]
+    dy = (o32.fcols['det_y'] - o64.fcols['det_y'])[same]
+    rms = float(torch.sqrt(torch.mean(dx * dx + dy * dy)) * 1e3)
+    res = {'workload': f'cat_rowland_{args.facets} with CATL1L2Stack facets, {n} photons, '
+                       'line spectrum',
+           'photons_per_s_fp32': n / ms[32] * 1e3, 'photons_per_s_fp64': n / ms[64] * 1e3,
+           'kernel_ms_fp32': ms[32], 'kernel_ms_fp64': ms[64],
+           'fp32_rms_um_vs_fp64': rms, 'photons_on_ccds': int((o64.icols['CCD'] >= 0).sum()),
+           'through_l1_bars': int(((o64.icols['order_L1'] == 0) & (o64.icols['facet'] >= 0)).sum())}
+    if args.stack_cpu_sample > 0:
+        os.environ.setdefault('OMP_NUM_THREADS', '1')
+        from oracle import trace as OT
+        from oracle.describe import describe
+        s = min(args.stack_cpu_sample, n)
+        ph = {'dir': src.dir[:, :s].cpu().numpy().T.copy(), 'pol': src.pol[:, :s].cpu().numpy().T.copy(),
+              'energy': src.energy[:s].cpu().numpy(), 'prob': src.prob[:s].cpu().numpy()}
+        t = time.perf_counter()
+        ref = OT.trace(describe(inst), ph, uni[:, :s].cpu().numpy())
+        tc = time.perf_counter() - t
+        eq = all(np.array_equal(ref[c], o64.icols[c][:s].cpu().numpy().astype(np.int64))
+                 for c in ('facet', 'order', 'order_L1'))
+        rel = float(np.nanmax(np.abs(ref['L2Diffraction'] - o64.fcols['L2Diffraction'][:s].cpu().numpy())
+                              / np.maximum(1., np.abs(ref['L2Diffraction']))))
+        res['fp64_sample_indices_equal_oracle'] = bool(eq)
+        res['fp64_sample_scatter_max_rel_err'] = rel
+        res['cpu_baseline'] = {'value': s / tc, 'unit': 'photons/s', 'cores': 1, 'kind': 'port',
+                               'sample': f'first {s} photons, numpy oracle, one thread, {tc:.1f} s'}
+    log('stack', res)
     return res
 
 

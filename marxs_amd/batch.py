@@ -32,7 +32,7 @@ from .tracer import (CompiledInstrument, DevicePhotons, DeviceTables, PREC, rais
 __all__ = ['BatchedTrace', 'variant_cost', 'outputs_like']
 
 BLOCK = {32: 1024, 64: 768}
-BLOCK_STK = 512      # work-items per workgroup of the kernels with stack stages
+BLOCK_STK = 768      # work-items per workgroup of the kernels with stack stages
 
 
 def outputs_like(src):

@@ -46,7 +46,7 @@
 #define MX_BLOCK64 768
 #endif
 #ifndef MX_BLOCK_STK
-#define MX_BLOCK_STK 512        /* kernels with stack stages: 256 VGPRs, no spills */
+#define MX_BLOCK_STK 768        /* kernels with stack stages, both precisions */
 #endif
 #define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
 #ifndef MX_CDF_COUNT
@@ -520,8 +520,25 @@ __device__ __forceinline__ void thin_lens64(d3 c, double f, d3 q, d3 &d, d3 &pol
     pol = np_;
 }
 
+// Out-of-line double transcendentals for the kernels that run stack layers:
+// inlined into the trace loop next to each other their temporaries make the
+// kernel spill; as calls they share one set of registers.
+__device__ __noinline__ double acos_call(double x) { return acos(x); }
+__device__ __noinline__ double asin_call(double x) { return asin(x); }
+__device__ __noinline__ double sin_call(double x) { return sin(x); }
+__device__ __noinline__ double cos_call(double x) { return cos(x); }
+__device__ __noinline__ double pow_call(double x, double y) { return pow(x, y); }
+// Inverse of the standard normal CDF (the scatter's normal deviate from a
+// uniform draw).
+__device__ __noinline__ double inv_normal_cdf(double u) { return normcdfinv(u); }
+
+template <bool CALL> __device__ __forceinline__ double acos64(double x) {
+    return CALL ? acos_call(x) : acos(x);
+}
+
 // The blaze angle alone (grating.py:240-249), as diffract64 / diffract32
 // compute it, for order selectors that depend on it.
+template <bool CALL = false>
 __device__ __forceinline__ double blaze64(const double *et, const double *ep, double ly, d3 d) {
     d3 pp = unit3(d);
     d3 l = ld3(ep), n = ld3(et + 3);
@@ -529,7 +546,7 @@ __device__ __forceinline__ double blaze64(const double *et, const double *ep, do
     d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
     double cb = fabs(dot3(pperp, n));
     cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
-    double blaze = acos(cb);
+    double blaze = acos64<CALL>(cb);
     return blaze + (ep[7] + ly * ep[8]);
 }
 
@@ -547,6 +564,7 @@ __device__ __forceinline__ float blaze32(f3 n, FP pr, float ly, f3 d) {
 
 // FlatGrating.diffract_photons (marxs/optics/grating.py:233-266), FP64 in
 // the reference's operation order.  Returns the blaze angle.
+template <bool CALL = false>
 __device__ __forceinline__ double diffract64(const double *et, const double *ep, int sign_mode,
                                              double trans, double energy, double ly, int m,
                                              d3 &d, d3 &pol) {
@@ -557,7 +575,7 @@ __device__ __forceinline__ double diffract64(const double *et, const double *ep,
     d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
     double cb = fabs(dot3(pperp, n));
     cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
-    double blaze = acos(cb);
+    double blaze = acos64<CALL>(cb);
     blaze = blaze + (ep[7] + ly * ep[8]);
     double pd0 = dot3(pp, dg);
     double sgn;
@@ -628,18 +646,13 @@ __device__ __forceinline__ double trans_interp(const double *x, const double *y,
     return slope * (e - x[l]) + y[l];
 }
 
-// Inverse of the standard normal CDF (the scatter's normal deviate from a
-// uniform draw).  Not inlined: inlined into the trace loop its register
-// demand makes the kernel spill.
-__device__ __noinline__ double inv_normal_cdf(double u) { return normcdfinv(u); }
-
 // The reference's scatter applies transforms3d's axangle2mat(axis, angle)
 // as einsum('...ij,...i->...j', M, v) (marxs/optics/scatter.py:133-139,
 // marxs/math/rotations.py:54-87): the transposed matrix, axis normalised.
 __device__ __forceinline__ d3 axangle_t(d3 ax, double ang, d3 v) {
     ax = unit3(ax);
     const double x = ax.x, y = ax.y, z = ax.z;
-    const double c = cos(ang), s = sin(ang), C = 1 - c;
+    const double c = cos_call(ang), s = sin_call(ang), C = 1 - c;
     const double xs = x * s, ys = y * s, zs = z * s;
     const double xC = x * C, yC = y * C, zC = z * C;
     const double xyC = x * yC, yzC = y * zC, zxC = z * xC;
@@ -693,9 +706,9 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             int m; double ptot;
             const d3 d_old = d, pol_old = pol;
             if (!select_order<PREC>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds,
-                                    [&]() { return blaze64(et, gp, ly, d_old); }))
+                                    [&]() { return blaze64<true>(et, gp, ly, d_old); }))
                 count(cnt, MX_ST_ORDER_EMPTY);
-            const double blaze = diffract64(et, gp, l1 ? ei[4] : ei[2], (l1 ? ei[5] : ei[3]) ? 1.0 : -1.0,
+            const double blaze = diffract64<true>(et, gp, l1 ? ei[4] : ei[2], (l1 ? ei[5] : ei[3]) ? 1.0 : -1.0,
                                             energy, ly, m, d, pol);
             if (l1) {
                 const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
@@ -720,19 +733,19 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             R.prob = R.prob * ptot;
         } else if (op == MX_OP_QF) {
             const double fm = (double)m_last;
-            const double f = pow(sr[9], fm * fm);
+            const double f = pow_call(sr[9], fm * fm);
             if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
             R.prob = R.prob * f;
         } else if (op == MX_OP_L2ABS) {
-            const double ang = acos(fabs(dot3(unit3(d), ld3(et + 3))));
-            const double f = sr[11] - sr[12] * sin(ang);
+            const double ang = acos_call(fabs(dot3(unit3(d), ld3(et + 3))));
+            const double f = sr[11] - sr[12] * sin_call(ang);
             if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
             R.prob = R.prob * f;
         } else if (op == MX_OP_SCAT_L2 || op == MX_OP_SCAT_CONST) {
             double sigma;
             if (op == MX_OP_SCAT_L2) {
                 const double wave = (MX_KEV_NM * 1e-6) / energy;
-                sigma = (1.22 * 0.4) * asin(wave / sr[13]);
+                sigma = (1.22 * 0.4) * asin_call(wave / sr[13]);
             } else {
                 sigma = sr[13];
             }
@@ -941,9 +954,9 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             if (PREC == 64) {
                 id = ei[0];
                 if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds,
-                                        [&]() { return blaze64(et, ep, ly, R.d); }))
+                                        [&]() { return blaze64<STK>(et, ep, ly, R.d); }))
                     count(cnt, MX_ST_ORDER_EMPTY);
-                blaze = diffract64(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
+                blaze = diffract64<STK>(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
             } else {
                 const float wave = MX_E2W_F / (float)R.energy;
                 if (L && si[22] >= 0) {
