@@ -33,14 +33,14 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 13
+#define MX_ABI_VERSION 14
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
 #define MX_NSTATUS 8
 
 /* record widths of the instrument tables */
-#define MX_SI 48   /* int32 per stage   */
+#define MX_SI 64   /* int32 per stage   */
 #define MX_SD 32   /* double per stage  */
 #define MX_ET 16   /* double per element, intersection data */
 #define MX_EP 16   /* double per element, interaction parameters */
@@ -55,16 +55,19 @@ extern "C" {
 #define MX_STAGE_GRATING         5
 #define MX_STAGE_DETECTOR        6
 #define MX_STAGE_STACK           7   /* FlatStack: one intersection, a program of layers */
+#define MX_STAGE_CATSUPPORT      8   /* catsupportbars: probability 0 where the facet id < 0 */
 
 /* layer operations of a stack stage (stage_i[47]: bits 0-3 count, op k in
  * bits 4+3k): membrane grating, quality factor, L1 cross-support, L2
- * absorption, L2 diffraction scatter, Gaussian scatter of fixed sigma */
+ * absorption, L2 diffraction scatter, Gaussian scatter of fixed sigma,
+ * radial mirror scatter */
 #define MX_OP_GRATING    1
 #define MX_OP_QF         2
 #define MX_OP_L1         3
 #define MX_OP_L2ABS      4
 #define MX_OP_SCAT_L2    5
 #define MX_OP_SCAT_CONST 6
+#define MX_OP_RADIAL     7
 #define MX_STACK_DRAWS   5   /* uniforms per stack interaction */
 
 /* precision of mx_trace */

@@ -29,7 +29,7 @@ from .simulator import BaseContainer
 __all__ = ['CompiledTables', 'compile_tree', 'flatten', 'build_grid',
            'STAGE_APERTURE_CIRCLE', 'STAGE_APERTURE_RECT', 'STAGE_LENS_PERFECT',
            'STAGE_LENS_THIN', 'STAGE_GRATING', 'STAGE_DETECTOR', 'STAGE_STACK',
-           'ORDER_NONE', 'STACK_DRAWS']
+           'STAGE_CATSUPPORT', 'ORDER_NONE', 'STACK_DRAWS']
 
 STAGE_APERTURE_CIRCLE = 1
 STAGE_APERTURE_RECT = 2
@@ -38,22 +38,23 @@ STAGE_LENS_THIN = 4
 STAGE_GRATING = 5
 STAGE_DETECTOR = 6
 STAGE_STACK = 7
+STAGE_CATSUPPORT = 8    # catsupportbars: probability 0 where the facet id is < 0
 
 # layer operations of a stack stage, 3 bits each (stage_i[47]: bits 0-3 the
 # number of operations, operation k in bits 4 + 3k)
-OP_GRATING, OP_QF, OP_L1, OP_L2ABS, OP_SCAT_L2, OP_SCAT_CONST = 1, 2, 3, 4, 5, 6
+OP_GRATING, OP_QF, OP_L1, OP_L2ABS, OP_SCAT_L2, OP_SCAT_CONST, OP_RADIAL = 1, 2, 3, 4, 5, 6, 7
 STACK_DRAWS = 5     # draws per stack interaction: order, L1 order, L1 open, normal, angle
-SR = 16             # float64 per stack record
+SR = 20             # float64 per stack record
 
-SI, SD, ET, EP, EI, SELI = 48, 32, 16, 16, 8, 12
+SI, SD, ET, EP, EI, SELI = 64, 32, 16, 16, 8, 12
 ORDER_NONE = -2 ** 31
 
 # output column fields of a stage record, by bit (stage_i[10:19], then the
 # stack's second local-coordinate pair, L1 order and blaze angle and scatter
-# angle in stage_i[42:47])
+# angle in stage_i[42:47], the second scatter angle in stage_i[48])
 COL_FIELDS = ['loc0', 'loc1', 'id', 'order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1',
-              'loc0b', 'loc1b', 'order2', 'blaze2', 'scat']
-COL_SLOT = [10, 11, 12, 13, 14, 15, 16, 17, 18, 42, 43, 44, 45, 46]
+              'loc0b', 'loc1b', 'order2', 'blaze2', 'scat', 'scat2']
+COL_SLOT = [10, 11, 12, 13, 14, 15, 16, 17, 18, 42, 43, 44, 45, 46, 48]
 INT_FIELDS = {'id', 'order', 'pix0', 'pix1', 'order2'}
 
 
@@ -80,7 +81,8 @@ def flatten(tree):
 
 
 def _is_stack_layer(e):
-    return isinstance(e, (CG.QualityFactor, CG.L1, CG.L2Abs, SC.RandomGaussianScatter))
+    return isinstance(e, (CG.QualityFactor, CG.L1, CG.L2Abs, SC.RandomGaussianScatter,
+                          SC.RadialMirrorScatter))
 
 
 def stack_layers(e):
@@ -121,6 +123,8 @@ def stack_program(e):
             prog.append((OP_L2ABS, ly))
         elif isinstance(ly, CG.L2Diffraction):
             prog.append((OP_SCAT_L2, ly))
+        elif isinstance(ly, SC.RadialMirrorScatter):
+            prog.append((OP_RADIAL, ly))
         elif isinstance(ly, SC.RandomGaussianScatter):
             if callable(ly.scatter):
                 raise CompileError('a callable scatter has no GPU implementation')
@@ -130,7 +134,8 @@ def stack_program(e):
             raise CompileError('{0} has no GPU implementation as a stack layer'.format(
                 type(ly).__name__))
     ops = [op for op, _ in prog]
-    for group in ((OP_GRATING,), (OP_QF,), (OP_L1,), (OP_L2ABS,), (OP_SCAT_L2, OP_SCAT_CONST)):
+    for group in ((OP_GRATING,), (OP_QF,), (OP_L1,), (OP_L2ABS,),
+                  (OP_SCAT_L2, OP_SCAT_CONST, OP_RADIAL)):
         if sum(op in group for op in ops) > 1:
             raise CompileError('a stack is compiled with at most one layer of each type')
     return prog
@@ -144,6 +149,8 @@ def encode_program(prog):
 
 
 def leaf_kind(e):
+    if e is CG.catsupportbars:
+        return STAGE_CATSUPPORT
     if isinstance(e, O.FlatStack) or _is_stack_layer(e):
         stack_program(e)
         return STAGE_STACK
@@ -167,8 +174,10 @@ def leaf_kind(e):
 
 def leaf_columns(e, kind):
     """Output column names of a leaf, by field."""
-    loc = getattr(e, 'loc_coos_name', None)
     cols = {f: None for f in COL_FIELDS}
+    if kind == STAGE_CATSUPPORT:
+        return cols
+    loc = getattr(e, 'loc_coos_name', None)
     if loc is not None:
         cols['loc0'], cols['loc1'] = loc[0], loc[1]
     cols['id'] = e.id_col
@@ -193,6 +202,8 @@ def leaf_columns(e, kind):
                 cols['order2'], cols['blaze2'] = ly.order_name, ly.blaze_name
             elif op in (OP_SCAT_L2, OP_SCAT_CONST):
                 cols['scat'] = ly.scattername
+            elif op == OP_RADIAL:
+                cols['scat'], cols['scat2'] = ly.inplanescattercol, ly.perpplanescattercol
     if kind == STAGE_GRATING:
         cols['order'] = e.order_name
         cols['blaze'] = e.blaze_name
@@ -273,7 +284,7 @@ def _group(leaves):
         kind = leaf_kind(e)
         cols = leaf_columns(e, kind)
         prog = encode_program(stack_program(e)) if kind == STAGE_STACK else 0
-        aperture = kind in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT)
+        aperture = kind in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT, STAGE_CATSUPPORT)
         if (stages and not aperture and stages[-1].kind == kind
                 and stages[-1].columns == cols and stages[-1].program == prog):
             stages[-1].elements.append(e)
@@ -415,6 +426,21 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
     e = 0
     for s, st in enumerate(stages):
         st.elem0 = e
+        if st.kind == STAGE_CATSUPPORT:
+            # catgrating.py:314-324 reads the 'facet' id column written by an
+            # earlier stage (the function occupies an unused element row)
+            if icol_kind.get('facet') != 'id':
+                raise CompileError('catsupportbars is compiled only after a stage that '
+                                   'writes the facet id column "facet"')
+            rec = stage_i[s]
+            rec[:] = -1
+            rec[:10] = 0
+            rec[0] = STAGE_CATSUPPORT
+            rec[2] = e
+            rec[6] = icols.index('facet')
+            rec[19] = 0
+            e += len(st.elements)
+            continue
         frames = [plane_frame(x.geometry) for x in st.elements]
         if st.kind in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT):
             st.uslot, st.n_uslot = uslot, 2
@@ -815,8 +841,9 @@ def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
     quality factor (9), L1 open fraction (10), L2 open fraction and
     shadowing coefficient (11, 12), scatter parameter (13: sigma in rad, or
     the L2 inner free width in mm), offset and length of the L1 transmission
-    table in sel_d (14, 15: energies in keV, then transmissions).  L1
-    selector row, sign mode and transmission flag in ei[7], ei[4], ei[5]."""
+    table in sel_d (14, 15: energies in keV, then transmissions), radial
+    scatter: perpendicular sigma (16) and centre (17-19).  L1 selector row,
+    sign mode and transmission flag in ei[7], ei[4], ei[5]."""
     rec = np.zeros(SR)
     ei[4] = ei[5] = ei[7] = -1
     for op, ly in stack_program(x):
@@ -847,5 +874,9 @@ def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
             rec[13] = ly.innerfree
         elif op == OP_SCAT_CONST:
             rec[13] = float(ly.scatter)
+        elif op == OP_RADIAL:
+            rec[13] = ly.inplanescatter
+            rec[16] = ly.perpplanescatter
+            rec[17:20] = np.asarray(ly.pos4d, dtype=float)[:3, 3]
     ei[6] = len(sel_d)
     sel_d.extend(rec.tolist())

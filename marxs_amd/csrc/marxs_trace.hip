@@ -467,6 +467,7 @@ __device__ __forceinline__ void put_defaults(const MxPhotons &P, SIP si, int64_t
     if (always & 2048) put_i(P, si[44], i, MX_ORDER_NONE);
     if (always & 4096) put_f(P, si[45], i, qnan);
     if (always & 8192) put_f(P, si[46], i, qnan);
+    if (always & 16384) put_f(P, si[48], i, qnan);
 }
 
 __device__ __forceinline__ int pixel_index(double detpix, int npix) {
@@ -679,11 +680,16 @@ __device__ __forceinline__ d3 axangle_t(d3 ax, double ang, d3 v) {
 //           dir x guess, then 2 pi draw 4 about dir, polarization
 //           parallel-transported; sigma = 1.22 0.4 asin(lambda / inner
 //           free width) for L2 diffraction (catgrating.py:252-271)
+//  radial   (scatter.py:49-78) radial = last interaction point - centre;
+//           Phi^-1(draw 3) sigma_in about dir x radial, then
+//           Phi^-1(draw 4) sigma_perp about radial (a zero sigma skips its
+//           rotation), polarization parallel-transported
 // Draw j of the k-th interaction in the stage: slot 5 k + j (cyclic).
 // Stack record sr (float64, sel_d + ei[6]): L1 l (0-2), d (3-5), period (6),
 // blaze centre and slope (7, 8), quality factor (9), L1 open fraction (10),
 // L2 open fraction and coefficient (11, 12), scatter parameter (13),
-// transmission table offset and length (14, 15).
+// transmission table offset and length (14, 15), radial scatter sigma_perp
+// (16) and centre (17-19).
 template <int PREC>
 __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons &P, Ray &R, const cptr<int> si,
                                              const double *et, const double *ep, const int *ei, double ly,
@@ -759,6 +765,23 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             pol = ptrans64(d, nd, pol);
             d = nd;
             put_f(P, si[46], i, ang);
+        } else if (op == MX_OP_RADIAL) {
+            const d3 radial = sub3(R.p, ld3(sr + 17));
+            const d3 perpplane = cross3(d, radial);
+            d3 nd = d;
+            double a_in = 0.0, a_perp = 0.0;
+            if (sr[13] != 0.0) {
+                a_in = inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 3, i, cnt)) * sr[13];
+                nd = axangle_t(perpplane, a_in, d);
+            }
+            if (sr[16] != 0.0) {
+                a_perp = inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 4, i, cnt)) * sr[16];
+                nd = axangle_t(radial, a_perp, nd);
+            }
+            pol = ptrans64(d, nd, pol);
+            d = nd;
+            put_f(P, si[46], i, a_in);
+            put_f(P, si[48], i, a_perp);
         } else {
             count(cnt, MX_ST_BAD_ARGS);
         }
@@ -1114,6 +1137,14 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             put_f(P, si[10], i, x);
             put_f(P, si[11], i, y);
             put_i(P, si[12], i, cp(T.elem_i)[(size_t)e0 * MX_EI]);
+            continue;
+        }
+        if (kind == MX_STAGE_CATSUPPORT) {
+            // catsupportbars (marxs/missions/mitsnl/catgrating.py:314-324):
+            // the facet frame absorbs photons that met no facet; si[6] is
+            // the facet id column, written by an earlier stage of this trace
+            const int c = si[6];
+            if (c < 0 || P.icol[c][i] < 0) R.prob = 0.0;
             continue;
         }
         if (kind < MX_STAGE_LENS_PERFECT || kind > (STK ? MX_STAGE_STACK : MX_STAGE_DETECTOR)) {

@@ -43,7 +43,8 @@ def _selector(sel):
     raise TypeError(name)
 
 
-LAYER_TYPES = ('QualityFactor', 'L1', 'L2Abs', 'L2Diffraction', 'RandomGaussianScatter')
+LAYER_TYPES = ('QualityFactor', 'L1', 'L2Abs', 'L2Diffraction', 'RandomGaussianScatter',
+               'RadialMirrorScatter')
 
 
 def _layer(e):
@@ -72,6 +73,11 @@ def _layer(e):
         totalarea = e.period ** 2 / 2 * np.sqrt(3)
         return {'op': 'l2abs', 'openfraction': openfraction,
                 'coef': e.bardepth * e.innerfree / totalarea, 'loc': loc}
+    if name == 'RadialMirrorScatter':
+        return {'op': 'radial', 'sigma_in': float(e.inplanescatter),
+                'sigma_perp': float(e.perpplanescatter), 'name_in': e.inplanescattercol,
+                'name_perp': e.perpplanescattercol,
+                'center': np.array(e.pos4d, dtype=float)[:3, 3], 'loc': loc}
     if name in ('L2Diffraction', 'RandomGaussianScatter'):
         if name == 'L2Diffraction':
             return {'op': 'scatter', 'mode': 'l2', 'innerfree': e.innerfree,
@@ -84,6 +90,9 @@ def _layer(e):
 def describe(tree):
     out = []
     for e in _leaves(tree):
+        if callable(e) and getattr(e, '__name__', '') == 'catsupportbars':
+            out.append({'kind': 'catsupportbars'})
+            continue
         name = type(e).__name__
         if _is_stack(e) or name in LAYER_TYPES:
             stack = _is_stack(e)

@@ -30,6 +30,18 @@ tests/test_setup.py; committed vectors in tests/golden):
 * ``thin_lens`` -- focusing of paraxial rays (closed form) and unit
   rotation invariants.
 * ``detector`` -- ``test_detector.py:16-35`` pixel answers.
+* ``stack_layers`` (tests/test_oracle_stack.py) -- CAT support answers of
+  ``marxs/missions/mitsnl/tests/test_cat.py`` (Si transmission 0.22458 of
+  4 um at 1 keV; L2 open fraction 0.81 on axis, smaller off axis; L2
+  broadening ~0.5 arcsec at 0.25 keV; NonParallelCATGrating == CATGrating
+  bit for bit), the L1 table interpolation against scipy's ``interp1d``,
+  the scatter answers of ``marxs/optics/tests/test_scatter.py`` (widths
+  arctan(sigma) on a detector 1 mm away; exact zeros for sigma 0 and for the
+  coordinate a rotation cannot reach), the transposed axis-angle matrix
+  against scipy's ``Rotation``, and invariants (unit directions,
+  polarization perpendicular, probabilities in [0, 1]).
+* ``catsupportbars`` -- the rule itself (probability 0 where the facet id is
+  negative), checked on a hand-made table.
 """
 import math
 
@@ -353,8 +365,11 @@ def columns_of(el):
     kind = el['kind']
     cols = {'loc0': None, 'loc1': None, 'id': el.get('id_col'), 'order': None,
             'blaze': None, 'detpix0': None, 'detpix1': None, 'pix0': None, 'pix1': None}
+    if kind == 'catsupportbars':
+        return {k: None for k in cols}
     if kind == 'stack':
-        cols.update({'loc0b': None, 'loc1b': None, 'order2': None, 'blaze2': None, 'scat': None})
+        cols.update({'loc0b': None, 'loc1b': None, 'order2': None, 'blaze2': None, 'scat': None,
+                     'scat2': None})
         pairs = stack_loc_pairs(el)
         if len(pairs) > 2:
             raise ValueError('more than two local-coordinate column pairs in a stack')
@@ -369,6 +384,8 @@ def columns_of(el):
                 cols['order2'], cols['blaze2'] = ly['order_name'], ly['blaze_name']
             elif ly['op'] == 'scatter' and not (ly['mode'] == 'const' and ly['sigma'] == 0):
                 cols['scat'] = ly['name']
+            elif ly['op'] == 'radial':
+                cols['scat'], cols['scat2'] = ly['name_in'], ly['name_perp']
         return cols
     default_loc = {'aperture_circle': ['y', 'z'], 'aperture_rect': ['y', 'z'],
                    'lens_perfect': ['mirror_x', 'mirror_y'], 'lens_thin': ['y', 'z'],
@@ -390,15 +407,17 @@ def columns_of(el):
 
 def uniform_slots(elements, n_draws=2):
     """Slot base per element: consecutive elements of one kind writing the same
-    columns share a stage; an aperture stage takes 2 draws (phi or y, then r
-    or z), a grating stage ``n_draws`` (k-th interaction of a photon in the
-    stage uses draw k, draws reused cyclically beyond that)."""
+    columns (stacks: and with the same layer operations) share a stage; an
+    aperture stage takes 2 draws (phi or y, then r or z), a grating stage
+    ``n_draws`` (k-th interaction of a photon in the stage uses draw k, draws
+    reused cyclically beyond that), a stack stage 5 per interaction."""
     out = []
     slot = 0
     prev = None
     stage_id = -1
     for el in elements:
-        key = (el['kind'], tuple(sorted((k, v) for k, v in columns_of(el).items() if v)))
+        key = (el['kind'], tuple(sorted((k, v) for k, v in columns_of(el).items() if v)),
+               stack_ops(el))
         new_stage = (prev is None or key != prev or el['kind'] in APERTURES)
         if new_stage:
             n = (2 if el['kind'] in APERTURES else
@@ -415,6 +434,22 @@ def uniform_slots(elements, n_draws=2):
 
 
 INT_COLS = ('id', 'order', 'pix0', 'pix1', 'order2')
+
+
+def stack_ops(el):
+    """The layer operations of a stack that change photons (a constant
+    scatter of sigma 0 does nothing); () for other elements."""
+    if el['kind'] != 'stack':
+        return ()
+    ops = []
+    for ly in el['layers']:
+        if ly['op'] == 'scatter':
+            if ly['mode'] == 'const' and ly['sigma'] == 0:
+                continue
+            ops.append(('scatter', ly['mode']))
+        else:
+            ops.append((ly['op'],))
+    return tuple(ops)
 
 
 def axangle_t_apply(axes, angles, v):
@@ -510,6 +545,28 @@ def stack_layers(el, fr, ph, hit, ly, uniforms, base, nslot, k, cols_out):
             f = layer['openfraction'] - layer['coef'] * np.sin(angle)
             _check_prob(f)
             ph['prob'][hit] = ph['prob'][hit] * f
+        elif op == 'radial':
+            # RadialMirrorScatter.specific_process_photons, scatter.py:49-78:
+            # radial from the centre to the last interaction point (the
+            # position before this element), rotations applied transposed
+            d0 = ph['dir'][hit]
+            radial = ph['pos'][hit] - layer['center'][None, :]
+            perpplane = np.cross(d0, radial)
+            if layer['sigma_in'] != 0:
+                inplane = ndtri(u(3)) * layer['sigma_in']
+                out = axangle_t_apply(perpplane, inplane, d0)
+            else:
+                inplane = np.zeros(len(d0))
+                out = d0.copy()
+            if layer['sigma_perp'] != 0:
+                perp = ndtri(u(4)) * layer['sigma_perp']
+                out = axangle_t_apply(radial, perp, out)
+            else:
+                perp = np.zeros(len(d0))
+            ph['pol'][hit] = parallel_transport(d0, out, pol_old)
+            ph['dir'][hit] = out
+            cols_out['scat'] = inplane
+            cols_out['scat2'] = perp
         elif op == 'scatter':
             if layer['mode'] == 'const' and layer['sigma'] == 0:
                 continue
@@ -556,8 +613,16 @@ def trace(elements, photons, uniforms, n_draws=2):
         if stage_id != cur_stage:
             count = np.zeros(n, dtype=np.int64)
             cur_stage = stage_id
-        fr = frame(el['pos4d'])
         kind = el['kind']
+        if kind == 'catsupportbars':
+            # catgrating.py:314-324: the facet frame absorbs photons that
+            # met no facet (all photons when there is no facet column)
+            if 'facet' in cols:
+                ph['prob'][cols['facet'] < 0] = 0.
+            else:
+                ph['prob'][:] = 0.
+            continue
+        fr = frame(el['pos4d'])
         cnames = columns_of(el)
         if kind in APERTURES:
             hit = np.ones(n, dtype=bool)
@@ -636,7 +701,7 @@ def trace(elements, photons, uniforms, n_draws=2):
         if kind == 'stack':
             extra['loc0b'], extra['loc1b'] = ly, lz
         for key in ('order', 'blaze', 'detpix0', 'detpix1', 'pix0', 'pix1', 'order2',
-                    'blaze2', 'scat'):
+                    'blaze2', 'scat', 'scat2'):
             if key in out:
                 extra[key] = out[key]
         for key, val in extra.items():

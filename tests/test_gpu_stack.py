@@ -190,3 +190,49 @@ def test_batched_stack_variants_equal_single(gpu):
         for k in a:
             x, y = a[k].cpu().numpy(), b[k].cpu().numpy()
             assert np.array_equal(x, y, equal_nan=x.dtype.kind == 'f'), k
+
+
+def test_radial_mirror_scatter_and_catsupportbars(gpu):
+    """Aperture, lens, RadialMirrorScatter at the lens (blur in and across
+    the plane of reflection), CAT stack facets, catsupportbars, CCDs."""
+    from marxs_amd.scatter import RadialMirrorScatter
+    inst0, parts = _stack_instrument(200)
+    rms = RadialMirrorScatter(inplanescatter=2e-5, perpplanescatter=4e-6,
+                              position=[12500., 0, 0], zoom=[1, 700, 700])
+    inst = mx.Sequence(elements=[parts['aperture'], parts['lens'], rms, parts['gratings'],
+                                 CG.catsupportbars, parts['ccds']])
+    ci = _ci(inst)
+    ph, u = make_inputs(40000, 31, ci.n_uniform_slots)
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(ci, ph, u, precision=64)
+    compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    assert np.all(got['prob'][got['facet'] < 0] == 0)
+    assert (got['inplanescatter'] != 0).all() and (got['perpplanescatter'] != 0).all()
+    got32, st32 = gpu_trace(ci, ph, u, precision=32)
+    both = (ref['CCD'] >= 0) & (got32['CCD'] == ref['CCD']) & (got32['facet'] == ref['facet'])
+    err = np.hypot(got32['det_x'] - ref['det_x'], got32['det_y'] - ref['det_y'])[both] * 1e3
+    assert both.sum() > 1000 and np.sqrt(np.mean(err ** 2)) <= 0.5
+
+
+@pytest.mark.parametrize('sig', [(0.1, 0.), (0.1, 0.01), (0., 0.1), (0., 0.)])
+def test_radial_scatter_sigmas(gpu, sig):
+    """The reference's scatter test set-ups (test_scatter.py:11-75) on the
+    GPU: photons at (0, 1, 0) and (0, 0, 1), detector 1 mm behind."""
+    from marxs_amd.scatter import RadialMirrorScatter
+    det = mx.FlatDetector(position=[-1, 0, 0], zoom=1000)
+    for pos in ([0., 1., 0.], [0., 0., 1.]):
+        inst = mx.Sequence(elements=[RadialMirrorScatter(inplanescatter=sig[0],
+                                                         perpplanescatter=sig[1]), det])
+        ci = _ci(inst)
+        t = mx.generate_test_photons(2000)
+        ph = {'pos': np.tile(np.array(pos), (2000, 1)),
+              'dir': np.asarray(t['dir'], dtype=float)[:, :3].copy(),
+              'pol': np.asarray(t['polarization'], dtype=float)[:, :3].copy(),
+              'energy': np.asarray(t['energy'], dtype=float),
+              'prob': np.asarray(t['probability'], dtype=float)}
+        u = np.random.default_rng(9).random((ci.n_uniform_slots, 2000))
+        ref = oracle_trace(inst, ph, u)
+        got, st = gpu_trace(ci, ph, u, precision=64)
+        compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+        if pos[1] == 1. and sig[0] == 0.:
+            assert np.all(got['det_x'] == 1.0)

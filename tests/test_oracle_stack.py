@@ -131,3 +131,74 @@ def test_cat_l1l2_stack_columns_and_invariants():
     # |d1 x d2| < 1e-8 (polarization.py:151-170): perpendicular to that level
     assert np.allclose(np.sum(out['dir'] * out['pol'], axis=1), 0, atol=1e-8)
     assert np.array_equal(out['grat_y'], out['y'])
+
+
+def test_axangle_transposed_matches_scipy_rotation():
+    """einsum('...ij,...i->...j', axangle2mat(axis, a), v) rotates v by -a
+    about the axis: scipy's Rotation of rotvec -a * axis / |axis|."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(4)
+    ax = rng.normal(size=(500, 3)) * rng.uniform(0.1, 10, (500, 1))
+    ang = rng.uniform(-3, 3, 500)
+    v = rng.normal(size=(500, 3))
+    got = OT.axangle_t_apply(ax, ang, v)
+    ref = Rotation.from_rotvec(-ang[:, None] * ax / np.linalg.norm(ax, axis=1)[:, None]).apply(v)
+    assert np.allclose(got, ref, rtol=0, atol=1e-13)
+
+
+def _rms_photons(pos):
+    t = mx.generate_test_photons(500)
+    t['pos'] = np.tile(np.array(pos + [1.]), (500, 1))
+    return _ph(t)
+
+
+def test_radial_scatter_in_plane():
+    """test_scatter.py:11-25: photons at (0, 1, 0) scatter in the x-y plane
+    only: det_y exactly 0, det_x width arctan(0.1)."""
+    from marxs_amd.scatter import RadialMirrorScatter
+    det = mx.FlatDetector(position=[-1, 0, 0], zoom=1000)
+    out = run(mx.Sequence(elements=[RadialMirrorScatter(inplanescatter=0.1), det]),
+              _rms_photons([0., 1., 0.]))
+    assert np.all(out['det_y'] == 0)
+    assert np.isclose(np.std(out['det_x']), np.arctan(0.1), rtol=0.1)
+    assert np.all(out['perpplanescatter'] == 0)
+
+
+def test_radial_scatter_turned_with_perpendicular():
+    """test_scatter.py:27-45: 90 deg turned, with perpendicular scatter."""
+    from marxs_amd.scatter import RadialMirrorScatter
+    det = mx.FlatDetector(position=[-1, 0, 0], zoom=1000)
+    out = run(mx.Sequence(elements=[RadialMirrorScatter(inplanescatter=0.1, perpplanescatter=0.01),
+                                    det]), _rms_photons([0., 0., 1.]))
+    assert np.isclose(np.std(out['det_y']), np.arctan(0.1), rtol=0.1)
+    assert np.isclose(np.std(out['det_x']), np.arctan(0.01), rtol=0.1)
+
+
+@pytest.mark.parametrize('sig_in, sig_perp', [(0., 0.1), (0., 0.)])
+def test_radial_scatter_zero_sigmas(sig_in, sig_perp):
+    """test_scatter.py:47-75: a zero sigma leaves that rotation out exactly."""
+    from marxs_amd.scatter import RadialMirrorScatter
+    det = mx.FlatDetector(position=[-1, 0, 0], zoom=1000)
+    out = run(mx.Sequence(elements=[RadialMirrorScatter(inplanescatter=sig_in,
+                                                        perpplanescatter=sig_perp), det]),
+              _rms_photons([0., 1., 0.]))
+    assert np.all(out['det_x'] == 1.0)
+    if sig_perp:
+        assert np.isclose(np.std(out['det_y']), np.arctan(0.1), rtol=0.1)
+    else:
+        assert np.all(out['det_y'] == 0)
+    assert np.all(out['inplanescatter'] == 0)
+
+
+def test_catsupportbars_rule():
+    """catgrating.py:314-324 on a hand-made case: photons that met no facet
+    (facet id -1) get probability 0; without a facet column every photon."""
+    grat = mx.CATGrating(order_selector=mx.OrderSelector([0]), d=2e-4, zoom=[1, 5, 5],
+                         id_col='facet', id_num=4)
+    t = _ph(mx.generate_test_photons(6))
+    t['pos'][:, 1] = [0., 1., 2., 7., 8., -9.]
+    out = run([grat, CG.catsupportbars], t)
+    assert np.array_equal(out['facet'], [4, 4, 4, -1, -1, -1])
+    assert np.array_equal(out['prob'], [1., 1., 1., 0., 0., 0.])
+    out = run([mx.FlatDetector(position=[-5, 0, 0]), CG.catsupportbars], t)
+    assert np.all(out['prob'] == 0)

@@ -100,3 +100,28 @@ def test_l1_transmission_table_validated():
                l1_transmission=(np.array([1.]), np.array([0.1])))
     with pytest.raises(C.CompileError, match='transmission table'):
         C.compile_tree(l1)
+
+
+def test_radial_scatter_record_and_columns():
+    from marxs_amd.scatter import RadialMirrorScatter
+    rms = RadialMirrorScatter(inplanescatter=1e-5, perpplanescatter=2e-6, position=[5, 1, 2])
+    t = C.compile_tree(rms)
+    assert _decode(t.stage_i[0, 47]) == [C.OP_RADIAL]
+    rec = t.sel_d[t.elem_i[0, 6]:t.elem_i[0, 6] + C.SR]
+    assert rec[13] == 1e-5 and rec[16] == 2e-6 and np.array_equal(rec[17:20], [5, 1, 2])
+    assert t.fcols[t.stage_i[0, 46]] == 'inplanescatter'
+    assert t.fcols[t.stage_i[0, 48]] == 'perpplanescatter'
+    # one scatter layer per stack
+    st = mx.FlatStack(elements=[RadialMirrorScatter, RandomGaussianScatter],
+                      keywords=[{'inplanescatter': 1e-5}, {'scatter': 1e-5}])
+    with pytest.raises(C.CompileError, match='at most one layer'):
+        C.compile_tree(st)
+
+
+def test_catsupportbars_stage():
+    grat = mx.CATGrating(order_selector=mx.OrderSelector([0]), d=2e-4, id_col='facet', id_num=2)
+    t = C.compile_tree([grat, CG.catsupportbars, mx.FlatDetector(position=[-5, 0, 0])])
+    assert [s.kind for s in t.stages] == [C.STAGE_GRATING, C.STAGE_CATSUPPORT, C.STAGE_DETECTOR]
+    assert t.stage_i[1, 6] == t.icols.index('facet') and t.n_uniform_slots == 2
+    with pytest.raises(C.CompileError, match='facet id column'):
+        C.compile_tree([mx.FlatDetector(), CG.catsupportbars])
