@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 14
+#define MX_ABI_VERSION 15
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -56,6 +56,7 @@ extern "C" {
 #define MX_STAGE_DETECTOR        6
 #define MX_STAGE_STACK           7   /* FlatStack: one intersection, a program of layers */
 #define MX_STAGE_CATSUPPORT      8   /* catsupportbars: probability 0 where the facet id < 0 */
+#define MX_STAGE_CYLINDER        9   /* CircularDetector: tube geometry, record in sel_d at elem_i[6] */
 
 /* layer operations of a stack stage (stage_i[47]: bits 0-3 count, op k in
  * bits 4+3k): membrane grating, quality factor, L1 cross-support, L2
@@ -112,7 +113,7 @@ typedef struct MxTables {
     const unsigned char *lds_image32;  /* bytes copied to LDS by each workgroup, FP32 */
     const unsigned char *lds_image64;  /* the same for FP64 */
     int32_t first_draw_slot;   /* uniform slot of the first grating draw (-1: none), prefetched */
-    int32_t has_stack;         /* 1: some stage is MX_STAGE_STACK (selects the kernel that runs them) */
+    int32_t ext_stages;        /* 1: some stage is MX_STAGE_STACK or MX_STAGE_CYLINDER (selects the kernels that run them) */
 } MxTables;
 
 typedef struct MxPhotons {

@@ -16,7 +16,7 @@ from .optics import (OrderSelector, EfficiencyFile, InterpolateEfficiencyTable, 
                      FlatGrating, CATGrating,
                      NonParallelCATGrating, CircleAperture, RectangleAperture,
                      PerfectLens, ThinLens, FlatDetector, FlatOpticalElement,
-                     OpticalElement, FlatStack)
+                     OpticalElement, FlatStack, CircularDetector)
 from .scatter import RandomGaussianScatter  # noqa: E402
 from .catgrating import CATL1L2Stack  # noqa: E402
 from .simulator import Sequence, Parallel, ParallelCalculated  # noqa: E402

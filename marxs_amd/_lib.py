@@ -15,7 +15,7 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseAr
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 14
+MX_ABI_VERSION = 15
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -30,7 +30,7 @@ class MxTables(ctypes.Structure):
                 ('elem_i', P), ('elem_t32', P), ('elem_p32', P), ('sel_i', P),
                 ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P),
                 ('succ_start', P), ('succ_items', P), ('lds_image32', P), ('lds_image64', P),
-                ('first_draw_slot', ctypes.c_int32), ('has_stack', ctypes.c_int32)]
+                ('first_draw_slot', ctypes.c_int32), ('ext_stages', ctypes.c_int32)]
 
 
 class MxPhotons(ctypes.Structure):

@@ -53,8 +53,8 @@ def tune_trace(ci, dp, uniforms, precision=32, candidates=(2, 4, 8, 16), reps=3,
 
 
 def _batched_candidates(bt):
-    from .batch import BLOCK, BLOCK_STK
-    bs = BLOCK_STK if bt.has_stack else BLOCK[bt.precision]
+    from .batch import BLOCK, BLOCK_EXT
+    bs = BLOCK_EXT if bt.ext_stages else BLOCK[bt.precision]
     out = []
     for mult in (1, 2, 4, 8):
         for scaled in (False, True):

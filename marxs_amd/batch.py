@@ -32,7 +32,7 @@ from .tracer import (CompiledInstrument, DevicePhotons, DeviceTables, PREC, rais
 __all__ = ['BatchedTrace', 'variant_cost', 'outputs_like']
 
 BLOCK = {32: 1024, 64: 768}
-BLOCK_STK = 768      # work-items per workgroup of the kernels with stack stages
+BLOCK_EXT = 768      # work-items per workgroup of the kernels with stack or tube stages
 
 
 def outputs_like(src):
@@ -111,7 +111,7 @@ class BatchedTrace:
         self._photon_structs = [ci.photons_struct(dp, u, src) for ci, dp, u, src in
                                 zip(self.instruments, self.outputs, self.uniforms, self.inputs)]
         self.costs = np.array([variant_cost(ci) for ci in self.instruments])
-        self.has_stack = any(ci.dev_tables.struct.has_stack for ci in self.instruments)
+        self.ext_stages = any(ci.dev_tables.struct.ext_stages for ci in self.instruments)
         self.queue = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.graph = None
         self.plan = None
@@ -128,7 +128,7 @@ class BatchedTrace:
         variants first), 'given', or a permutation.  lds_budget: bytes of
         tables staged in LDS per workgroup (None = compile default).
         """
-        bs = BLOCK_STK if self.has_stack else BLOCK[self.precision]
+        bs = BLOCK_EXT if self.ext_stages else BLOCK[self.precision]
         nv = self.n_variants
         if tile_photons is None:
             tile_photons = bs

@@ -24,12 +24,13 @@ from . import catgrating as CG
 from . import scatter as SC
 from .base import SimulationSetupError
 from .geometry import plane_frame
+from .mathutils import decompose44
 from .simulator import BaseContainer
 
 __all__ = ['CompiledTables', 'compile_tree', 'flatten', 'build_grid',
            'STAGE_APERTURE_CIRCLE', 'STAGE_APERTURE_RECT', 'STAGE_LENS_PERFECT',
            'STAGE_LENS_THIN', 'STAGE_GRATING', 'STAGE_DETECTOR', 'STAGE_STACK',
-           'STAGE_CATSUPPORT', 'ORDER_NONE', 'STACK_DRAWS']
+           'STAGE_CATSUPPORT', 'STAGE_CYLINDER', 'ORDER_NONE', 'STACK_DRAWS']
 
 STAGE_APERTURE_CIRCLE = 1
 STAGE_APERTURE_RECT = 2
@@ -39,6 +40,10 @@ STAGE_GRATING = 5
 STAGE_DETECTOR = 6
 STAGE_STACK = 7
 STAGE_CATSUPPORT = 8    # catsupportbars: probability 0 where the facet id is < 0
+STAGE_CYLINDER = 9      # CircularDetector (tube geometry)
+# stages that test flat elements (records, culling grid, LDS staging)
+PLANE_ARRAYS = (3, 4, 5, 6, 7)
+CR = 40                 # float64 per tube record
 
 # layer operations of a stack stage, 3 bits each (stage_i[47]: bits 0-3 the
 # number of operations, operation k in bits 4 + 3k)
@@ -169,6 +174,8 @@ def leaf_kind(e):
         return STAGE_GRATING
     if isinstance(e, O.FlatDetector):
         return STAGE_DETECTOR
+    if isinstance(e, O.CircularDetector):
+        return STAGE_CYLINDER
     raise CompileError('{0} has no GPU implementation'.format(type(e).__name__))
 
 
@@ -207,9 +214,9 @@ def leaf_columns(e, kind):
     if kind == STAGE_GRATING:
         cols['order'] = e.order_name
         cols['blaze'] = e.blaze_name
-    if kind == STAGE_DETECTOR:
+    if kind in (STAGE_DETECTOR, STAGE_CYLINDER):
         cols['detpix0'], cols['detpix1'] = e.detpix_name
-        if e.pixel_name is not None:
+        if getattr(e, 'pixel_name', None) is not None:
             cols['pix0'], cols['pix1'] = e.pixel_name
     return cols
 
@@ -452,7 +459,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
         # stage origin: mean element centre (FP32 tables are relative to it)
         origin = np.mean([f['center'] for f in frames], axis=0)
         g = None
-        if grid and len(st.elements) >= grid_min_elements and st.kind >= STAGE_LENS_PERFECT:
+        if grid and len(st.elements) >= grid_min_elements and st.kind in PLANE_ARRAYS:
             g = build_grid(frames, **grid_kw)
         if g is not None:
             origin = g['origin']
@@ -673,7 +680,7 @@ def _build_lds(tables, budget32=None, budget64=None):
             parts[0] = (0, np.ascontiguousarray(tables.sel_i).astype(np.int32).view(np.uint8).ravel().copy())
         for s, st in enumerate(tables.stages):
             n = len(st.elements)
-            if st.kind < STAGE_LENS_PERFECT or n < 2:
+            if st.kind not in PLANE_ARRAYS or n < 2:
                 continue
             recs = (tables.elem_t32 if prec == 32 else tables.elem_t)[st.elem0:st.elem0 + n]
             params = _param_records32(tables, st) if (prec == 32 and st.kind == STAGE_GRATING) else None
@@ -811,6 +818,8 @@ def _element_params(kind, x, f, ep, ei, sel_index, sel_rows, sel_d, sel_o):
         _grating_params(x, ep, ei, 1, sel_index, sel_rows, sel_d, sel_o)
     elif kind == STAGE_STACK:
         _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o)
+    elif kind == STAGE_CYLINDER:
+        _tube_params(x, ei, sel_d)
     elif kind == STAGE_DETECTOR:
         ep[6] = x.pixsize
         ep[7] = x.centerpix[0]
@@ -878,5 +887,28 @@ def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
             rec[13] = ly.inplanescatter
             rec[16] = ly.perpplanescatter
             rec[17:20] = np.asarray(ly.pos4d, dtype=float)[:3, 3]
+    ei[6] = len(sel_d)
+    sel_d.extend(rec.tolist())
+
+
+def _tube_params(x, ei, sel_d):
+    """Record of CR float64 in sel_d at ei[6] for a tube element
+    (``Cylinder.intersect``, ``geometry.py:495-564``;
+    ``CircularDetector``, ``detector.py:113-116``): inverse pose (0:16) and
+    pose (16:32) row-major, radius R = x zoom (32), z zoom (33), azimuth
+    limits taken modulo 2 pi as ``angle_between`` does (34, 35), pixel size
+    (36), centre pixel (37, 38)."""
+    pos4d = np.asarray(x.pos4d, dtype=float)
+    rec = np.zeros(CR)
+    rec[0:16] = np.linalg.inv(pos4d).ravel()
+    rec[16:32] = pos4d.ravel()
+    zoom = decompose44(pos4d)[2]
+    rec[32], rec[33] = zoom[0], zoom[2]
+    twopi = 2 * np.pi
+    lim = np.asarray(x.geometry.coos_limits[0], dtype=float)
+    rec[34] = (twopi + (lim[0] % twopi)) % twopi
+    rec[35] = (twopi + (lim[1] % twopi)) % twopi
+    rec[36] = x.pixsize
+    rec[37], rec[38] = x.centerpix[0], x.centerpix[1]
     ei[6] = len(sel_d)
     sel_d.extend(rec.tolist())

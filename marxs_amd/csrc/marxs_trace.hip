@@ -45,8 +45,8 @@
 #ifndef MX_BLOCK64
 #define MX_BLOCK64 768
 #endif
-#ifndef MX_BLOCK_STK
-#define MX_BLOCK_STK 768        /* kernels with stack stages, both precisions */
+#ifndef MX_BLOCK_EXT
+#define MX_BLOCK_EXT 768        /* kernels with stack or tube stages, both precisions */
 #endif
 #define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
 #ifndef MX_CDF_COUNT
@@ -529,6 +529,7 @@ __device__ __noinline__ double asin_call(double x) { return asin(x); }
 __device__ __noinline__ double sin_call(double x) { return sin(x); }
 __device__ __noinline__ double cos_call(double x) { return cos(x); }
 __device__ __noinline__ double pow_call(double x, double y) { return pow(x, y); }
+__device__ __noinline__ double atan2_call(double y, double x) { return atan2(y, x); }
 // Inverse of the standard normal CDF (the scatter's normal deviate from a
 // uniform draw).
 __device__ __noinline__ double inv_normal_cdf(double u) { return normcdfinv(u); }
@@ -790,6 +791,93 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
     else { R.df = tof(d); R.polf = tof(pol); }
 }
 
+// ------------------------------------------------------------ tube stages
+// numpy's floor modulo (the sign of the divisor), as Python's % on floats.
+__device__ __forceinline__ double pymod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0.0) != (m < 0.0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+// angle_between (marxs/math/utils.py:180-215) with the borders already
+// reduced to [0, 2 pi).
+__device__ __forceinline__ bool angle_between(double ang, double b1, double b2) {
+    const double twopi = 2 * M_PI;
+    const double a = pymod(twopi + pymod(ang, twopi), twopi);
+    return b1 < b2 ? (b1 <= a && a <= b2) : (b1 <= a || a <= b2);
+}
+
+// m (3 rows of a row-major 4x4) applied to (v, w), summed in index order.
+__device__ __forceinline__ d3 aff3(const double *m, d3 v, double w) {
+    return mk(((m[0] * v.x + m[1] * v.y) + m[2] * v.z) + m[3] * w,
+              ((m[4] * v.x + m[5] * v.y) + m[6] * v.z) + m[7] * w,
+              ((m[8] * v.x + m[9] * v.y) + m[10] * v.z) + m[11] * w);
+}
+
+// Cylinder.intersect (marxs/math/geometry.py:495-564) for one ray: in the
+// local frame (inverse pose) the unit circle gives a t^2 + b t + c = 0;
+// each root is a hit when forward, |z| <= 1 and inside the azimuth limits;
+// of two hits the nearer one counts.  cr: tube record (inverse pose,
+// pose, R, z zoom, limits, pixel size, centre pixel).  Returns the global
+// point, phi and z in global units.
+__device__ __forceinline__ bool tube_hit(const double *cr, d3 p, d3 d, d3 &q, double &phi, double &z) {
+    const d3 xyz = aff3(cr, p, 1.0), dl = aff3(cr, d, 0.0);
+    const double x = xyz.x, y = xyz.y, rx = dl.x, ry = dl.y;
+    const double c = (x * x + y * y) - 1.;
+    const double b = 2 * (x * rx + y * ry);
+    const double a = rx * rx + ry * ry;
+    const double under = b * b - (4 * a) * c;
+    if (!(under >= 0.0)) return false;
+    const double denom = 2 * a;
+    const double root = sqrt(under);
+    const double a1 = (-b + root) / denom, a2 = (-b - root) / denom;
+    const double phi1 = atan2_call(y + a1 * ry, x + a1 * rx);
+    const double phi2 = atan2_call(y + a2 * ry, x + a2 * rx);
+    const double z1 = xyz.z + a1 * dl.z, z2 = xyz.z + a2 * dl.z;
+    bool h1 = (a1 >= 0.0) && (fabs(z1) <= 1.) && angle_between(phi1, cr[34], cr[35]);
+    bool h2 = (a2 >= 0.0) && (fabs(z2) <= 1.) && angle_between(phi2, cr[34], cr[35]);
+    if (h2 && a2 < a1) h1 = false;
+    if (h1 && a2 >= a1) h2 = false;
+    if (!(h1 || h2)) return false;
+    const double t = h1 ? a1 : a2;
+    phi = h1 ? phi1 : phi2;
+    z = (h1 ? z1 : z2) * cr[33];
+    q = aff3(cr + 16, mk(x + t * rx, y + t * ry, xyz.z + t * dl.z), 1.0);
+    return true;
+}
+
+// A stage of tubes (CircularDetector, marxs/optics/detector.py:113-116):
+// elements in ascending order, each with the ray as left by the ones
+// before; detpix_x = phi R / pixsize + centre, detpix_y = z / pixsize +
+// centre.  Float64 in both precisions.
+template <int PREC>
+__device__ __forceinline__ void tube_stage(const MxTables &T, const MxPhotons &P, const cptr<int> si,
+                                           const int64_t i, const Counters &cnt, Ray &R) {
+    const int n_el = si[1], e0 = si[2];
+    const d3 d = PREC == 64 ? R.d : tod(R.df);
+    int nhit = 0;
+    for (int j = 0; j < n_el; ++j) {
+        const int *ei = T.elem_i + (size_t)(e0 + j) * MX_EI;
+        const double *cr = T.sel_d + ei[6];
+        d3 q;
+        double phi, z;
+        if (!tube_hit(cr, R.p, d, q, phi, z)) continue;
+        if (nhit > 0) count(cnt, MX_ST_MULTI_HIT);
+        put_f(P, si[15], i, phi * cr[32] / cr[36] + cr[37]);
+        put_f(P, si[16], i, z / cr[36] + cr[38]);
+        put_f(P, si[10], i, phi);
+        put_f(P, si[11], i, z);
+        put_i(P, si[12], i, ei[0]);
+        R.p = q;
+        ++nhit;
+    }
+    if (nhit == 0) put_defaults(P, si, i);
+}
+
 // Next hit of the ray among candidates [tb, te) (cell list when `list`,
 // element indices otherwise) with index > last, tested in ascending order.
 // ET points at the stage's element records (LDS or global).  FP64: q, ly,
@@ -843,7 +931,7 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
 
 // One array stage (lens, grating facets, detectors) for one photon.  L: the
 // stage's records, parameters and culling lists are in LDS.
-template <int PREC, bool L, bool STK>
+template <int PREC, bool L, bool EXT>
 __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &P, const int s,
                                             const int64_t i, const bool use_grid, const Counters &cnt,
                                             const unsigned char *lds, Ray &R) {
@@ -977,9 +1065,9 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             if (PREC == 64) {
                 id = ei[0];
                 if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds,
-                                        [&]() { return blaze64<STK>(et, ep, ly, R.d); }))
+                                        [&]() { return blaze64<EXT>(et, ep, ly, R.d); }))
                     count(cnt, MX_ST_ORDER_EMPTY);
-                blaze = diffract64<STK>(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
+                blaze = diffract64<EXT>(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
             } else {
                 const float wave = MX_E2W_F / (float)R.energy;
                 if (L && si[22] >= 0) {
@@ -1008,7 +1096,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             R.prob = R.prob * ptot;
             put_i(P, si[13], i, m);
             put_f(P, si[14], i, blaze);
-        } else if (STK && kind == MX_STAGE_STACK) {
+        } else if (EXT && kind == MX_STAGE_STACK) {
             id = ei[0];
             stack_layers<PREC>(T, P, R, si, et, ep, ei, ly, nhit, i, cnt, lds);
             put_f(P, si[42], i, ly);
@@ -1075,7 +1163,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
 
 // ======================================================= the trace kernel
 // One photon through all stages.
-template <int PREC, bool STK>
+template <int PREC, bool EXT>
 __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
                                           const bool use_grid, const Counters &cnt,
                                           const unsigned char *lds) {
@@ -1147,12 +1235,16 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             if (c < 0 || P.icol[c][i] < 0) R.prob = 0.0;
             continue;
         }
-        if (kind < MX_STAGE_LENS_PERFECT || kind > (STK ? MX_STAGE_STACK : MX_STAGE_DETECTOR)) {
+        if (EXT && kind == MX_STAGE_CYLINDER) {
+            tube_stage<PREC>(T, P, si, i, cnt, R);
+            continue;
+        }
+        if (kind < MX_STAGE_LENS_PERFECT || kind > (EXT ? MX_STAGE_STACK : MX_STAGE_DETECTOR)) {
             count(cnt, MX_ST_BAD_ARGS);
             continue;
         }
-        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, STK>(T, P, s, i, use_grid, cnt, lds, R);
-        else array_stage<PREC, false, STK>(T, P, s, i, use_grid, cnt, lds, R);
+        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, EXT>(T, P, s, i, use_grid, cnt, lds, R);
+        else array_stage<PREC, false, EXT>(T, P, s, i, use_grid, cnt, lds, R);
     }
 
     if (PREC == 32) { R.d = tod(R.df); R.pol = tod(R.polf); }
@@ -1165,9 +1257,9 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
 // Persistent kernel: each workgroup copies the LDS image (element records,
 // parameters, culling lists and selector tables of the stages marked in
 // stage_i/sel_i) once, then walks photons with a grid stride.
-// STK: the tables hold stack stages (a separate instantiation, so that the
+// EXT: the tables hold stack or tube stages (a separate instantiation, so that the
 // layer code does not weigh on the register allocation of the others).
-template <int PREC, int BS, bool STK>
+template <int PREC, int BS, bool EXT>
 __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, int flags,
                                                        unsigned long long *status) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -1185,7 +1277,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, i
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
-        trace_one<PREC, STK>(T, P, i, use_grid, cnt, lds);
+        trace_one<PREC, EXT>(T, P, i, use_grid, cnt, lds);
     __syncthreads();
     if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
@@ -1196,7 +1288,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, i
 // tile and finds its variant (binary search over tile_first); the LDS image
 // is reloaded only when the variant changes.  Every exit is uniform over the
 // workgroup (the tile index is read from LDS after a barrier).
-template <int PREC, int BS, bool STK>
+template <int PREC, int BS, bool EXT>
 __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         const MxTables *__restrict__ tabs, const MxPhotons *__restrict__ phs,
         const int64_t *__restrict__ tile_first, const int32_t *__restrict__ tile_photons,
@@ -1250,7 +1342,7 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         int64_t e = b + tp;
         if (e > P.n) e = P.n;
         if (b < 0 || tp <= 0) e = b;
-        for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC, STK>(T, P, i, use_grid, cnt, lds);
+        for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC, EXT>(T, P, i, use_grid, cnt, lds);
         __syncthreads();
         // this tile's counters go to its variant; the next claim's barrier
         // orders the reset before the next tile counts
@@ -1298,8 +1390,8 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
         return 1;
     }
     if (photons->n == 0 || tables->n_stages == 0) return 0;
-    const bool stk = tables->has_stack != 0;
-    const int block = stk ? MX_BLOCK_STK : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
+    const bool ext = tables->ext_stages != 0;
+    const int block = ext ? MX_BLOCK_EXT : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
     const int64_t nblk = (photons->n + block - 1) / block;
     hipStream_t st = (hipStream_t)stream;
     const int lds = precision == MX_PREC_FP64 ? tables->lds_bytes64 : tables->lds_bytes32;
@@ -1316,14 +1408,14 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
     int64_t grid = (int64_t)cus * per_cu;
     if (grid > nblk) grid = nblk;
-    if (precision == MX_PREC_FP64 && stk)
-        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK_STK, true>), dim3((unsigned)grid), dim3(block), lds, st,
+    if (precision == MX_PREC_FP64 && ext)
+        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK_EXT, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else if (precision == MX_PREC_FP64)
         hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64, false>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
-    else if (stk)
-        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK_STK, true>), dim3((unsigned)grid), dim3(block), lds, st,
+    else if (ext)
+        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK_EXT, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else
         hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32, false>), dim3((unsigned)grid), dim3(block), lds, st,
@@ -1352,7 +1444,7 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
         return 1;
     }
     int lds = 0;
-    bool stk = false;
+    bool ext = false;
     for (int v = 0; v < b->n_variants; ++v) {
         if (check_tables(b->tables + v) || check_photons(b->photons + v)) {
             char buf[512];
@@ -1368,10 +1460,10 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
             return 1;
         }
         if (l > lds) lds = l;
-        if (t.has_stack) stk = true;
+        if (t.ext_stages) ext = true;
     }
     if (b->n_tiles == 0) return 0;
-    const int block = stk ? MX_BLOCK_STK : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
+    const int block = ext ? MX_BLOCK_EXT : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
     int64_t grid = b->grid;
     if (grid <= 0) {
         int dev = 0, cus = 0;
@@ -1392,9 +1484,9 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
     hipLaunchKernelGGL((mx_trace_batched_kernel<PR, BSZ, S>), dim3((unsigned)grid), dim3(block), lds, st,  \
                        b->tables_dev, b->photons_dev, b->tile_first, b->tile_photons, b->n_variants,      \
                        (long long)b->n_tiles, (int)flags, b->queue, b->status)
-    if (precision == MX_PREC_FP64 && stk) MX_LAUNCH_BATCHED(64, MX_BLOCK_STK, true);
+    if (precision == MX_PREC_FP64 && ext) MX_LAUNCH_BATCHED(64, MX_BLOCK_EXT, true);
     else if (precision == MX_PREC_FP64) MX_LAUNCH_BATCHED(64, MX_BLOCK64, false);
-    else if (stk) MX_LAUNCH_BATCHED(32, MX_BLOCK_STK, true);
+    else if (ext) MX_LAUNCH_BATCHED(32, MX_BLOCK_EXT, true);
     else MX_LAUNCH_BATCHED(32, MX_BLOCK32, false);
 #undef MX_LAUNCH_BATCHED
     hipError_t err = hipGetLastError();

@@ -15,7 +15,7 @@ import numpy as np
 from .base import parse_position_keywords
 
 __all__ = ['Geometry', 'FinitePlane', 'PlaneWithHole', 'RectangleHole',
-           'CircularHole', 'plane_frame']
+           'CircularHole', 'Cylinder', 'plane_frame']
 
 
 class Geometry:
@@ -74,6 +74,43 @@ class CircularHole(PlaneWithHole):
             raise ValueError('phi[1] - phi[0] must be less than 2 pi.')
         self.phi = phi
         super().__init__(kwargs)
+
+
+class Cylinder(Geometry):
+    """A tube: the unit circle in the local x-y plane, |z| <= 1, mapped by
+    ``pos4d`` (``geometry.py:383-564``).  ``phi_lim`` restricts the tube to a
+    wedge of azimuth (radian, default the full circle).  Local coordinates
+    of a hit are the azimuth phi and z in global units (local z times the z
+    zoom); of the two intersections of a ray with the tube the nearer forward
+    one within the limits counts."""
+    shape = 'surface'
+    loc_coos_name = ['phi', 'z']
+
+    def __init__(self, kwargs=None):
+        kwargs = {} if kwargs is None else kwargs
+        self.coos_limits = [np.asanyarray(kwargs.pop('phi_lim', [-np.pi, np.pi]), dtype=float),
+                            np.array([-1., 1.])]
+        super().__init__(kwargs)
+
+    def __getitem__(self, key):
+        if key == 'R':
+            from .mathutils import decompose44
+            return decompose44(self.pos4d)[2][0]
+        return super().__getitem__(key)
+
+    @classmethod
+    def from_rowland(cls, rowland, width, rotation=0., kwargs=None):
+        """The tube that follows the Rowland circle of ``rowland``
+        (``geometry.py:430-462``): radius r around (R, 0, 0) of the torus
+        frame, half-width ``width`` along the torus axis, turned by
+        ``rotation`` about the tube axis."""
+        from .mathutils import axangle2mat, compose
+        rot = axangle2mat(np.array([0, 0, 1.]), rotation)
+        pos4d = np.dot(rowland.pos4d, compose([rowland.R, 0, 0], rot,
+                                              [rowland.r, rowland.r, width]))
+        det_kwargs = {'pos4d': pos4d}
+        det_kwargs.update(kwargs or {})
+        return cls(det_kwargs)
 
 
 def plane_frame(geometry):

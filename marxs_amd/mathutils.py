@@ -100,16 +100,16 @@ def decompose44(a):
     a = np.asarray(a, dtype=float)
     t = a[:3, 3].copy()
     c0, c1, c2 = (a[:3, i].copy() for i in range(3))
-    sx = np.sqrt(np.dot(c0, c0))
+    sx = np.sqrt(np.sum(c0 ** 2))
     c0 = c0 / sx
     p01 = np.dot(c0, c1)
     c1 = c1 - p01 * c0
-    sy = np.sqrt(np.dot(c1, c1))
+    sy = np.sqrt(np.sum(c1 ** 2))
     c1 = c1 / sy
     p02 = np.dot(c0, c2)
     p12 = np.dot(c1, c2)
-    c2 = c2 - p02 * c0 - p12 * c1
-    sz = np.sqrt(np.dot(c2, c2))
+    c2 = c2 - (p02 * c0 + p12 * c1)
+    sz = np.sqrt(np.sum(c2 ** 2))
     c2 = c2 / sz
     rot = np.column_stack([c0, c1, c2])
     if np.linalg.det(rot) < 0:

@@ -23,13 +23,14 @@ import numpy as np
 
 from .base import (SimulationSequenceElement, GeometryError,
                    SimulationSetupWarning)
-from .geometry import Geometry, FinitePlane, CircularHole, RectangleHole
+from .geometry import Geometry, FinitePlane, CircularHole, RectangleHole, Cylinder
 from .mathutils import decompose44
 
 __all__ = ['OpticalElement', 'FlatOpticalElement', 'OrderSelector',
            'EfficiencyFile', 'FlatGrating', 'CATGrating',
            'NonParallelCATGrating', 'FlatAperture', 'CircleAperture',
            'RectangleAperture', 'PerfectLens', 'ThinLens', 'FlatDetector', 'FlatStack',
+           'CircularDetector',
            'InterpolateEfficiencyTable', 'read_efficiency_table', 'tablerows_to_2d',
            'DataFileFormatException']
 
@@ -404,3 +405,19 @@ class FlatDetector(FlatOpticalElement):
                               'size in direction {0}. It will be rounded.'.format('xy'[i]),
                               SimulationSetupWarning)
             self.centerpix[i] = (self.npix[i] - 1) / 2
+
+
+class CircularDetector(OpticalElement):
+    """A detector shaped like a tube (``detector.py:78-118``), e.g. one that
+    follows the Rowland circle exactly (``Cylinder.from_rowland``).  The
+    radius is the x/y zoom, the half-length the z zoom.  Columns: local
+    ``det_phi`` (rad) and ``det_y`` (mm), ``detpix_x = det_phi R / pixsize``
+    and ``detpix_y = det_y / pixsize`` (centre pixel 0)."""
+    loc_coos_name = ['det_phi', 'det_y']
+    detpix_name = ['detpix_x', 'detpix_y']
+    centerpix = [0, 0]
+    default_geometry = Cylinder
+
+    def __init__(self, pixsize=1, **kwargs):
+        self.pixsize = pixsize
+        super().__init__(**kwargs)

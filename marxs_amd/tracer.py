@@ -23,7 +23,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .compiler import compile_tree, flatten, CompileError, ORDER_NONE, STAGE_STACK
+from .compiler import (compile_tree, flatten, CompileError, ORDER_NONE, STAGE_STACK,
+                       STAGE_CYLINDER)
 from .simulator import BaseContainer, Sequence
 from . import random as mxrandom
 
@@ -73,7 +74,7 @@ class DeviceTables:
                      'succ_items', 'lds_image32', 'lds_image64'):
             setattr(s, name, k[name].data_ptr())
         s.first_draw_slot = self.host.first_draw_slot
-        s.has_stack = int(any(st.kind == STAGE_STACK for st in self.host.stages))
+        s.ext_stages = int(any(st.kind in (STAGE_STACK, STAGE_CYLINDER) for st in self.host.stages))
         return s
 
     def struct_for(self, existing):

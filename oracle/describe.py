@@ -128,6 +128,12 @@ def describe(tree):
             if name == 'NonParallelCATGrating':
                 d['blaze_center'] = e.blaze_center
                 d['d_blaze_mm'] = e.d_blaze_mm
+        elif name == 'CircularDetector':
+            d['kind'] = 'cylinder'
+            d['pixsize'] = e.pixsize
+            d['centerpix'] = list(e.centerpix)
+            d['detpix'] = e.detpix_name
+            d['phi_lim'] = np.array(e.geometry.coos_limits[0], dtype=float)
         elif name == 'FlatDetector':
             d['kind'] = 'detector'
             d['pixsize'] = e.pixsize

@@ -334,6 +334,82 @@ def thin_lens(el, fr, dir, q, pol):
     return new_d, new_p
 
 
+def angle_between(angle, border1, border2):
+    """marxs/math/utils.py:180-215: angle in the cyclic interval
+    [border1, border2] (all taken modulo 2 pi with numpy's floor modulo)."""
+    twopi = 2 * np.pi
+    b1 = (twopi + (border1 % twopi)) % twopi
+    b2 = (twopi + (border2 % twopi)) % twopi
+    ang = (twopi + (angle % twopi)) % twopi
+    if b1 < b2:
+        return (b1 <= ang) & (ang <= b2)
+    return (b1 <= ang) | (ang <= b2)
+
+
+def zooms_gram_schmidt(pos4d):
+    """Zooms of transforms3d.affines.decompose44 (used by the reference's
+    Cylinder, geometry.py:428-432,556-562): Gram-Schmidt on the columns of
+    the linear part; x zoom negative for a reflection."""
+    m = np.array(pos4d, dtype=float)[:3, :3]
+    m0, m1, m2 = m[:, 0].copy(), m[:, 1].copy(), m[:, 2].copy()
+    sx = math.sqrt(np.sum(m0 ** 2))
+    m0 /= sx
+    m1 -= np.dot(m0, m1) * m0
+    sy = math.sqrt(np.sum(m1 ** 2))
+    m1 /= sy
+    m2 -= (np.dot(m0, m2) * m0 + np.dot(m1, m2) * m1)
+    sz = math.sqrt(np.sum(m2 ** 2))
+    m2 /= sz
+    if np.linalg.det(np.column_stack([m0, m1, m2])) < 0:
+        sx = -sx
+    return sx, sy, sz
+
+
+def _affine_apply(m, v, w):
+    """np.dot(m, e2h(v, w).T).T, summed in index order."""
+    return np.stack([((m[j, 0] * v[:, 0] + m[j, 1] * v[:, 1]) + m[j, 2] * v[:, 2]) + m[j, 3] * w
+                     for j in range(3)], axis=1)
+
+
+def cylinder_intersect(el, pos, dir):
+    """Cylinder.intersect, marxs/math/geometry.py:495-564: in the local
+    frame (inverse pose) solve a t^2 + b t + c = 0 for the unit circle; both
+    roots that are forward, within |z| <= 1 and within the azimuth limits
+    are hits, the nearer wins.  Returns hit, global point, phi, z (times
+    the z zoom)."""
+    P = np.array(el['pos4d'], dtype=float)
+    inv = np.linalg.inv(P)
+    xyz = _affine_apply(inv, pos, 1.)
+    d = _affine_apply(inv, dir, 0.)
+    x, y = xyz[:, 0], xyz[:, 1]
+    rx, ry = d[:, 0], d[:, 1]
+    with np.errstate(invalid='ignore', divide='ignore'):
+        c = (x * x + y * y) - 1.
+        b = 2 * (x * rx + y * ry)
+        a = rx * rx + ry * ry
+        under = b ** 2 - 4 * a * c
+        denom = 2 * a
+        root = np.sqrt(under)
+        a1 = (-b + root) / denom
+        a2 = (-b - root) / denom
+        phi1 = np.arctan2(y + a1 * ry, x + a1 * rx)
+        phi2 = np.arctan2(y + a2 * ry, x + a2 * rx)
+        z1 = xyz[:, 2] + a1 * d[:, 2]
+        z2 = xyz[:, 2] + a2 * d[:, 2]
+        lim = el['phi_lim']
+        h1 = (under >= 0) & (a1 >= 0) & (np.abs(z1) <= 1.) & angle_between(phi1, lim[0], lim[1])
+        h2 = (under >= 0) & (a2 >= 0) & (np.abs(z2) <= 1.) & angle_between(phi2, lim[0], lim[1])
+    h1 = h1 & ~(h2 & (a2 < a1))
+    h2 = h2 & ~(h1 & (a2 >= a1))
+    hit = h1 | h2
+    t = np.where(h1, a1, a2)
+    phi = np.where(h1, phi1, phi2)
+    z = np.where(h1, z1, z2) * zooms_gram_schmidt(P)[2]
+    ql = np.stack([x + t * rx, y + t * ry, xyz[:, 2] + t * d[:, 2]], axis=1)
+    q = _affine_apply(P, ql, 1.)
+    return hit, q, phi, z
+
+
 def detector_params(el):
     """FlatDetector.__init__, detector.py:59-70."""
     z = zooms(el['pos4d'])
@@ -390,6 +466,7 @@ def columns_of(el):
     default_loc = {'aperture_circle': ['y', 'z'], 'aperture_rect': ['y', 'z'],
                    'lens_perfect': ['mirror_x', 'mirror_y'], 'lens_thin': ['y', 'z'],
                    'grating': ['grat_y', 'grat_z'], 'detector': ['det_x', 'det_y'],
+                   'cylinder': ['det_phi', 'det_y'],
                    }[kind]
     loc = el.get('loc', default_loc)
     if loc is not None:
@@ -397,6 +474,8 @@ def columns_of(el):
     if kind == 'grating':
         cols['order'] = el.get('order_name', 'order')
         cols['blaze'] = el.get('blaze_name', 'blaze')
+    if kind == 'cylinder':
+        cols['detpix0'], cols['detpix1'] = el.get('detpix', ['detpix_x', 'detpix_y'])
     if kind == 'detector':
         cols['detpix0'], cols['detpix1'] = el.get('detpix', ['detpix_x', 'detpix_y'])
         pix = el.get('pix', ['pix_x', 'pix_y'])
@@ -631,6 +710,16 @@ def trace(elements, photons, uniforms, n_draws=2):
             f = dot(ph['dir'], -fr['e_x'])
             f = np.clip(f, 0, 1.)
             out = {'probability': f}
+        elif kind == 'cylinder':
+            # CircularDetector.specific_process_photons, detector.py:113-116
+            hit, q, ly, lz = cylinder_intersect(el, ph['pos'], ph['dir'])
+            if not hit.any():
+                continue
+            q, ly, lz = q[hit], ly[hit], lz[hit]
+            R = zooms_gram_schmidt(el['pos4d'])[0]
+            out = {'detpix0': ly * R / el['pixsize'] + el['centerpix'][0],
+                   'detpix1': lz / el['pixsize'] + el['centerpix'][1]}
+            count[hit] += 1
         else:
             hit, q, ly, lz = intersect(fr, ph['pos'], ph['dir'])
             if not hit.any():

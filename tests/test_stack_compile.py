@@ -125,3 +125,23 @@ def test_catsupportbars_stage():
     assert t.stage_i[1, 6] == t.icols.index('facet') and t.n_uniform_slots == 2
     with pytest.raises(C.CompileError, match='facet id column'):
         C.compile_tree([mx.FlatDetector(), CG.catsupportbars])
+
+
+def test_tube_stage_record():
+    from marxs_amd.geometry import Cylinder
+    rowland = mx.RowlandTorus(5900., 6000.)
+    det = mx.CircularDetector(geometry=Cylinder.from_rowland(
+        rowland, width=50., rotation=np.pi, kwargs={'phi_lim': [-np.pi / 2, np.pi / 2]}),
+        pixsize=0.024, id_col='tube', id_num=1)
+    t = C.compile_tree([det, mx.CircularDetector(zoom=[3, 3, 1], pixsize=0.024, id_col='tube',
+                                                  id_num=2)])
+    assert [s.kind for s in t.stages] == [C.STAGE_CYLINDER] and len(t.stages[0].elements) == 2
+    rec = t.sel_d[t.elem_i[0, 6]:t.elem_i[0, 6] + C.CR]
+    assert np.allclose(rec[0:16].reshape(4, 4) @ det.pos4d, np.eye(4), atol=1e-12)
+    assert np.array_equal(rec[16:32].reshape(4, 4), det.pos4d)
+    assert np.isclose(rec[32], 6000.) and np.isclose(rec[33], 50.)
+    assert np.isclose(rec[34], 1.5 * np.pi) and np.isclose(rec[35], 0.5 * np.pi)
+    assert rec[36] == 0.024
+    names = t.stages[0].columns
+    assert (names['loc0'], names['loc1'], names['detpix0']) == ('det_phi', 'det_y', 'detpix_x')
+    assert t.stage_i[0, 3] == 0 and t.stage_i[0, 20] == -1    # no grid, not in LDS
