@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 15
+#define MX_ABI_VERSION 16
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -246,6 +246,45 @@ typedef struct MxPoseArgs {
 } MxPoseArgs;
 
 int mx_facet_pose(const MxPoseArgs *args, void *stream);
+
+/* ------------------------------------------------------------ per-order statistics
+ * Figures of merit of a traced photon list, per grating order, without
+ * leaving HBM.  Replaces the host loops of effectivearea_from_photonlist
+ * (marxs/analysis/gratings.py:333-363: summed probability per order) and
+ * resolvingpower_from_photonlist (215-270: astropy sigma_clipped_stats of
+ * the dispersion coordinate per order: non-finite values dropped, then
+ * rounds of keeping values within median -/+ sigma * std (ddof 0,
+ * inclusive) until a round removes nothing or maxiters rounds ran; mean,
+ * median and std of the kept values), as CaptureResAeff calls them
+ * (389-503).  Orders must be distinct.  The caller passes a device
+ * workspace of at least mx_order_stats_workspace(n, n_orders, value !=
+ * NULL) bytes; the call allocates nothing. */
+#define MX_MAX_ORDERS 64
+#define MX_OS_FILTER 1   /* count only photons with finite value and probability > 0
+                            (CaptureResAeff.res_filter, gratings.py:446-451) */
+typedef struct MxOrderStats {
+    int64_t n;                 /* photons, < 2^31 */
+    const int32_t *order;      /* [n] order column */
+    const double *value;       /* [n] dispersion coordinate; NULL: sums and counts only */
+    const double *prob;        /* [n] probability */
+    const int32_t *orders;     /* [n_orders] distinct order numbers */
+    int32_t n_orders;          /* 1 .. MX_MAX_ORDERS */
+    int32_t flags;             /* MX_OS_FILTER */
+    double sigma;              /* clipping threshold in std (the reference: 3) */
+    int32_t maxiters;          /* clipping rounds at most (the reference: 5) */
+    int32_t reserved;
+    void *workspace;           /* device scratch */
+    int64_t workspace_bytes;
+    double *prob_sum;          /* out [n_orders]: summed probability, every photon of the order */
+    int64_t *count;            /* out [n_orders]: photons of the order (after MX_OS_FILTER) */
+    double *mean;              /* out [n_orders]: clipped statistics (value != NULL) */
+    double *median;
+    double *std;
+    int64_t *n_kept;           /* out [n_orders]: values kept by the clipping */
+} MxOrderStats;
+
+int64_t mx_order_stats_workspace(int64_t n, int32_t n_orders, int32_t with_values);
+int mx_order_stats(const MxOrderStats *args, void *stream);
 
 /* Fill n int32 values with v (a kernel, so that it can be captured in a
  * hipGraph without a memset node).  No reference counterpart: device

@@ -9,13 +9,14 @@ import os
 
 import torch  # noqa: F401  (loads the HIP runtime first)
 
-__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseArgs', 'MX_MAX_FCOLS', 'MX_MAX_ICOLS',
-           'MX_NSTATUS', 'LIB_PATH', 'check']
+__all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseArgs', 'MxOrderStats',
+           'MX_MAX_FCOLS', 'MX_MAX_ICOLS', 'MX_NSTATUS', 'MX_MAX_ORDERS', 'MX_OS_FILTER', 'LIB_PATH',
+           'check']
 
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 15
+MX_ABI_VERSION = 16
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -66,8 +67,18 @@ class MxPoseArgs(ctypes.Structure):
                 ('y_shift', ctypes.c_double)]
 
 
+class MxOrderStats(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int64), ('order', P), ('value', P), ('prob', P), ('orders', P),
+                ('n_orders', ctypes.c_int32), ('flags', ctypes.c_int32), ('sigma', ctypes.c_double),
+                ('maxiters', ctypes.c_int32), ('reserved', ctypes.c_int32), ('workspace', P),
+                ('workspace_bytes', ctypes.c_int64), ('prob_sum', P), ('count', P), ('mean', P),
+                ('median', P), ('std', P), ('n_kept', P)]
+
+
 MX_AJ_GEOM = 16
 MX_AJ_NGRAD = 13
+MX_MAX_ORDERS = 64
+MX_OS_FILTER = 1
 
 
 def _load():
@@ -89,6 +100,10 @@ def _load():
     lib.mx_adjoint.argtypes = [ctypes.POINTER(MxAdjointArgs), P]
     lib.mx_facet_pose.restype = ctypes.c_int
     lib.mx_facet_pose.argtypes = [ctypes.POINTER(MxPoseArgs), P]
+    lib.mx_order_stats_workspace.restype = ctypes.c_int64
+    lib.mx_order_stats_workspace.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]
+    lib.mx_order_stats.restype = ctypes.c_int
+    lib.mx_order_stats.argtypes = [ctypes.POINTER(MxOrderStats), P]
     v = lib.mx_abi_version()
     if v != MX_ABI_VERSION:
         raise ImportError(f'libmarxs_hip.so ABI {v}, expected {MX_ABI_VERSION}: rebuild it')

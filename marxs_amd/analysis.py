@@ -1,0 +1,245 @@
+"""Figures of merit of a traced photon list per grating order
+(``marxs/analysis/gratings.py``), computed on the GPU.
+
+``effectivearea_from_photonlist`` (``gratings.py:333-363``),
+``resolvingpower_from_photonlist`` (``215-270``), ``average_R_Aeff``
+(``187-212``), ``CaptureResAeff`` and ``CaptureResAeff_CCDgaps``
+(``389-553``) keep their names, arguments and results.  The photon list may
+be a :class:`~marxs_amd.tracer.DevicePhotons` (columns stay in HBM), a
+:class:`~marxs_amd.table.PhotonTable` or a dict of arrays (columns are copied
+to the GPU).  The per-order sums and the sigma clipping (astropy's
+``sigma_clipped_stats`` with the reference's defaults: 3 sigma, at most 5
+rounds, median centre) run in ``mx_order_stats``; only the few per-order
+numbers come back to the host.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .compiler import ORDER_NONE
+
+__all__ = ['AnalysisError', 'order_stats', 'sigma_clipped_stats',
+           'effectivearea_from_photonlist', 'resolvingpower_from_photonlist',
+           'average_R_Aeff', 'CaptureResAeff', 'CaptureResAeff_CCDgaps']
+
+RES_FWHM = 2.3548     # FWHM / sigma (gratings.py:268)
+
+
+class AnalysisError(Exception):
+    pass
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError('the analysis reductions run on the GPU; no GPU is visible')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def _column(photons, name, device):
+    """Column ``name`` of a photon list as a device tensor (1-d)."""
+    if hasattr(photons, 'fcols') and hasattr(photons, 'icols'):
+        if name in photons.fcols:
+            return photons.fcols[name]
+        if name in photons.icols:
+            return photons.icols[name]
+        if name in ('probability', 'prob'):
+            return photons.prob
+        if name == 'energy':
+            return photons.energy
+        raise KeyError(name)
+    v = photons[name]
+    if isinstance(v, torch.Tensor):
+        return v.to(device)
+    return torch.as_tensor(np.asarray(v), device=device)
+
+
+def _orders_i32(col):
+    """Order column as int32 with the no-order sentinel (NaN or non-integer
+    values never equal an order number)."""
+    if col.dtype == torch.int32:
+        return col.contiguous()
+    if col.dtype.is_floating_point:
+        ok = torch.isfinite(col) & (col == torch.round(col)) & (col.abs() < 2 ** 31 - 1)
+        return torch.where(ok, col, torch.full_like(col, float(ORDER_NONE))).to(torch.int32).contiguous()
+    return col.to(torch.int32).contiguous()
+
+
+def order_stats(order, prob, orders, value=None, res_filter=False, sigma=3., maxiters=5,
+                select=None, stream=None):
+    """Per-order reductions on the GPU (``mx_order_stats``).
+
+    order, prob, value: device tensors of one photon list; orders: distinct
+    order numbers (at most 64).  ``res_filter``: count only photons with a
+    finite value and probability > 0; ``select``: boolean tensor, photons
+    outside it are ignored.  Returns numpy arrays ``prob_sum``, ``count`` and,
+    with a value column, the sigma-clipped ``mean``, ``median``, ``std`` and
+    ``n_kept``."""
+    dev = order.device
+    orders = np.asarray(orders, dtype=np.int64).ravel()
+    if len(orders) == 0:
+        empty = np.zeros(0)
+        return {'prob_sum': empty, 'count': np.zeros(0, np.int64), 'mean': empty,
+                'median': empty, 'std': empty, 'n_kept': np.zeros(0, np.int64)}
+    if len(np.unique(orders)) != len(orders):
+        raise ValueError('order numbers must be distinct')
+    if len(orders) > _lib.MX_MAX_ORDERS:
+        raise ValueError(f'at most {_lib.MX_MAX_ORDERS} orders per call')
+    o = _orders_i32(order)
+    if select is not None:
+        o = torch.where(torch.as_tensor(select, device=dev), o,
+                        torch.full_like(o, ORDER_NONE)).contiguous()
+    p = prob.to(torch.float64).contiguous()
+    v = value.to(torch.float64).contiguous() if value is not None else None
+    n = o.numel()
+    if p.numel() != n or (v is not None and v.numel() != n):
+        raise ValueError('columns of different lengths')
+    no = len(orders)
+    ords = torch.as_tensor(orders.astype(np.int32), device=dev)
+    wsb = int(_lib.lib.mx_order_stats_workspace(n, no, int(v is not None)))
+    if wsb < 0:
+        raise ValueError('order statistics: invalid sizes')
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+    f64 = dict(dtype=torch.float64, device=dev)
+    out = {'prob_sum': torch.empty(no, **f64), 'count': torch.empty(no, dtype=torch.int64, device=dev),
+           'mean': torch.empty(no, **f64), 'median': torch.empty(no, **f64),
+           'std': torch.empty(no, **f64), 'n_kept': torch.empty(no, dtype=torch.int64, device=dev)}
+    a = _lib.MxOrderStats()
+    a.n = n
+    a.order = o.data_ptr()
+    a.value = v.data_ptr() if v is not None else None
+    a.prob = p.data_ptr()
+    a.orders = ords.data_ptr()
+    a.n_orders = no
+    a.flags = _lib.MX_OS_FILTER if res_filter else 0
+    a.sigma = float(sigma)
+    a.maxiters = int(maxiters)
+    a.workspace = ws.data_ptr()
+    a.workspace_bytes = wsb
+    for k in ('prob_sum', 'count', 'mean', 'median', 'std', 'n_kept'):
+        setattr(a, k, out[k].data_ptr())
+    if stream is None:
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(_lib.lib.mx_order_stats(a, stream))
+    res = {k: t.cpu().numpy() for k, t in out.items()}
+    if v is None:
+        for k in ('mean', 'median', 'std', 'n_kept'):
+            res.pop(k)
+    return res
+
+
+def sigma_clipped_stats(data, sigma=3., maxiters=5):
+    """(mean, median, std) of ``data`` after sigma clipping (astropy's
+    ``sigma_clipped_stats`` with median centre and std ddof 0), on the GPU."""
+    dev = data.device if isinstance(data, torch.Tensor) else _device()
+    x = torch.as_tensor(data, dtype=torch.float64, device=dev).reshape(-1)
+    zero = torch.zeros(x.numel(), dtype=torch.int32, device=dev)
+    r = order_stats(zero, torch.ones_like(x), [0], value=x, sigma=sigma, maxiters=maxiters)
+    return r['mean'][0], r['median'][0], r['std'][0]
+
+
+def effectivearea_from_photonlist(photons, orders, n_photons, A_geom=1., ordercol='order'):
+    """Summed probability per order / ``n_photons`` * ``A_geom``
+    (gratings.py:333-363)."""
+    dev = _device()
+    order = _column(photons, ordercol, dev)
+    prob = _column(photons, 'probability', dev)
+    r = order_stats(order, prob, orders)
+    return r['prob_sum'] / n_photons * A_geom
+
+
+def _resolving_power(order, value, orders, zeropos, res_filter, prob, select=None):
+    """gratings.py:255-270 on device columns; orders with 20 photons or
+    fewer give NaN; zeroth order found from >= 20 photons when not given."""
+    orders = np.asarray(orders)
+    want = list(dict.fromkeys([int(x) for x in orders] + ([0] if zeropos is None else [])))
+    r = order_stats(order, prob, want, value=value, res_filter=res_filter, select=select)
+    idx = {o: i for i, o in enumerate(want)}
+    if zeropos is None:
+        i0 = idx[0]
+        if r['count'][i0] < 20:
+            raise AnalysisError('Too few photons in list to determine position of order 0 '
+                                'automatically.')
+        zeropos = r['mean'][i0]
+    pos = np.array([r['mean'][idx[int(o)]] if r['count'][idx[int(o)]] > 20 else np.nan
+                    for o in orders], dtype=float)
+    std = np.array([r['std'][idx[int(o)]] if r['count'][idx[int(o)]] > 20 else np.nan
+                    for o in orders], dtype=float)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        res = np.abs(pos - zeropos) / (std * RES_FWHM)
+    return res, pos, std
+
+
+def resolvingpower_from_photonlist(photons, orders, col='proj_x', zeropos=None, ordercol='order'):
+    """Resolving power, mean position and width of ``col`` per order
+    (gratings.py:215-270): sigma-clipped statistics; NaN for orders with 20
+    photons or fewer; the zeroth order position from order 0 when
+    ``zeropos`` is None (AnalysisError below 20 photons)."""
+    dev = _device()
+    order = _column(photons, ordercol, dev)
+    value = _column(photons, col, dev)
+    prob = torch.ones(order.numel(), dtype=torch.float64, device=dev)
+    return _resolving_power(order, value, orders, zeropos, False, prob)
+
+
+def average_R_Aeff(r, aeff, axis=None):
+    """Summed effective area and area-weighted average of the finite
+    resolving powers (gratings.py:187-212)."""
+    aeff_avg = aeff.sum(axis=axis)
+    res_avg = np.ma.average(np.ma.masked_invalid(r), weights=aeff, axis=axis)
+    return res_avg, aeff_avg
+
+
+class CaptureResAeff:
+    """Resolving power and effective area per order of a simulation
+    (gratings.py:389-503).  ``aeff_filter``/``res_filter`` select photons;
+    the defaults use every photon for the area and photons with a finite
+    dispersion coordinate and probability > 0 for the resolving power.
+    Overrides return boolean arrays or tensors over the photon list, or
+    None for "all photons"."""
+
+    def __init__(self, A_geom=1, order_col='order', orders=np.arange(-10, 11),
+                 dispersion_coord='det_x', zeropos=None):
+        self.A_geom = A_geom
+        self.order_col = order_col
+        self.orders = np.asanyarray(orders)
+        self.dispersion_coord = dispersion_coord
+        self.zeropos = zeropos
+
+    def aeff_filter(self, photons):
+        return None
+
+    def res_filter(self, photons):
+        return None
+
+    def __call__(self, photons, n_photons=None):
+        dev = _device()
+        order = _column(photons, self.order_col, dev)
+        prob = _column(photons, 'probability', dev)
+        value = _column(photons, self.dispersion_coord, dev)
+        if n_photons is None:
+            n_photons = order.numel()
+        sel = self.aeff_filter(photons)
+        r = order_stats(order, prob, self.orders, select=sel)
+        aeff = r['prob_sum'] / n_photons * self.A_geom
+        rsel = self.res_filter(photons)
+        try:
+            res, pos, std = _resolving_power(order, value, self.orders, self.zeropos,
+                                             rsel is None, prob, select=rsel)
+        except AnalysisError:
+            res = np.nan * np.ones(len(self.orders))
+        disporders = self.orders != 0
+        avggratres, aeffgrat = average_R_Aeff(res[disporders], aeff[disporders])
+        aeff0 = np.sum(aeff[~disporders])
+        return {'Aeff0': aeff0, 'Aeffgrat': aeffgrat, 'Aeff': aeff, 'Rgrat': avggratres, 'R': res}
+
+
+class CaptureResAeff_CCDgaps(CaptureResAeff):
+    """As CaptureResAeff, with only photons where ``aeff_filter_col`` >= 0
+    counted for the effective area (gratings.py:505-553)."""
+
+    def __init__(self, aeff_filter_col='CCD_ID', **kwargs):
+        super().__init__(**kwargs)
+        self.aeff_filter_col = aeff_filter_col
+
+    def aeff_filter(self, photons):
+        return _column(photons, self.aeff_filter_col, _device()) >= 0

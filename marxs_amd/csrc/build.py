@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
-SOURCES = ['marxs_trace.hip', 'marxs_adjoint.hip']
+SOURCES = ['marxs_trace.hip', 'marxs_adjoint.hip', 'marxs_analysis.hip']
 TARGET = os.path.join(PKG, 'libmarxs_hip.so')
 
 FLAGS = [

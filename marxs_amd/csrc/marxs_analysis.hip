@@ -1,0 +1,355 @@
+// marxs_analysis.hip -- per-order figures of merit of a traced photon list
+// on the GPU (MI355X, gfx950).
+//
+// The reference analyses every tolerancing variant on the host:
+// effectivearea_from_photonlist (marxs/analysis/gratings.py:333-363) sums the
+// probability of each grating order, resolvingpower_from_photonlist (215-270)
+// takes astropy's sigma_clipped_stats of the dispersion coordinate of each
+// order (sigma 3, at most 5 rounds, median centre, std with ddof 0: drop
+// non-finite values, then repeat {keep values within median -/+ 3 std,
+// inclusive} until a round removes nothing).
+//
+// Here the photon list stays in HBM:
+//   1. count   one wave per chunk of photons: per order, the summed
+//              probability, the photons counted (flag MX_OS_FILTER: only
+//              finite value and probability > 0) and those with a finite
+//              value, each lane in its own LDS slot (no atomics: the sums
+//              have a fixed order for a given photon count);
+//   2. scan    per order, chunk offsets into one array of segments (one
+//              segment per order) and the totals;
+//   3. scatter the finite values of each order into its segment;
+//   4. sort    each segment (hipCUB segmented radix sort, caller workspace);
+//   5. clip    one workgroup per order: on a sorted segment the kept set of
+//              every round is a contiguous range, so a round is a median
+//              read, two block reductions (mean, then squared deviations)
+//              and two binary searches for the new ends.
+// The library allocates nothing: all scratch is the caller's workspace.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+#include <stdio.h>
+#include <math.h>
+
+#include "marxs_hip.h"
+
+__attribute__((visibility("hidden"))) void mx_set_error(const char *msg);
+
+#define MX_OS_WAVE 64          // work-items per counting / scatter workgroup
+#define MX_OS_CLIP 256         // work-items per clipping workgroup
+#define MX_OS_CHUNK 16384      // photons per counting workgroup (at least)
+#define MX_OS_MAX_BLOCKS 1024
+
+static int64_t os_blocks(int64_t n) {
+    int64_t nb = (n + MX_OS_CHUNK - 1) / MX_OS_CHUNK;
+    if (nb < 1) nb = 1;
+    if (nb > MX_OS_MAX_BLOCKS) nb = MX_OS_MAX_BLOCKS;
+    return nb;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workspace layout (bytes, each part 256-aligned).
+struct OsLayout {
+    size_t cnt_b, nval_b, psum_b, off_b, seg, vals, sorted, temp, temp_bytes, total;
+};
+
+static int os_sort_temp_bytes(int64_t n, int no, size_t &bytes) {
+    bytes = 0;
+    if (hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, (const double *)nullptr,
+                                                   (double *)nullptr, (int)n, no, (const int *)nullptr,
+                                                   (const int *)nullptr, 0, 64, (hipStream_t)0) != hipSuccess)
+        return 1;
+    return 0;
+}
+
+static int os_layout(int64_t n, int no, bool values, OsLayout &L) {
+    const int64_t nb = os_blocks(n);
+    size_t o = 0;
+    L.cnt_b = o;  o += align256(sizeof(int64_t) * nb * no);
+    L.nval_b = o; o += align256(sizeof(int64_t) * nb * no);
+    L.psum_b = o; o += align256(sizeof(double) * nb * no);
+    L.off_b = o;  o += align256(sizeof(int64_t) * nb * no);
+    L.seg = o;    o += align256(sizeof(int32_t) * 2 * no);
+    L.vals = o;   o += values ? align256(sizeof(double) * (n > 0 ? n : 1)) : 0;
+    L.sorted = o; o += values ? align256(sizeof(double) * (n > 0 ? n : 1)) : 0;
+    L.temp = o;
+    L.temp_bytes = 0;
+    if (values && os_sort_temp_bytes(n, no, L.temp_bytes)) return 1;
+    o += align256(L.temp_bytes > 0 ? L.temp_bytes : 1);
+    L.total = o;
+    return 0;
+}
+
+// index of order o in the list (linear: at most MX_MAX_ORDERS entries), -1
+__device__ __forceinline__ int order_index(const int *ords, int no, int o) {
+    int k = -1;
+    for (int j = 0; j < no; ++j)
+        if (ords[j] == o) { k = j; break; }
+    return k;
+}
+
+__device__ __forceinline__ bool counted(const MxOrderStats &a, int64_t i, double &v) {
+    v = a.value ? a.value[i] : 0.0;
+    if (!(a.flags & MX_OS_FILTER)) return true;
+    return isfinite(v) && a.prob[i] > 0.0;
+}
+
+// Pass 1: per chunk and order, probability sum, counted photons, finite
+// values.  One wave per workgroup; lane l accumulates into slot [k][l].
+__global__ __launch_bounds__(MX_OS_WAVE) void mx_os_count_kernel(MxOrderStats a, int64_t chunk,
+                                                                  int64_t *cnt_b, int64_t *nval_b,
+                                                                  double *psum_b) {
+    extern __shared__ __align__(16) unsigned char sm[];
+    const int no = a.n_orders, lane = threadIdx.x;
+    double *sp = (double *)sm;                          // [no][64]
+    int *sc = (int *)(sp + no * MX_OS_WAVE);            // [no][64]
+    int *sv = sc + no * MX_OS_WAVE;                     // [no][64]
+    int *ords = sv + no * MX_OS_WAVE;                   // [no]
+    for (int k = lane; k < no; k += MX_OS_WAVE) ords[k] = a.orders[k];
+    for (int k = 0; k < no; ++k) {
+        sp[k * MX_OS_WAVE + lane] = 0.0;
+        sc[k * MX_OS_WAVE + lane] = 0;
+        sv[k * MX_OS_WAVE + lane] = 0;
+    }
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    int64_t b1 = b0 + chunk;
+    if (b1 > a.n) b1 = a.n;
+    for (int64_t i = b0 + lane; i < b1; i += MX_OS_WAVE) {
+        const int k = order_index(ords, no, a.order[i]);
+        if (k < 0) continue;
+        sp[k * MX_OS_WAVE + lane] += a.prob[i];
+        double v;
+        if (counted(a, i, v)) {
+            sc[k * MX_OS_WAVE + lane] += 1;
+            if (a.value && isfinite(v)) sv[k * MX_OS_WAVE + lane] += 1;
+        }
+    }
+    __syncthreads();
+    for (int k = lane; k < no; k += MX_OS_WAVE) {
+        double s = 0.0;
+        int64_t c = 0, v = 0;
+        for (int l = 0; l < MX_OS_WAVE; ++l) {
+            s += sp[k * MX_OS_WAVE + l];
+            c += sc[k * MX_OS_WAVE + l];
+            v += sv[k * MX_OS_WAVE + l];
+        }
+        const size_t o = (size_t)blockIdx.x * no + k;
+        psum_b[o] = s;
+        cnt_b[o] = c;
+        nval_b[o] = v;
+    }
+}
+
+// Pass 2: one lane per order walks the chunks in order.
+__global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scan_kernel(MxOrderStats a, int64_t nb,
+                                                                 const int64_t *cnt_b, const int64_t *nval_b,
+                                                                 const double *psum_b, int64_t *off_b,
+                                                                 int32_t *seg) {
+    __shared__ int64_t tot[MX_MAX_ORDERS];
+    const int no = a.n_orders, k = threadIdx.x;
+    if (k < no) {
+        double s = 0.0;
+        int64_t c = 0, v = 0;
+        for (int64_t b = 0; b < nb; ++b) {
+            const size_t o = (size_t)b * no + k;
+            off_b[o] = v;
+            v += nval_b[o];
+            c += cnt_b[o];
+            s += psum_b[o];
+        }
+        a.prob_sum[k] = s;
+        a.count[k] = c;
+        tot[k] = v;
+    }
+    __syncthreads();
+    if (k < no) {
+        int64_t start = 0;
+        for (int j = 0; j < k; ++j) start += tot[j];
+        seg[k] = (int32_t)start;
+        seg[no + k] = (int32_t)(start + tot[k]);
+    }
+}
+
+// Pass 3: values of each order into its segment (lane-ordered within a chunk).
+__global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats a, int64_t chunk,
+                                                                    const int64_t *off_b, const int32_t *seg,
+                                                                    double *vals) {
+    extern __shared__ __align__(16) unsigned char sm[];
+    const int no = a.n_orders, lane = threadIdx.x;
+    int64_t *pos = (int64_t *)sm;                       // [no][64]
+    int *ords = (int *)(pos + no * MX_OS_WAVE);         // [no]
+    for (int k = lane; k < no; k += MX_OS_WAVE) ords[k] = a.orders[k];
+    for (int k = 0; k < no; ++k) pos[k * MX_OS_WAVE + lane] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    int64_t b1 = b0 + chunk;
+    if (b1 > a.n) b1 = a.n;
+    for (int64_t i = b0 + lane; i < b1; i += MX_OS_WAVE) {
+        const int k = order_index(ords, no, a.order[i]);
+        double v;
+        if (k >= 0 && counted(a, i, v) && isfinite(v)) pos[k * MX_OS_WAVE + lane] += 1;
+    }
+    __syncthreads();
+    // exclusive prefix over lanes, plus the chunk and segment offsets
+    for (int k = lane; k < no; k += MX_OS_WAVE) {
+        int64_t run = (int64_t)seg[k] + off_b[(size_t)blockIdx.x * no + k];
+        for (int l = 0; l < MX_OS_WAVE; ++l) {
+            const int64_t c = pos[k * MX_OS_WAVE + l];
+            pos[k * MX_OS_WAVE + l] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int64_t i = b0 + lane; i < b1; i += MX_OS_WAVE) {
+        const int k = order_index(ords, no, a.order[i]);
+        double v;
+        if (k >= 0 && counted(a, i, v) && isfinite(v)) {
+            const int64_t p = pos[k * MX_OS_WAVE + lane];
+            pos[k * MX_OS_WAVE + lane] = p + 1;
+            vals[p] = v;
+        }
+    }
+}
+
+// sum of f(x[j]) over [lo, hi) by the whole workgroup (fixed order: lane
+// strides, then a tree in LDS); every work-item gets the result.
+template <typename F>
+__device__ __forceinline__ double block_sum(const double *x, int64_t lo, int64_t hi, double *red, F f) {
+    double s = 0.0;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += MX_OS_CLIP) s += f(x[j]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = MX_OS_CLIP / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double median_sorted(const double *x, int64_t lo, int64_t hi) {
+    const int64_t m = hi - lo;
+    if (m & 1) return x[lo + m / 2];
+    return (x[lo + m / 2 - 1] + x[lo + m / 2]) / 2;
+}
+
+// Pass 5: sigma clipping of each sorted segment, one workgroup per order.
+__global__ __launch_bounds__(MX_OS_CLIP) void mx_os_clip_kernel(MxOrderStats a, const int32_t *seg,
+                                                                 const double *x) {
+    __shared__ double red[MX_OS_CLIP];
+    __shared__ int64_t s_lo, s_hi;
+    const int k = blockIdx.x, no = a.n_orders;
+    int64_t lo = seg[k], hi = seg[no + k];
+    const double nan = __builtin_nan("");
+    int64_t nchanged = 1;
+    for (int it = 0; nchanged != 0 && it < a.maxiters; ++it) {
+        const int64_t m = hi - lo;
+        if (m == 0) break;
+        const double cen = median_sorted(x, lo, hi);
+        const double mean = block_sum(x, lo, hi, red, [](double v) { return v; }) / (double)m;
+        const double var = block_sum(x, lo, hi, red, [mean](double v) { const double d = v - mean; return d * d; }) / (double)m;
+        const double sd = sqrt(var);
+        const double vmin = cen - sd * a.sigma;
+        const double vmax = cen + sd * a.sigma;
+        if (threadIdx.x == 0) {
+            int64_t l = lo, h = hi;            // first index with x >= vmin
+            while (l < h) { const int64_t mid = (l + h) >> 1; if (x[mid] >= vmin) h = mid; else l = mid + 1; }
+            const int64_t nlo = l;
+            h = hi;                            // first index with x > vmax
+            while (l < h) { const int64_t mid = (l + h) >> 1; if (x[mid] > vmax) h = mid; else l = mid + 1; }
+            s_lo = nlo;
+            s_hi = l;
+        }
+        __syncthreads();
+        nchanged = m - (s_hi - s_lo);
+        lo = s_lo;
+        hi = s_hi;
+        __syncthreads();
+    }
+    const int64_t m = hi - lo;
+    double mean = nan, med = nan, sd = nan;
+    if (m > 0) {
+        med = median_sorted(x, lo, hi);
+        mean = block_sum(x, lo, hi, red, [](double v) { return v; }) / (double)m;
+        sd = sqrt(block_sum(x, lo, hi, red, [mean](double v) { const double d = v - mean; return d * d; }) / (double)m);
+    }
+    if (threadIdx.x == 0) {
+        a.mean[k] = mean;
+        a.median[k] = med;
+        a.std[k] = sd;
+        a.n_kept[k] = m;
+    }
+}
+
+extern "C" int64_t mx_order_stats_workspace(int64_t n, int32_t n_orders, int32_t with_values) {
+    if (n < 0 || n_orders < 1 || n_orders > MX_MAX_ORDERS) return -1;
+    OsLayout L;
+    if (os_layout(n, n_orders, with_values != 0, L)) return -1;
+    return (int64_t)L.total;
+}
+
+extern "C" int mx_order_stats(const MxOrderStats *a, void *stream) {
+    if (!a) { mx_set_error("order stats: arguments NULL"); return 1; }
+    if (a->n < 0 || a->n > 0x7fffffffLL) { mx_set_error("order stats: photon count outside [0, 2^31)"); return 1; }
+    if (a->n_orders < 1 || a->n_orders > MX_MAX_ORDERS) { mx_set_error("order stats: n_orders outside [1, 64]"); return 1; }
+    if (!a->orders || !a->prob_sum || !a->count || (a->n > 0 && (!a->order || !a->prob))) {
+        mx_set_error("order stats: a required pointer is NULL");
+        return 1;
+    }
+    const bool values = a->value != nullptr;
+    if (values && (!a->mean || !a->median || !a->std || !a->n_kept || a->maxiters < 0)) {
+        mx_set_error("order stats: statistics outputs NULL or maxiters negative");
+        return 1;
+    }
+    if ((a->flags & MX_OS_FILTER) && !values) {
+        mx_set_error("order stats: the finite-value filter needs a value column");
+        return 1;
+    }
+    OsLayout L;
+    if (os_layout(a->n, a->n_orders, values, L)) { mx_set_error("order stats: sort size query failed"); return 2; }
+    if (!a->workspace || a->workspace_bytes < (int64_t)L.total) {
+        mx_set_error("order stats: workspace missing or smaller than mx_order_stats_workspace()");
+        return 1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char *ws = (unsigned char *)a->workspace;
+    int64_t *cnt_b = (int64_t *)(ws + L.cnt_b);
+    int64_t *nval_b = (int64_t *)(ws + L.nval_b);
+    double *psum_b = (double *)(ws + L.psum_b);
+    int64_t *off_b = (int64_t *)(ws + L.off_b);
+    int32_t *seg = (int32_t *)(ws + L.seg);
+    const int no = a->n_orders;
+    const int64_t nb = os_blocks(a->n);
+    const int64_t chunk = (a->n + nb - 1) / nb > 0 ? (a->n + nb - 1) / nb : 1;
+    const size_t lds1 = (size_t)no * MX_OS_WAVE * (8 + 4 + 4) + (size_t)no * 4;
+    hipLaunchKernelGGL(mx_os_count_kernel, dim3((unsigned)nb), dim3(MX_OS_WAVE), lds1, st, *a, chunk,
+                       cnt_b, nval_b, psum_b);
+    hipLaunchKernelGGL(mx_os_scan_kernel, dim3(1), dim3(MX_OS_WAVE), 0, st, *a, nb, cnt_b, nval_b, psum_b,
+                       off_b, seg);
+    if (values) {
+        double *vals = (double *)(ws + L.vals), *sorted = (double *)(ws + L.sorted);
+        const size_t lds3 = (size_t)no * MX_OS_WAVE * 8 + (size_t)no * 4;
+        hipLaunchKernelGGL(mx_os_scatter_kernel, dim3((unsigned)nb), dim3(MX_OS_WAVE), lds3, st, *a, chunk,
+                           off_b, seg, vals);
+        size_t tb = L.temp_bytes;
+        if (hipcub::DeviceSegmentedRadixSort::SortKeys(ws + L.temp, tb, (const double *)vals, sorted,
+                                                       (int)a->n, no, (const int *)seg,
+                                                       (const int *)(seg + no), 0, 64, st) != hipSuccess) {
+            mx_set_error("order stats: segmented sort failed");
+            return 2;
+        }
+        hipLaunchKernelGGL(mx_os_clip_kernel, dim3((unsigned)no), dim3(MX_OS_CLIP), 0, st, *a, seg,
+                           (const double *)sorted);
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "order stats: launch failed: %s", hipGetErrorString(err));
+        mx_set_error(buf);
+        return 2;
+    }
+    return 0;
+}

@@ -44,7 +44,8 @@ def test_struct_layout_matches_header(lib, tmp_path):
               'MxPhotons': [f[0] for f in lib.MxPhotons._fields_],
               'MxAdjointArgs': [f[0] for f in lib.MxAdjointArgs._fields_],
               'MxBatch': [f[0] for f in lib.MxBatch._fields_],
-              'MxPoseArgs': [f[0] for f in lib.MxPoseArgs._fields_]}
+              'MxPoseArgs': [f[0] for f in lib.MxPoseArgs._fields_],
+              'MxOrderStats': [f[0] for f in lib.MxOrderStats._fields_]}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "marxs_hip.h"',
            'int main(void) {']
     for st, fs in fields.items():
@@ -93,3 +94,18 @@ def test_batched_argument_errors_need_no_gpu(lib):
     b.n_variants = 2
     assert lib.lib.mx_trace_batched(ctypes.byref(b), 32, 0, None) != 0
     assert b'NULL' in lib.lib.mx_last_error()
+
+
+def test_order_stats_argument_errors_need_no_gpu(lib):
+    a = lib.MxOrderStats()
+    assert lib.lib.mx_order_stats(None, None) != 0
+    a.n, a.n_orders = 10, 0
+    assert lib.lib.mx_order_stats(ctypes.byref(a), None) != 0
+    assert b'n_orders' in lib.lib.mx_last_error()
+    a.n_orders = 3
+    assert lib.lib.mx_order_stats(ctypes.byref(a), None) != 0
+    assert b'NULL' in lib.lib.mx_last_error()
+    a.n = -1
+    assert lib.lib.mx_order_stats(ctypes.byref(a), None) != 0
+    assert lib.lib.mx_order_stats_workspace(-1, 3, 0) == -1
+    assert lib.lib.mx_order_stats_workspace(10, 65, 0) == -1

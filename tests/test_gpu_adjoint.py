@@ -167,7 +167,9 @@ def test_fused_graph_replays_follow_the_parameters(problem):
         got = (buf['loss'][0].item(), buf['grad7'].clone())
         # eager evaluation in between replays
         L, gt, gp = prob.loss_and_grad(th, prob.period0, precision=64)
-        assert got[0] == L.item()
+        # the loss comes from moments summed with atomic adds: equal up to
+        # the summation order
+        assert abs(got[0] - L.item()) <= 1e-13 * abs(L.item())
         # gradients are sums of atomic adds: equal up to summation order
         ref = torch.cat([gt, gp[:, None]], 1)
         scale = torch.abs(ref).max(0).values

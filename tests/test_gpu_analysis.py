@@ -1,0 +1,128 @@
+"""Per-order figures of merit on the GPU (mx_order_stats) against the
+oracle's restatement (needs an MI355X).  Sums of probability within 1e-12
+relative (summation order differs), counts and kept-value counts exact,
+clipped mean / std within 1e-9 relative, medians exact (they are elements
+or the mean of two elements of the same sorted values)."""
+import numpy as np
+import pytest
+
+from oracle import analysis as OA
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, gpu, dtype=None):
+    import torch
+    return torch.as_tensor(np.asarray(x), device=gpu, dtype=dtype)
+
+
+def _oracle_stats(order, value, prob, orders, res_filter=False):
+    out = {k: [] for k in ('prob_sum', 'count', 'mean', 'median', 'std', 'n_kept')}
+    for o in orders:
+        m = order == o
+        out['prob_sum'].append(prob[m].sum())
+        c = m & (np.isfinite(value) & (prob > 0) if res_filter else True)
+        out['count'].append(int(c.sum()))
+        mean, med, std = OA.sigma_clipped_stats(value[c])
+        out['mean'].append(mean)
+        out['median'].append(med)
+        out['std'].append(std)
+    return {k: np.array(v) for k, v in out.items()}
+
+
+@pytest.mark.parametrize('n, res_filter', [(200_000, False), (200_000, True), (1000, False),
+                                           (3_000_000, True)])
+def test_order_stats_against_oracle(gpu, n, res_filter):
+    import torch
+    from marxs_amd.analysis import order_stats
+    rng = np.random.default_rng(n + res_filter)
+    orders = np.arange(-12, 3)
+    order = rng.integers(-14, 4, n).astype(np.int32)
+    order[rng.random(n) < 0.05] = -2 ** 31                      # no order
+    value = 300. * order + rng.normal(0, 1, n)
+    out_idx = rng.random(n) < 0.003
+    value[out_idx] += rng.choice([-1, 1], out_idx.sum()) * rng.uniform(10, 100, out_idx.sum())
+    value[rng.random(n) < 0.01] = np.nan
+    value[rng.random(n) < 0.001] = np.inf
+    prob = rng.uniform(0, 1, n)
+    prob[rng.random(n) < 0.05] = 0.
+    got = order_stats(_t(order, gpu), _t(prob, gpu), orders, value=_t(value, gpu),
+                      res_filter=res_filter)
+    ref = _oracle_stats(order, value, prob, orders, res_filter)
+    assert np.allclose(got['prob_sum'], ref['prob_sum'], rtol=1e-12, atol=0)
+    assert np.array_equal(got['count'], ref['count'])
+    for k in ('mean', 'std'):
+        assert np.allclose(got[k], ref[k], rtol=1e-9, atol=1e-12, equal_nan=True), k
+    assert np.array_equal(got['median'], ref['median'], equal_nan=True)
+
+
+def test_clipping_closed_form_on_gpu(gpu):
+    from marxs_amd.analysis import sigma_clipped_stats
+    rng = np.random.default_rng(1)
+    u = rng.uniform(-1, 1, 100_000)
+    data = np.concatenate([u, [100., -150.], [np.nan, np.inf]])
+    rng.shuffle(data)
+    m, med, s = sigma_clipped_stats(_t(data, gpu))
+    assert np.isclose(m, np.mean(u), rtol=1e-12) and med == np.median(u)
+    assert np.isclose(s, np.std(u), rtol=1e-12)
+    # empty and all-NaN input
+    assert np.isnan(sigma_clipped_stats(_t(np.full(10, np.nan), gpu))[0])
+
+
+def test_reference_known_answers_on_gpu(gpu):
+    """marxs/analysis/tests/test_grating.py:86-96, 150-238"""
+    from marxs_amd import analysis as A
+    from marxs_amd.table import PhotonTable
+    p = PhotonTable({'a': np.array([0, 1, 2, 2, 1]), 'probability': [.1, 1., 1., .3, .5]})
+    aeff = A.effectivearea_from_photonlist(p, [2, 1, 0, -1], 5, A_geom=2., ordercol='a')
+    assert aeff[3] == 0 and np.isclose(aeff[2], .04) and np.isclose(aeff[1], .6)
+    rng = np.random.default_rng(0)
+    order = np.zeros(200, dtype=int)
+    order[50:] = 5
+    x = np.full(200, -100.)
+    x[50:] = rng.normal(scale=1, size=150)
+    t = PhotonTable({'order': order, 'xpos': x, 'probability': np.ones(200),
+                     'CCD': np.where(np.arange(200) < 100, 0, -1)})
+    r = A.CaptureResAeff(A_geom=10., order_col='order', orders=[0, 2, 5], dispersion_coord='xpos')
+    out = r(t)
+    ref = OA.capture_res_aeff(order, x, np.ones(200), [0, 2, 5], A_geom=10.)
+    assert np.allclose(out['Aeff'], [2.5, 0., 7.5]) and np.isclose(out['Aeffgrat'], 7.5)
+    assert np.isnan(out['R'][1]) and np.isclose(out['R'][2], ref['R'][2], rtol=1e-9)
+    t['probability'] = np.zeros(200)
+    out = r(t)
+    assert np.all(out['Aeff'] == 0) and np.isnan(out['R'][2])
+    t['probability'] = np.ones(200)
+    t['xpos'] = np.full(200, np.nan)
+    out = r(t)
+    assert np.allclose(out['Aeff'], [2.5, 0., 7.5]) and np.isnan(out['R'][2])
+    t['xpos'] = x
+    gaps = A.CaptureResAeff_CCDgaps(A_geom=10., order_col='order', orders=[0, 2, 5],
+                                    dispersion_coord='xpos', aeff_filter_col='CCD')
+    out = gaps(t)
+    assert np.allclose(out['Aeff'], [2.5, 0., 2.5]) and np.isnan(out['R'][1])
+    with pytest.raises(A.AnalysisError):
+        A.resolvingpower_from_photonlist(t[:10], [5], col='xpos')
+
+
+def test_trace_then_capture_on_device(gpu):
+    """Trace the 200-facet Rowland instrument on the GPU, then CaptureResAeff
+    on the device columns; the oracle's figures of merit on the same
+    (downloaded) columns agree."""
+    from marxs_amd import configs, analysis as A
+    from tests.helpers import make_inputs, gpu_trace
+    from marxs_amd.tracer import CompiledInstrument
+    inst, parts = configs.cat_rowland_instrument(200)
+    ci = CompiledInstrument(inst)
+    ph, u = make_inputs(400_000, 7, ci.n_uniform_slots, 'lines')
+    got, st = gpu_trace(ci, ph, u, precision=64)
+    orders = np.arange(-12, 3)
+    cap = A.CaptureResAeff(A_geom=np.pi * (600 ** 2 - 300 ** 2), orders=orders,
+                           dispersion_coord='det_x', zeropos=0.)
+    t = {'order': got['order'].astype(np.int32), 'det_x': got['det_x'],
+         'probability': got['prob']}
+    out = cap(t)
+    ref = OA.capture_res_aeff(got['order'], got['det_x'], got['prob'], orders,
+                              A_geom=np.pi * (600 ** 2 - 300 ** 2), zeropos=0.)
+    assert np.allclose(out['Aeff'], ref['Aeff'], rtol=1e-12)
+    assert np.allclose(out['R'], ref['R'], rtol=1e-9, equal_nan=True)
+    assert np.isfinite(out['R']).sum() >= 5
