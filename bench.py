@@ -28,6 +28,14 @@ Two more sections run after it (``--sections``):
              L2 absorption and L2 diffraction), 1e7 photons: FP32 and FP64
              rates, FP64 indices against the oracle on a sample, the
              oracle's rate on that sample.
+``pipeline`` a full analysis run: aperture, lens, RadialMirrorScatter at the
+             lens (2e-5 / 4e-6 rad), the 600 CAT facets, catsupportbars and a
+             CircularDetector on the Rowland circle, 1e7 photons at 0.65 keV
+             (resolving power is a per-energy figure); FP32 / FP64
+             trace rates, then CaptureResAeff (effective area and
+             sigma-clipped resolving power of orders -12..2) on the device
+             columns, against the numpy oracle of the figures of merit on the
+             same columns.
 ``batched``  64 design variants (Rowland R ~ U(5000, 7000) mm, blaze ~
              U(1, 2) deg, 50-800 facets, own CCD rows), 2e5 photons each at
              0.3-2.0 keV, traced in one autotuned, hipGraph-replayed launch,
@@ -35,7 +43,7 @@ Two more sections run after it (``--sections``):
              batched results are compared with the sequential FP64 traces.
 
     python bench.py [--steps K] [--warmup W] [--photons N] [--cpu-sample S]
-                    [--sections trace,adjoint,stack,batched]
+                    [--sections trace,adjoint,stack,pipeline,batched]
 """
 import argparse
 import json
@@ -74,7 +82,7 @@ def parse():
                     help='profiling: time one precision only, skip comparisons')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='profiling: run one device copy of this size first (counter calibration)')
-    ap.add_argument('--sections', default='trace,adjoint,stack,batched')
+    ap.add_argument('--sections', default='trace,adjoint,stack,pipeline,batched')
     ap.add_argument('--stack-cpu-sample', type=int, default=100_000)
     ap.add_argument('--cpu-procs', type=int, default=0,
                     help='processes for the multi-process CPU baseline (0 = min(16, cpu count))')
@@ -336,6 +344,8 @@ def main():
         res['adjoint'] = bench_adjoint(args, dev)
     if 'stack' in sections:
         res['stack'] = bench_stack(args, dev)
+    if 'pipeline' in sections:
+        res['pipeline'] = bench_pipeline(args, dev)
     if 'batched' in sections:
         res['batched'] = bench_batched(args, dev)
     print(json.dumps(res), flush=True)
@@ -452,10 +462,10 @@ def bench_adjoint(args, dev):
     return res
 
 
-def gpu_inputs(ci, n, seed, dev):
+def gpu_inputs(ci, n, seed, dev, mono=None):
     """The trace workload's synthetic photons, generated on the GPU: line
-    spectrum, on-axis directions, polarization angle 2 pi u; returns
-    (DevicePhotons, uniforms)."""
+    spectrum (or all at energy ``mono`` keV), on-axis directions,
+    polarization angle 2 pi u; returns (DevicePhotons, uniforms)."""
     import numpy as np
     import torch
     from marxs_amd import configs
@@ -467,6 +477,8 @@ def gpu_inputs(ci, n, seed, dev):
     cont = u[0] < 0.2
     idx = torch.clamp(((u[0] - 0.2) / 0.2).long(), 0, 3)
     energy = torch.where(cont, 0.3 + 1.7 * u[1], lines[idx])
+    if mono is not None:
+        energy = torch.full_like(energy, float(mono))
     phi = 2 * np.pi * u[2]
     dirs = torch.zeros((3, n), dtype=torch.float64, device=dev)
     dirs[0] = -1.
@@ -537,6 +549,81 @@ def bench_stack(args, dev):
         res['cpu_baseline'] = {'value': s / tc, 'unit': 'photons/s', 'cores': 1, 'kind': 'port',
                                'sample': f'first {s} photons, numpy oracle, one thread, {tc:.1f} s'}
     log('stack', res)
+    return res
+
+
+def bench_pipeline(args, dev):
+    """Mirror scatter, 600 CAT facets, facet frame, Rowland-circle tube;
+    then the per-order figures of merit on the device columns."""
+    import numpy as np
+    import torch
+    import marxs_amd as mx
+    from marxs_amd import configs, catgrating, analysis
+    from marxs_amd.scatter import RadialMirrorScatter
+    from marxs_amd.geometry import Cylinder
+    from marxs_amd.tracer import CompiledInstrument, raise_for_status
+    from marxs_amd.batch import outputs_like
+    inst0, parts = configs.cat_rowland_instrument(args.facets)
+    rms = RadialMirrorScatter(inplanescatter=2e-5, perpplanescatter=4e-6,
+                              position=[12500., 0, 0], zoom=[1, 700, 700])
+    # the quarter of the Rowland circle on the dispersed side (negative
+    # orders), from just past the focus
+    tube = mx.CircularDetector(geometry=Cylinder.from_rowland(
+        parts['rowland'], width=50., rotation=np.pi, kwargs={'phi_lim': [-np.pi / 2, 0.02]}),
+        pixsize=0.024)
+    inst = mx.Sequence(elements=[parts['aperture'], parts['lens'], rms, parts['gratings'],
+                                 catgrating.catsupportbars, tube])
+    ci = CompiledInstrument(inst, grid=bool(args.grid), device=dev)
+    n = args.photons
+    src, uni = gpu_inputs(ci, n, args.seed + 7, dev, mono=0.65)
+    status = torch.zeros(8, dtype=torch.int64, device=dev)
+    outs = {p: outputs_like(src) for p in (32, 64)}
+    ms = {}
+    for prec in (32, 64):
+        for _ in range(args.warmup):
+            ci.launch(outs[prec], uni, precision=prec, src=src, status=status)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            ci.launch(outs[prec], uni, precision=prec, src=src, status=status)
+        b.record()
+        torch.cuda.synchronize()
+        ms[prec] = a.elapsed_time(b) / args.steps
+    raise_for_status(status.cpu().numpy())
+    o64 = outs[64]
+    orders = np.arange(-12, 3)
+    area = np.pi * (600. ** 2 - 300. ** 2)
+    cap = analysis.CaptureResAeff(A_geom=area, order_col='order', orders=orders,
+                                  dispersion_coord='det_phi')
+    out = cap(o64)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        out = cap(o64)
+    t_an = (time.perf_counter() - t) / args.steps
+    host = {'order': o64.icols['order'].cpu().numpy(), 'det_phi': o64.fcols['det_phi'].cpu().numpy(),
+            'prob': o64.prob.cpu().numpy()}
+    from oracle import analysis as OA
+    t = time.perf_counter()
+    ref = OA.capture_res_aeff(host['order'], host['det_phi'], host['prob'], orders, A_geom=area)
+    t_cpu = time.perf_counter() - t
+    fin = np.isfinite(ref['R'])
+    res = {'workload': f'aperture, lens, RadialMirrorScatter, cat_rowland_{args.facets} facets, '
+                       f'catsupportbars, Rowland-circle CircularDetector; {n} photons at 0.65 keV',
+           'photons_per_s_fp32': n / ms[32] * 1e3, 'photons_per_s_fp64': n / ms[64] * 1e3,
+           'kernel_ms_fp32': ms[32], 'kernel_ms_fp64': ms[64],
+           'photons_on_tube': int(torch.isfinite(o64.fcols['det_phi']).sum()),
+           'capture_res_aeff_ms_gpu': t_an * 1e3,
+           'capture_res_aeff_ms_cpu_oracle': t_cpu * 1e3,
+           'capture_photons_per_s_gpu': n / t_an,
+           'R_orders_finite': int(fin.sum()),
+           'R_max_rel_diff_vs_oracle': float(np.max(np.abs(out['R'][fin] - ref['R'][fin])
+                                                    / np.abs(ref['R'][fin]))) if fin.any() else None,
+           'Aeff_max_rel_diff_vs_oracle': float(np.max(np.abs(out['Aeff'] - ref['Aeff'])
+                                                       / np.maximum(ref['Aeff'], 1e-300))),
+           'Rgrat': float(out['Rgrat']), 'Aeffgrat_mm2': float(out['Aeffgrat'])}
+    log('pipeline', res)
     return res
 
 
