@@ -608,7 +608,7 @@ def bench_pipeline(args, dev):
     t = time.perf_counter()
     ref = OA.capture_res_aeff(host['order'], host['det_phi'], host['prob'], orders, A_geom=area)
     t_cpu = time.perf_counter() - t
-    fin = np.isfinite(ref['R'])
+    fin = np.isfinite(ref['R']) & (ref['R'] > 0)
     res = {'workload': f'aperture, lens, RadialMirrorScatter, cat_rowland_{args.facets} facets, '
                        f'catsupportbars, Rowland-circle CircularDetector; {n} photons at 0.65 keV',
            'photons_per_s_fp32': n / ms[32] * 1e3, 'photons_per_s_fp64': n / ms[64] * 1e3,

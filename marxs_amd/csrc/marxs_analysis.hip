@@ -15,10 +15,13 @@
 //              finite value and probability > 0) and those with a finite
 //              value, each lane in its own LDS slot (no atomics: the sums
 //              have a fixed order for a given photon count);
-//   2. scan    per order, chunk offsets into one array of segments (one
-//              segment per order) and the totals;
+//   2. scan    per order (one workgroup each), chunk offsets into one array
+//              of segments (one segment per order) and the totals;
 //   3. scatter the finite values of each order into its segment;
-//   4. sort    each segment (hipCUB segmented radix sort, caller workspace);
+//   4. sort    all values by value, then (stably) by order: two device-wide
+//              hipCUB radix sorts leave every segment sorted (a segmented
+//              sort would give each of the few, large segments one
+//              workgroup);
 //   5. clip    one workgroup per order: on a sorted segment the kept set of
 //              every round is a contiguous range, so a round is a median
 //              read, two block reductions (mean, then squared deviations)
@@ -36,9 +39,11 @@
 __attribute__((visibility("hidden"))) void mx_set_error(const char *msg);
 
 #define MX_OS_WAVE 64          // work-items per counting / scatter workgroup
-#define MX_OS_CLIP 256         // work-items per clipping workgroup
-#define MX_OS_CHUNK 16384      // photons per counting workgroup (at least)
-#define MX_OS_MAX_BLOCKS 1024
+#define MX_OS_CLIP 1024        // work-items per clipping workgroup
+#define MX_OS_UNROLL 8         // independent loads in flight per work-item
+#define MX_OS_CHUNK 4096       // photons per counting workgroup (at least)
+#define MX_OS_MAX_BLOCKS 16384
+#define MX_OS_SCAN 256         // work-items per scan workgroup (one per order)
 
 static int64_t os_blocks(int64_t n) {
     int64_t nb = (n + MX_OS_CHUNK - 1) / MX_OS_CHUNK;
@@ -51,15 +56,22 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace layout (bytes, each part 256-aligned).
 struct OsLayout {
-    size_t cnt_b, nval_b, psum_b, off_b, seg, vals, sorted, temp, temp_bytes, total;
+    size_t cnt_b, nval_b, psum_b, off_b, tot, seg, vals, sorted, ord, ord2, temp, temp_bytes, total;
 };
 
-static int os_sort_temp_bytes(int64_t n, int no, size_t &bytes) {
-    bytes = 0;
-    if (hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, (const double *)nullptr,
-                                                   (double *)nullptr, (int)n, no, (const int *)nullptr,
-                                                   (const int *)nullptr, 0, 64, (hipStream_t)0) != hipSuccess)
+// temporary storage of the two radix sorts (by value with the order index
+// as payload, then by order index with the value as payload)
+static int os_sort_temp_bytes(int64_t n, size_t &bytes) {
+    size_t b1 = 0, b2 = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, b1, (const double *)nullptr, (double *)nullptr,
+                                           (const uint8_t *)nullptr, (uint8_t *)nullptr, (int)n, 0, 64,
+                                           (hipStream_t)0) != hipSuccess)
         return 1;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, b2, (const uint8_t *)nullptr, (uint8_t *)nullptr,
+                                           (const double *)nullptr, (double *)nullptr, (int)n, 0, 8,
+                                           (hipStream_t)0) != hipSuccess)
+        return 1;
+    bytes = b1 > b2 ? b1 : b2;
     return 0;
 }
 
@@ -70,12 +82,15 @@ static int os_layout(int64_t n, int no, bool values, OsLayout &L) {
     L.nval_b = o; o += align256(sizeof(int64_t) * nb * no);
     L.psum_b = o; o += align256(sizeof(double) * nb * no);
     L.off_b = o;  o += align256(sizeof(int64_t) * nb * no);
+    L.tot = o;    o += align256(sizeof(int64_t) * no);
     L.seg = o;    o += align256(sizeof(int32_t) * 2 * no);
     L.vals = o;   o += values ? align256(sizeof(double) * (n > 0 ? n : 1)) : 0;
     L.sorted = o; o += values ? align256(sizeof(double) * (n > 0 ? n : 1)) : 0;
+    L.ord = o;    o += values ? align256(n > 0 ? n : 1) : 0;
+    L.ord2 = o;   o += values ? align256(n > 0 ? n : 1) : 0;
     L.temp = o;
     L.temp_bytes = 0;
-    if (values && os_sort_temp_bytes(n, no, L.temp_bytes)) return 1;
+    if (values && os_sort_temp_bytes(n, L.temp_bytes)) return 1;
     o += align256(L.temp_bytes > 0 ? L.temp_bytes : 1);
     L.total = o;
     return 0;
@@ -142,28 +157,57 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_count_kernel(MxOrderStats a,
     }
 }
 
-// Pass 2: one lane per order walks the chunks in order.
-__global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scan_kernel(MxOrderStats a, int64_t nb,
+// Pass 2: one workgroup per order.  Work-item t takes a contiguous run of
+// chunks; the run totals are scanned in order by work-item 0, then every
+// work-item writes its chunks' offsets.  Sums run in a fixed order.
+__global__ __launch_bounds__(MX_OS_SCAN) void mx_os_scan_kernel(MxOrderStats a, int64_t nb,
                                                                  const int64_t *cnt_b, const int64_t *nval_b,
                                                                  const double *psum_b, int64_t *off_b,
-                                                                 int32_t *seg) {
-    __shared__ int64_t tot[MX_MAX_ORDERS];
-    const int no = a.n_orders, k = threadIdx.x;
-    if (k < no) {
-        double s = 0.0;
-        int64_t c = 0, v = 0;
-        for (int64_t b = 0; b < nb; ++b) {
-            const size_t o = (size_t)b * no + k;
-            off_b[o] = v;
-            v += nval_b[o];
-            c += cnt_b[o];
-            s += psum_b[o];
+                                                                 int64_t *tot) {
+    __shared__ int64_t sv[MX_OS_SCAN], sc[MX_OS_SCAN];
+    __shared__ double ss[MX_OS_SCAN];
+    const int no = a.n_orders, k = blockIdx.x, t = threadIdx.x;
+    const int64_t per = (nb + MX_OS_SCAN - 1) / MX_OS_SCAN;
+    const int64_t b0 = (int64_t)t * per;
+    const int64_t b1 = b0 + per < nb ? b0 + per : nb;
+    int64_t v = 0, c = 0;
+    double s = 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+        const size_t o = (size_t)b * no + k;
+        v += nval_b[o];
+        c += cnt_b[o];
+        s += psum_b[o];
+    }
+    sv[t] = v;
+    sc[t] = c;
+    ss[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0, cc = 0;
+        double ps = 0.0;
+        for (int j = 0; j < MX_OS_SCAN; ++j) {
+            const int64_t x = sv[j];
+            sv[j] = run;
+            run += x;
+            cc += sc[j];
+            ps += ss[j];
         }
-        a.prob_sum[k] = s;
-        a.count[k] = c;
-        tot[k] = v;
+        a.prob_sum[k] = ps;
+        a.count[k] = cc;
+        tot[k] = run;
     }
     __syncthreads();
+    int64_t run = sv[t];
+    for (int64_t b = b0; b < b1; ++b) {
+        const size_t o = (size_t)b * no + k;
+        off_b[o] = run;
+        run += nval_b[o];
+    }
+}
+
+// Segment bounds: order k's values start after those of orders < k.
+__global__ void mx_os_seg_kernel(int no, const int64_t *tot, int32_t *seg) {
+    const int k = threadIdx.x;
     if (k < no) {
         int64_t start = 0;
         for (int j = 0; j < k; ++j) start += tot[j];
@@ -175,7 +219,7 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scan_kernel(MxOrderStats a, 
 // Pass 3: values of each order into its segment (lane-ordered within a chunk).
 __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats a, int64_t chunk,
                                                                     const int64_t *off_b, const int32_t *seg,
-                                                                    double *vals) {
+                                                                    double *vals, uint8_t *ordk) {
     extern __shared__ __align__(16) unsigned char sm[];
     const int no = a.n_orders, lane = threadIdx.x;
     int64_t *pos = (int64_t *)sm;                       // [no][64]
@@ -209,7 +253,18 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
             const int64_t p = pos[k * MX_OS_WAVE + lane];
             pos[k * MX_OS_WAVE + lane] = p + 1;
             vals[p] = v;
+            ordk[p] = (uint8_t)k;
         }
+    }
+}
+
+// Items [end of the last segment, n) of the sort input: value +inf, order
+// index 255 (past any order), so that both sorts leave them at the end.
+__global__ void mx_os_tail_kernel(const int32_t *last_end, int64_t n, double *vals, uint8_t *ordk) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && i >= (int64_t)last_end[0]) {
+        vals[i] = __builtin_inf();
+        ordk[i] = 255;
     }
 }
 
@@ -217,8 +272,24 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
 // strides, then a tree in LDS); every work-item gets the result.
 template <typename F>
 __device__ __forceinline__ double block_sum(const double *x, int64_t lo, int64_t hi, double *red, F f) {
+    // MX_OS_UNROLL independent accumulators per work-item, each over its own
+    // stride of the range, added in a fixed order
+    double acc[MX_OS_UNROLL];
+#pragma unroll
+    for (int u = 0; u < MX_OS_UNROLL; ++u) acc[u] = 0.0;
+    const int64_t step = (int64_t)MX_OS_CLIP * MX_OS_UNROLL;
+    int64_t j = lo + threadIdx.x;
+    for (; j + (MX_OS_UNROLL - 1) * MX_OS_CLIP < hi; j += step) {
+        double v[MX_OS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MX_OS_UNROLL; ++u) v[u] = x[j + (int64_t)u * MX_OS_CLIP];
+#pragma unroll
+        for (int u = 0; u < MX_OS_UNROLL; ++u) acc[u] += f(v[u]);
+    }
+    for (; j < hi; j += MX_OS_CLIP) acc[0] += f(x[j]);
     double s = 0.0;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += MX_OS_CLIP) s += f(x[j]);
+#pragma unroll
+    for (int u = 0; u < MX_OS_UNROLL; ++u) s += acc[u];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = MX_OS_CLIP / 2; w > 0; w >>= 1) {
@@ -320,6 +391,7 @@ extern "C" int mx_order_stats(const MxOrderStats *a, void *stream) {
     int64_t *nval_b = (int64_t *)(ws + L.nval_b);
     double *psum_b = (double *)(ws + L.psum_b);
     int64_t *off_b = (int64_t *)(ws + L.off_b);
+    int64_t *tot = (int64_t *)(ws + L.tot);
     int32_t *seg = (int32_t *)(ws + L.seg);
     const int no = a->n_orders;
     const int64_t nb = os_blocks(a->n);
@@ -327,20 +399,33 @@ extern "C" int mx_order_stats(const MxOrderStats *a, void *stream) {
     const size_t lds1 = (size_t)no * MX_OS_WAVE * (8 + 4 + 4) + (size_t)no * 4;
     hipLaunchKernelGGL(mx_os_count_kernel, dim3((unsigned)nb), dim3(MX_OS_WAVE), lds1, st, *a, chunk,
                        cnt_b, nval_b, psum_b);
-    hipLaunchKernelGGL(mx_os_scan_kernel, dim3(1), dim3(MX_OS_WAVE), 0, st, *a, nb, cnt_b, nval_b, psum_b,
-                       off_b, seg);
+    hipLaunchKernelGGL(mx_os_scan_kernel, dim3((unsigned)no), dim3(MX_OS_SCAN), 0, st, *a, nb, cnt_b, nval_b,
+                       psum_b, off_b, tot);
+    hipLaunchKernelGGL(mx_os_seg_kernel, dim3(1), dim3(MX_OS_WAVE), 0, st, no, (const int64_t *)tot, seg);
     if (values) {
         double *vals = (double *)(ws + L.vals), *sorted = (double *)(ws + L.sorted);
         const size_t lds3 = (size_t)no * MX_OS_WAVE * 8 + (size_t)no * 4;
+        uint8_t *ordk = (uint8_t *)(ws + L.ord), *ordk2 = (uint8_t *)(ws + L.ord2);
         hipLaunchKernelGGL(mx_os_scatter_kernel, dim3((unsigned)nb), dim3(MX_OS_WAVE), lds3, st, *a, chunk,
-                           off_b, seg, vals);
+                           off_b, seg, vals, ordk);
+        // the segments fill [0, total); total <= n is known on the device
+        // only, so the sorts run over n items with the tail set to +inf /
+        // an order index past the last, which sorts behind every segment
+        hipLaunchKernelGGL(mx_os_tail_kernel, dim3((unsigned)((a->n + 255) / 256 > 0 ? (a->n + 255) / 256 : 1)),
+                           dim3(256), 0, st, seg + no + no - 1, a->n, vals, ordk);
         size_t tb = L.temp_bytes;
-        if (hipcub::DeviceSegmentedRadixSort::SortKeys(ws + L.temp, tb, (const double *)vals, sorted,
-                                                       (int)a->n, no, (const int *)seg,
-                                                       (const int *)(seg + no), 0, 64, st) != hipSuccess) {
-            mx_set_error("order stats: segmented sort failed");
+        if (hipcub::DeviceRadixSort::SortPairs(ws + L.temp, tb, (const double *)vals, sorted,
+                                               (const uint8_t *)ordk, ordk2, (int)a->n, 0, 64, st) != hipSuccess) {
+            mx_set_error("order stats: sort by value failed");
             return 2;
         }
+        tb = L.temp_bytes;
+        if (hipcub::DeviceRadixSort::SortPairs(ws + L.temp, tb, (const uint8_t *)ordk2, ordk,
+                                               (const double *)sorted, vals, (int)a->n, 0, 8, st) != hipSuccess) {
+            mx_set_error("order stats: sort by order failed");
+            return 2;
+        }
+        sorted = vals;
         hipLaunchKernelGGL(mx_os_clip_kernel, dim3((unsigned)no), dim3(MX_OS_CLIP), 0, st, *a, seg,
                            (const double *)sorted);
     }
