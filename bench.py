@@ -80,8 +80,6 @@ def parse():
     ap.add_argument('--grid', type=int, default=1)
     ap.add_argument('--only', choices=['fp32', 'fp64'], default=None,
                     help='profiling: time one precision only, skip comparisons')
-    ap.add_argument('--copy-calibration-mib', type=int, default=0,
-                    help='profiling: run one device copy of this size first (counter calibration)')
     ap.add_argument('--sections', default='trace,adjoint,stack,pipeline,batched')
     ap.add_argument('--stack-cpu-sample', type=int, default=100_000)
     ap.add_argument('--cpu-procs', type=int, default=0,
@@ -182,13 +180,6 @@ def main():
         kern = [a.elapsed_time(b) * 1e-3 for a, b in ev]
         return wall, float(np.mean(kern))
 
-    if args.copy_calibration_mib:
-        a = torch.empty(args.copy_calibration_mib * 2 ** 20 // 8, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log(f'copy calibration: {a.numel() * 8} bytes read, {a.numel() * 8} bytes written')
-        del a, b
     sections = set(args.sections.split(','))
     if args.only:
         prec = 32 if args.only == 'fp32' else 64
