@@ -88,6 +88,15 @@ class AdjointProblem:
         self.device = torch.device(device) if device is not None else pos.device
         self.gratings = gratings
         self.facets = list(gratings.elements)
+        from .optics import FlatGrating, FlatDetector
+        from .compiler import CompileError
+        if not all(isinstance(e, FlatGrating) and not callable(e._d) for e in self.facets):
+            raise CompileError('the adjoint replays flat grating facets with a fixed period '
+                               '(FlatGrating, CATGrating, NonParallelCATGrating); stacks and '
+                               'callable periods are not differentiated')
+        dets = list(detectors.elements) if hasattr(detectors, 'elements') else list(detectors)
+        if not all(isinstance(d, FlatDetector) for d in dets):
+            raise CompileError('the adjoint replays flat detectors (FlatDetector) only')
         self.coord = coord
         f64 = dict(dtype=torch.float64, device=self.device)
         self.pos = pos.to(**f64).contiguous()

@@ -145,3 +145,16 @@ def test_tube_stage_record():
     names = t.stages[0].columns
     assert (names['loc0'], names['loc1'], names['detpix0']) == ('det_phi', 'det_y', 'detpix_x')
     assert t.stage_i[0, 3] == 0 and t.stage_i[0, 20] == -1    # no grid, not in LDS
+
+
+def test_adjoint_rejects_stacks_and_tubes():
+    """The adjoint replays flat gratings and flat detectors only."""
+    import torch
+    from marxs_amd.adjoint import AdjointProblem
+    inst, parts = configs.cat_rowland_instrument(
+        20, facet_class=CG.CATL1L2Stack, facet_kwargs={'l1_transmission': SI})
+    z = torch.zeros((3, 4), dtype=torch.float64)
+    zi = torch.zeros(4, dtype=torch.int32)
+    with pytest.raises(C.CompileError, match='flat grating'):
+        AdjointProblem(parts['gratings'], parts['ccds'], z, z, z[0], z[0], zi, zi, zi,
+                       device='cpu')
