@@ -28,6 +28,10 @@ Two more sections run after it (``--sections``):
              L2 absorption and L2 diffraction), 1e7 photons: FP32 and FP64
              rates, FP64 indices against the oracle on a sample, the
              oracle's rate on that sample.
+``interp``   the trace workload with a blaze-dependent
+             InterpolateEfficiencyTable (wavelength x blaze angle, orders
+             -12..2) as every facet's order selector: FP32 and FP64 rates,
+             FP64 indices against the oracle on a sample.
 ``pipeline`` a full analysis run: aperture, lens, RadialMirrorScatter at the
              lens (2e-5 / 4e-6 rad), the 600 CAT facets, catsupportbars and a
              CircularDetector on the Rowland circle, 1e7 photons at 0.65 keV
@@ -43,7 +47,7 @@ Two more sections run after it (``--sections``):
              batched results are compared with the sequential FP64 traces.
 
     python bench.py [--steps K] [--warmup W] [--photons N] [--cpu-sample S]
-                    [--sections trace,adjoint,stack,pipeline,batched]
+                    [--sections trace,adjoint,stack,interp,pipeline,batched]
 """
 import argparse
 import json
@@ -80,7 +84,7 @@ def parse():
     ap.add_argument('--grid', type=int, default=1)
     ap.add_argument('--only', choices=['fp32', 'fp64'], default=None,
                     help='profiling: time one precision only, skip comparisons')
-    ap.add_argument('--sections', default='trace,adjoint,stack,pipeline,batched')
+    ap.add_argument('--sections', default='trace,adjoint,stack,interp,pipeline,batched')
     ap.add_argument('--stack-cpu-sample', type=int, default=100_000)
     ap.add_argument('--cpu-procs', type=int, default=0,
                     help='processes for the multi-process CPU baseline (0 = min(16, cpu count))')
@@ -335,6 +339,8 @@ def main():
         res['adjoint'] = bench_adjoint(args, dev)
     if 'stack' in sections:
         res['stack'] = bench_stack(args, dev)
+    if 'interp' in sections:
+        res['interp'] = bench_interp(args, dev)
     if 'pipeline' in sections:
         res['pipeline'] = bench_pipeline(args, dev)
     if 'batched' in sections:
@@ -540,6 +546,65 @@ def bench_stack(args, dev):
         res['cpu_baseline'] = {'value': s / tc, 'unit': 'photons/s', 'cores': 1, 'kind': 'port',
                                'sample': f'first {s} photons, numpy oracle, one thread, {tc:.1f} s'}
     log('stack', res)
+    return res
+
+
+def bench_interp(args, dev):
+    """600 CAT facets with a blaze-dependent interpolated efficiency table."""
+    import numpy as np
+    import torch
+    from marxs_amd import configs
+    from marxs_amd.tracer import CompiledInstrument, raise_for_status
+    from marxs_amd.batch import outputs_like
+    inst, parts = configs.cat_rowland_instrument(args.facets,
+                                                 efficiency=configs.blaze_interp_efficiency())
+    ci = CompiledInstrument(inst, grid=bool(args.grid), device=dev)
+    n = args.photons
+    src, uni = gpu_inputs(ci, n, args.seed + 9, dev)
+    status = torch.zeros(8, dtype=torch.int64, device=dev)
+    outs = {p: outputs_like(src) for p in (32, 64)}
+    ms = {}
+    for prec in (32, 64):
+        for _ in range(args.warmup):
+            ci.launch(outs[prec], uni, precision=prec, src=src, status=status)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            ci.launch(outs[prec], uni, precision=prec, src=src, status=status)
+        b.record()
+        torch.cuda.synchronize()
+        ms[prec] = a.elapsed_time(b) / args.steps
+    raise_for_status(status.cpu().numpy())
+    o32, o64 = outs[32], outs[64]
+    same = ((o32.icols['CCD'] == o64.icols['CCD']) & (o64.icols['CCD'] >= 0)
+            & (o32.icols['facet'] == o64.icols['facet']))
+    dx = (o32.fcols['det_x'] - o64.fcols['det_x'])[same]
+    dy = (o32.fcols['det_y'] - o64.fcols['det_y'])[same]
+    res = {'workload': f'cat_rowland_{args.facets} with a blaze-dependent InterpolateEfficiencyTable '
+                       f'(19 wavelengths x 9 blaze angles x 15 orders), {n} photons, line spectrum',
+           'photons_per_s_fp32': n / ms[32] * 1e3, 'photons_per_s_fp64': n / ms[64] * 1e3,
+           'kernel_ms_fp32': ms[32], 'kernel_ms_fp64': ms[64],
+           'fp32_rms_um_vs_fp64': float(torch.sqrt(torch.mean(dx * dx + dy * dy)) * 1e3),
+           'order_fp32_equals_fp64_same_facet': float(
+               (o32.icols['order'] == o64.icols['order'])[o32.icols['facet'] == o64.icols['facet']]
+               .double().mean())}
+    if args.stack_cpu_sample > 0:
+        os.environ.setdefault('OMP_NUM_THREADS', '1')
+        from oracle import trace as OT
+        from oracle.describe import describe
+        s = min(args.stack_cpu_sample, n)
+        ph = {'dir': src.dir[:, :s].cpu().numpy().T.copy(), 'pol': src.pol[:, :s].cpu().numpy().T.copy(),
+              'energy': src.energy[:s].cpu().numpy(), 'prob': src.prob[:s].cpu().numpy()}
+        t = time.perf_counter()
+        ref = OT.trace(describe(inst), ph, uni[:, :s].cpu().numpy())
+        tc = time.perf_counter() - t
+        res['fp64_sample_indices_equal_oracle'] = bool(all(
+            np.array_equal(ref[c], o64.icols[c][:s].cpu().numpy().astype(np.int64))
+            for c in ('facet', 'order', 'CCD')))
+        res['cpu_baseline'] = {'value': s / tc, 'unit': 'photons/s', 'cores': 1, 'kind': 'port',
+                               'sample': f'first {s} photons, numpy oracle, one thread, {tc:.1f} s'}
+    log('interp', res)
     return res
 
 

@@ -66,6 +66,22 @@ def blaze_efficiency(energy_grid=None, orders=CAT_ORDERS, center=-6., sigma=3.):
     return EfficiencyFile.from_arrays(energy_grid, prob, orders)
 
 
+def blaze_interp_efficiency(orders=CAT_ORDERS, d_nm=200., sigma=2.5):
+    """Synthetic blaze-dependent order efficiency as an
+    ``InterpolateEfficiencyTable`` on wavelength 0.6-4.2 nm x blaze angle
+    0.5-2.5 deg: Gaussian in order number around the blaze condition
+    m = -2 d sin(blaze) / lambda (sigma 2.5 orders), total 0.55 + 0.1 blaze/deg."""
+    from .optics import InterpolateEfficiencyTable
+    wave = np.round(np.arange(0.6, 4.2001, 0.2), 10)
+    theta = np.deg2rad(np.arange(0.5, 2.5001, 0.25))
+    m = np.asarray(orders, dtype=float)[:, None, None]
+    center = -2 * d_nm * np.sin(theta)[None, None, :] / wave[None, :, None]
+    w = np.exp(-0.5 * ((m - center) / sigma) ** 2)
+    w = w / w.sum(axis=0, keepdims=True)
+    total = 0.55 + 0.1 * np.rad2deg(theta)[None, None, :]
+    return InterpolateEfficiencyTable.from_arrays(wave, theta, orders, w * total)
+
+
 def _gas_count(radius, d_element, phi):
     """Number of facets GratingArrayStructure places for these arguments
     (its yz layout only, without solving for positions on the torus)."""

@@ -304,3 +304,12 @@ def test_interpolated_table_in_use(gpu):
     cat = mx.CATGrating(order_selector=_interp_table(), d=0.001)
     p = cat(mx.generate_test_photons(5000))
     assert np.isclose((np.asarray(p['order']) == 0).sum(), 2500, rtol=.05)
+
+
+def test_cat_rowland_blaze_dependent_table_parity(gpu):
+    """200 CAT facets whose order selector is a blaze-dependent
+    InterpolateEfficiencyTable (the bench's interp workload at C2 size)."""
+    inst, parts = configs.cat_rowland_instrument(200, efficiency=configs.blaze_interp_efficiency())
+    ref, got, st, rep = _parity(inst, 40000, 21, energies='lines')
+    assert rep['order'][1] == 0 and rep['facet'][1] == 0
+    assert len(set(got['order'][got['facet'] >= 0].tolist())) >= 12
