@@ -32,6 +32,9 @@ Two more sections run after it (``--sections``):
              InterpolateEfficiencyTable (wavelength x blaze angle, orders
              -12..2) as every facet's order selector: FP32 and FP64 rates,
              FP64 indices against the oracle on a sample.
+``stages``   (not in the default list) the trace workload cut after each
+             stage (aperture; + lens; + facets; + CCDs without facets; all),
+             FP32 and FP64 kernel times: where the trace time goes.
 ``pipeline`` a full analysis run: aperture, lens, RadialMirrorScatter at the
              lens (2e-5 / 4e-6 rad), the 600 CAT facets, catsupportbars and a
              CircularDetector on the Rowland circle, 1e7 photons at 0.65 keV
@@ -341,6 +344,8 @@ def main():
         res['stack'] = bench_stack(args, dev)
     if 'interp' in sections:
         res['interp'] = bench_interp(args, dev)
+    if 'stages' in sections:
+        res['stages'] = bench_stages(args, dev)
     if 'pipeline' in sections:
         res['pipeline'] = bench_pipeline(args, dev)
     if 'batched' in sections:
@@ -547,6 +552,40 @@ def bench_stack(args, dev):
                                'sample': f'first {s} photons, numpy oracle, one thread, {tc:.1f} s'}
     log('stack', res)
     return res
+
+
+def bench_stages(args, dev):
+    """The 600-facet workload cut after each stage: kernel time per launch."""
+    import torch
+    import marxs_amd as mx
+    from marxs_amd import configs
+    from marxs_amd.tracer import CompiledInstrument
+    from marxs_amd.batch import outputs_like
+    inst, parts = configs.cat_rowland_instrument(args.facets)
+    cuts = {'aperture': ['aperture'], 'aperture+lens': ['aperture', 'lens'],
+            'aperture+lens+facets': ['aperture', 'lens', 'gratings'],
+            'aperture+lens+ccds': ['aperture', 'lens', 'ccds'],
+            'all': ['aperture', 'lens', 'gratings', 'ccds']}
+    out = {}
+    for name, keys in cuts.items():
+        ci = CompiledInstrument(mx.Sequence(elements=[parts[k] for k in keys]), grid=bool(args.grid),
+                                device=dev)
+        src, uni = gpu_inputs(ci, args.photons, args.seed, dev)
+        for prec in (32, 64):
+            o = outputs_like(src)
+            for _ in range(args.warmup):
+                ci.launch(o, uni, precision=prec, src=src)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(args.steps):
+                ci.launch(o, uni, precision=prec, src=src)
+            b.record()
+            torch.cuda.synchronize()
+            out[f'{name} fp{prec} ms'] = a.elapsed_time(b) / args.steps
+            log(f'stages {name} FP{prec}: {out[f"{name} fp{prec} ms"]:.3f} ms')
+        del src, uni
+    return out
 
 
 def bench_interp(args, dev):
