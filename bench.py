@@ -324,7 +324,9 @@ def main():
         'kernel_ms_fp32': k32 * 1e3,
         'kernel_ms_fp64': k64 * 1e3,
         'bytes_per_photon': BYTES_PER_PHOTON,
-        'status': {'grid_fallback': int(st[3]), 'multi_hit': int(st[4])},
+        # counters accumulate over the warm-up and timed launches of both precisions
+        'status_per_launch': {'grid_fallback': int(st[3]) / (2 * (args.warmup + args.steps)),
+                              'multi_hit': int(st[4]) / (2 * (args.warmup + args.steps))},
     }
     # brute-force equivalent VALU fractions: 33 flops per photon-element test
     res['valu_fraction_brute_force_equiv'] = {
