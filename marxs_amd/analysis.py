@@ -94,6 +94,12 @@ def order_stats(order, prob, orders, value=None, res_filter=False, sigma=3., max
     if p.numel() != n or (v is not None and v.numel() != n):
         raise ValueError('columns of different lengths')
     no = len(orders)
+    if n == 0:
+        res = {'prob_sum': np.zeros(no), 'count': np.zeros(no, np.int64)}
+        if v is not None:
+            res.update({'mean': np.full(no, np.nan), 'median': np.full(no, np.nan),
+                        'std': np.full(no, np.nan), 'n_kept': np.zeros(no, np.int64)})
+        return res
     ords = torch.as_tensor(orders.astype(np.int32), device=dev)
     wsb = int(_lib.lib.mx_order_stats_workspace(n, no, int(v is not None)))
     if wsb < 0:

@@ -126,3 +126,17 @@ def test_trace_then_capture_on_device(gpu):
     assert np.allclose(out['Aeff'], ref['Aeff'], rtol=1e-12)
     assert np.allclose(out['R'], ref['R'], rtol=1e-9, equal_nan=True)
     assert np.isfinite(out['R']).sum() >= 5
+
+
+def test_order_stats_edge_cases(gpu):
+    """No photons, orders absent from the list, one photon, duplicate orders."""
+    from marxs_amd.analysis import order_stats
+    e = np.zeros(0)
+    r = order_stats(_t(e.astype(np.int32), gpu), _t(e, gpu), [0, 1], value=_t(e, gpu))
+    assert np.array_equal(r['count'], [0, 0]) and np.all(r['prob_sum'] == 0)
+    assert np.all(np.isnan(r['mean'])) and np.array_equal(r['n_kept'], [0, 0])
+    r = order_stats(_t(np.array([5], np.int32), gpu), _t([0.5], gpu), [0, 5], value=_t([2.5], gpu))
+    assert r['count'].tolist() == [0, 1] and r['prob_sum'].tolist() == [0., 0.5]
+    assert r['mean'][1] == 2.5 and r['median'][1] == 2.5 and r['std'][1] == 0.
+    with pytest.raises(ValueError, match='distinct'):
+        order_stats(_t(np.array([5], np.int32), gpu), _t([0.5], gpu), [5, 5])
