@@ -69,9 +69,10 @@ def order_stats(order, prob, orders, value=None, res_filter=False, sigma=3., max
     """Per-order reductions on the GPU (``mx_order_stats``).
 
     order, prob, value: device tensors of one photon list; orders: distinct
-    order numbers (at most 64).  ``res_filter``: count only photons with a
+    order numbers.  ``res_filter``: count only photons with a
     finite value and probability > 0; ``select``: boolean tensor, photons
-    outside it are ignored.  Returns numpy arrays ``prob_sum``, ``count`` and,
+    outside it are ignored.  More than 64 orders take several launches.
+    Returns numpy arrays ``prob_sum``, ``count`` and,
     with a value column, the sigma-clipped ``mean``, ``median``, ``std`` and
     ``n_kept``."""
     dev = order.device
@@ -83,7 +84,11 @@ def order_stats(order, prob, orders, value=None, res_filter=False, sigma=3., max
     if len(np.unique(orders)) != len(orders):
         raise ValueError('order numbers must be distinct')
     if len(orders) > _lib.MX_MAX_ORDERS:
-        raise ValueError(f'at most {_lib.MX_MAX_ORDERS} orders per call')
+        # the kernels take at most MX_MAX_ORDERS orders per call
+        parts = [order_stats(order, prob, orders[k:k + _lib.MX_MAX_ORDERS], value, res_filter,
+                             sigma, maxiters, select, stream)
+                 for k in range(0, len(orders), _lib.MX_MAX_ORDERS)]
+        return {key: np.concatenate([r[key] for r in parts]) for key in parts[0]}
     o = _orders_i32(order)
     if select is not None:
         o = torch.where(torch.as_tensor(select, device=dev), o,

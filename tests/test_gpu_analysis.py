@@ -140,3 +140,16 @@ def test_order_stats_edge_cases(gpu):
     assert r['mean'][1] == 2.5 and r['median'][1] == 2.5 and r['std'][1] == 0.
     with pytest.raises(ValueError, match='distinct'):
         order_stats(_t(np.array([5], np.int32), gpu), _t([0.5], gpu), [5, 5])
+
+
+def test_order_stats_more_than_64_orders(gpu):
+    from marxs_amd.analysis import order_stats
+    rng = np.random.default_rng(4)
+    order = rng.integers(-50, 50, 100_000).astype(np.int32)
+    value = order * 10. + rng.normal(0, 1, order.size)
+    prob = rng.random(order.size)
+    orders = np.arange(-50, 50)
+    got = order_stats(_t(order, gpu), _t(prob, gpu), orders, value=_t(value, gpu))
+    ref = _oracle_stats(order, value, prob, orders)
+    assert np.array_equal(got['count'], ref['count'])
+    assert np.allclose(got['mean'], ref['mean'], rtol=1e-9)
