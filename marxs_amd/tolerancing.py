@@ -157,7 +157,8 @@ def run_tolerances(photons_in, instrum, wigglefunc, wiggleparts, parameters, ana
 
 
 def run_tolerances_batched(photons_in, instrum, wigglefunc, wiggleparts, parameters, analyzefunc,
-                           precision=64, uniforms=None, generator=None, tune=False):
+                           precision=64, uniforms=None, generator=None, tune=False,
+                           analyze_on_device=False):
     """``run_tolerances`` with every variant traced in one batched launch.
 
     Each parameter set modifies the instrument as in the sequential loop and
@@ -166,7 +167,10 @@ def run_tolerances_batched(photons_in, instrum, wigglefunc, wiggleparts, paramet
     and each variant's photon table is analysed in turn.  ``uniforms`` (slots,
     N) fixes the random draws (the same for every variant: common random
     numbers); otherwise every variant draws its own.  ``tune`` runs the tile
-    autotuner first.  Photons must not hold output columns of the
+    autotuner first.  ``analyze_on_device``: ``analyzefunc`` receives each
+    variant's ``DevicePhotons`` instead of a host table (e.g.
+    ``marxs_amd.analysis.CaptureResAeff``, whose reductions then run on the
+    columns in HBM).  Photons must not hold output columns of the
     instrument yet.
     """
     import torch
@@ -196,9 +200,12 @@ def run_tolerances_batched(photons_in, instrum, wigglefunc, wiggleparts, paramet
     bt.run()
     out = []
     for pars, ci, dp in zip(parameters, cis, bt.outputs):
-        photons = photons_in.copy()
-        write_back(dp, photons, ci.tables)
         cpars = copy(pars)
-        cpars.update(analyzefunc(photons))
+        if analyze_on_device:
+            cpars.update(analyzefunc(dp))
+        else:
+            photons = photons_in.copy()
+            write_back(dp, photons, ci.tables)
+            cpars.update(analyzefunc(photons))
         out.append(cpars)
     return out
