@@ -204,7 +204,9 @@ def test_run_tolerances_batched_with_device_figures_of_merit(gpu):
                             'polarization': np.c_[ph['pol'], np.zeros(n)],
                             'energy': ph['energy'], 'probability': ph['prob']})
     u = np.random.default_rng(10).random((CompiledInstrument(inst).n_uniform_slots, n))
-    cap = CaptureResAeff(A_geom=1., orders=np.arange(-12, 3), dispersion_coord='det_x')
+    # zeroth order misses the CCD row: its position is given
+    cap = CaptureResAeff(A_geom=1., orders=np.arange(-12, 3), dispersion_coord='det_x',
+                         zeropos=0.)
     params = [{'dx': s, 'seed': k} for k, s in enumerate([0., 0.5])]
 
     def wig(e, dx, seed):
