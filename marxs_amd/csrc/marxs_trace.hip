@@ -167,6 +167,13 @@ __device__ __forceinline__ f3 unitf(f3 v) {
     return mkf(v.x * r, v.y * r, v.z * r);
 }
 __device__ __forceinline__ f3 tof(d3 v) { return mkf((float)v.x, (float)v.y, (float)v.z); }
+// FP32 path: x / y as x times the hardware reciprocal (1 ulp), not the
+// correctly rounded division sequence (ten instructions); the lengths it
+// scales are offsets within one stage, so the error stays far below the
+// path's error budget.
+__device__ __forceinline__ float fdiv_fast(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 __device__ __forceinline__ d3 tod(f3 v) { return mk((double)v.x, (double)v.y, (double)v.z); }
 
 // ray-plane test in a stage-local frame: p is relative to the stage origin
@@ -184,7 +191,7 @@ __device__ __forceinline__ bool plane_test32(const float *__restrict__ et, f3 p,
     f3 dp = subf(c, p);
     float kn = dotf(dp, ex);
     float kd = dotf(d, ex);
-    k = __fdividef(kn, kd);
+    k = fdiv_fast(kn, kd);
     ly = k * dotf(d, ey) - dotf(dp, ey);
     lz = k * dotf(d, ez) - dotf(dp, ez);
     return (kd != 0.0f) && (k >= kmin) && (fabsf(ly) <= hy) && (fabsf(lz) <= hz);
@@ -406,7 +413,7 @@ __device__ __forceinline__ int cull_cell(SIP si, GP g, V r, V d, R kmin, int &b,
     if (!(pa == pa) || !(da == da) || !(pb1 == pb1) || !(pb2 == pb2) || !(db1 == db1) || !(db2 == db2))
         return 0;  // NaN ray: k is NaN for every element, nothing can be hit
     if (!(fabs(da) > (R)1e-12)) return 2;
-    R ida = (R)1 / da;
+    R ida = recip(da);
     R k1 = (g[9] - pa) * ida, k2 = (g[10] - pa) * ida;
     R kmax = k1 > k2 ? k1 : k2;
     R kmn = k1 > k2 ? k2 : k1;
@@ -616,7 +623,7 @@ __device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, i
     float blaze = acosf(cb) + (pr[7] + ly * pr[8]);
     float pd0 = dotf(pp, dg);
     float sgn = sign_mode == 0 ? -1.0f : (pd0 < 0.0f ? -1.0f : 1.0f);
-    float p_d = pd0 + (sgn * (float)m) * wave / pr[6];
+    float p_d = pd0 + fdiv_fast((sgn * (float)m) * wave, pr[6]);
     float p_n = sqrtf(1.0f - p_d * p_d - pl * pl);
     float pn0 = dotf(pp, n);
     float w = (pn0 > 0.0f ? 1.0f : (pn0 < 0.0f ? -1.0f : 0.0f)) * trans * p_n;
@@ -973,7 +980,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             f3 a = ldf3(sf);
             float da = dotf(R.df, a);
             if (da != 0.0f && da == da) {
-                kadv = -dotf(r0, a) / da;
+                kadv = fdiv_fast(-dotf(r0, a), da);
                 if (!(fabsf(kadv) < 1e30f)) kadv = 0.0f;
             }
             pr = mkf(r0.x + kadv * R.df.x, r0.y + kadv * R.df.y, r0.z + kadv * R.df.z);
@@ -1069,7 +1076,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
                     count(cnt, MX_ST_ORDER_EMPTY);
                 blaze = diffract64<EXT>(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
             } else {
-                const float wave = MX_E2W_F / (float)R.energy;
+                const float wave = fdiv_fast(MX_E2W_F, (float)R.energy);
                 if (L && si[22] >= 0) {
                     const float *prm = (const float *)(lds + si[22]) + (size_t)hit * 16;
                     const float *er = (const float *)rec_l + (size_t)hit * MX_ET;
