@@ -341,7 +341,11 @@ __device__ __forceinline__ bool select_interp(const MxTables &T, HP hd, double e
     const double *wg = T.sel_d + hd[2], *tg = T.sel_d + hd[4], *dat = T.sel_d + hd[3];
     int iw, it;
     double hx0, hx1, hy0, hy1;
-    hat1(wg, nw, MX_KEV_NM / energy, iw, hx0, hx1);
+    // the empty asm keeps the compiler from hoisting this division out of
+    // the stage loop, where every photon would pay for it
+    double e = energy;
+    asm volatile("" : "+v"(e));
+    hat1(wg, nw, MX_KEV_NM / e, iw, hx0, hx1);
     hat1(tg, nt, blaze, it, hy0, hy1);
     const size_t o00 = (size_t)iw * nt + it, o01 = o00 + 1, o10 = o00 + nt, o11 = o10 + 1;
     const size_t stride = (size_t)nw * nt;
@@ -1267,10 +1271,28 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
 // EXT: the tables hold stack or tube stages (a separate instantiation, so that the
 // layer code does not weigh on the register allocation of the others).
 template <int PREC, int BS, bool EXT>
-__global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T, MxPhotons P, int flags,
+__global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons P_arg, int flags,
                                                        unsigned long long *status) {
+    // Read the two structs from the kernel-argument segment through
+    // constant-address-space pointers (the first two arguments, at offsets 0
+    // and sizeof(MxTables)): indexed columns (P.fcol[c]) and table pointers
+    // then become scalar loads where they are used, instead of ~90 argument
+    // registers kept live for the whole kernel and spilled into vector lanes.
+    static_assert(sizeof(MxTables) % alignof(MxPhotons) == 0, "kernel argument layout");
+    const __attribute__((address_space(4))) unsigned char *ka =
+        (const __attribute__((address_space(4))) unsigned char *)__builtin_amdgcn_kernarg_segment_ptr();
+    const MxTables &T = *(const MxTables *)((const __attribute__((address_space(4))) MxTables *)ka);
+    const MxPhotons &P = *(const MxPhotons *)((const __attribute__((address_space(4))) MxPhotons *)(ka + sizeof(MxTables)));
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned s_count[MX_NCOUNT];
+    // the layout check: the structs read through the pointers are the
+    // arguments (otherwise nothing is traced and the call reports bad
+    // arguments)
+    if (P.n != P_arg.n || P.prob != P_arg.prob || T.n_stages != T_arg.n_stages ||
+        T.stage_i != T_arg.stage_i) {
+        if (status && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(status + MX_ST_BAD_ARGS, 1ULL);
+        return;
+    }
     const int lds_bytes = PREC == 64 ? T.lds_bytes64 : T.lds_bytes32;
     if (threadIdx.x < MX_NCOUNT) s_count[threadIdx.x] = 0;
     if (lds_bytes > 0) {
