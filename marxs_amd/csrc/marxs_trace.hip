@@ -1371,7 +1371,17 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         int64_t e = b + tp;
         if (e > P.n) e = P.n;
         if (b < 0 || tp <= 0) e = b;
-        for (int64_t i = b + threadIdx.x; i < e; i += BS) trace_one<PREC, EXT>(T, P, i, use_grid, cnt, lds);
+        for (int64_t i = b + threadIdx.x; i < e; i += BS) {
+            // the struct pointers pass through an empty asm for every photon,
+            // so that the compiler reloads struct fields where they are used
+            // (scalar loads) instead of keeping them live across the loop
+            const __attribute__((address_space(4))) MxTables *tv =
+                (const __attribute__((address_space(4))) MxTables *)tabs + v;
+            const __attribute__((address_space(4))) MxPhotons *pv =
+                (const __attribute__((address_space(4))) MxPhotons *)phs + v;
+            asm volatile("" : "+s"(tv), "+s"(pv));
+            trace_one<PREC, EXT>(*(const MxTables *)tv, *(const MxPhotons *)pv, i, use_grid, cnt, lds);
+        }
         __syncthreads();
         // this tile's counters go to its variant; the next claim's barrier
         // orders the reset before the next tile counts
