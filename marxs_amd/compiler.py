@@ -601,7 +601,8 @@ def _align(n, a=16):
 
 
 def _param_records32(tables, st):
-    """FP32 interaction parameters of a grating stage, 16 words per element:
+    """FP32 interaction parameters of a grating or stack stage (the
+    membrane), 16 words per element:
     l (3 f32), d = -e_perp (3 f32), period, blaze centre, blaze slope (f32),
     id, selector, sign mode, transmission, npix x, npix y (i32)."""
     n = len(st.elements)
@@ -683,7 +684,8 @@ def _build_lds(tables, budget32=None, budget64=None):
             if st.kind not in PLANE_ARRAYS or n < 2:
                 continue
             recs = (tables.elem_t32 if prec == 32 else tables.elem_t)[st.elem0:st.elem0 + n]
-            params = _param_records32(tables, st) if (prec == 32 and st.kind == STAGE_GRATING) else None
+            params = (_param_records32(tables, st)
+                      if (prec == 32 and st.kind in (STAGE_GRATING, STAGE_STACK)) else None)
             g = st.grid
             if g is not None and (max(len(g['items']), len(g['items_l']), n,
                                       len(g['succ_items']), len(g['succ_items_l'])) >= 65535):
@@ -843,6 +845,22 @@ def _grating_params(x, ep, ei, isel, sel_index, sel_rows, sel_d, sel_o):
     ei[isel + 2] = 1 if x.transmission else 0
 
 
+def _bucket_index(x, per_entry=4):
+    """Search accelerator of an increasing table x (stored after its values):
+    nb, 1 / width, then for k = 0 .. nb the first index i with
+    x[i] >= x[0] + k * width, the last set to len(x).  nb = 0 (plain binary
+    search) for tables of fewer than 8 entries."""
+    n = len(x)
+    nb = per_entry * n if n >= 8 else 0
+    if nb == 0:
+        return np.zeros(2)
+    width = (x[-1] - x[0]) / nb
+    first = np.searchsorted(x, x[0] + width * np.arange(nb + 1), side='left').astype(float)
+    first[0] = 0
+    first[-1] = n
+    return np.concatenate([[nb, 1.0 / width], first])
+
+
 def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
     """Parameters of a stack element.  The membrane grating as a grating
     element (ep[0:9], ei[1:4]); the rest in a record of SR float64 in sel_d
@@ -850,7 +868,8 @@ def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
     quality factor (9), L1 open fraction (10), L2 open fraction and
     shadowing coefficient (11, 12), scatter parameter (13: sigma in rad, or
     the L2 inner free width in mm), offset and length of the L1 transmission
-    table in sel_d (14, 15: energies in keV, then transmissions), radial
+    table in sel_d (14, 15: energies in keV, then transmissions, then its
+    bucket index, ``_bucket_index``), radial
     scatter: perpendicular sigma (16) and centre (17-19).  L1 selector row,
     sign mode and transmission flag in ei[7], ei[4], ei[5]."""
     rec = np.zeros(SR)
@@ -873,6 +892,7 @@ def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
                 sel_index[key] = len(sel_d)
                 sel_d.extend(te.tolist())
                 sel_d.extend(tv.tolist())
+                sel_d.extend(_bucket_index(te).tolist())
             rec[14] = sel_index[key]
             rec[15] = len(te)
         elif op == OP_QF:

@@ -46,7 +46,10 @@
 #define MX_BLOCK64 768
 #endif
 #ifndef MX_BLOCK_EXT
-#define MX_BLOCK_EXT 768        /* kernels with stack or tube stages, both precisions */
+#define MX_BLOCK_EXT 768        /* the FP64 kernels with stack or tube stages */
+#endif
+#ifndef MX_BLOCK_EXT32
+#define MX_BLOCK_EXT32 768      /* the FP32 kernels with stack or tube stages */
 #endif
 #define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
 #ifndef MX_CDF_COUNT
@@ -644,10 +647,22 @@ __device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, i
 // interp1d's arithmetic (marxs/missions/mitsnl/catgrating.py:189-192):
 // interval [lo, lo+1] with lo + 1 the first index whose energy is >= e,
 // clamped to [1, n-1]; slope * (e - x[lo]) + y[lo].  out: e outside the
-// table (interp1d raises).
+// table (interp1d raises).  After the n transmissions the table holds a
+// bucket index: nb, 1 / bucket width, then for k = 0 .. nb the first index
+// whose energy is >= x[0] + k * width (the last = n); the search starts from
+// the bucket of e widened by one bucket on each side, so that rounding in
+// the bucket number cannot leave the answer outside it.
 __device__ __forceinline__ double trans_interp(const double *x, const double *y, int n, double e, bool &out) {
     out = (e < x[0]) || (e > x[n - 1]);
+    const double *bk = y + n;
+    const int nb = (int)bk[0];
     int lo = 0, hi = n;
+    if (nb > 2) {
+        const double t = (e - x[0]) * bk[1];
+        const int b = t > 0.0 ? (t < (double)nb ? (int)t : nb - 1) : 0;
+        lo = (int)bk[2 + (b > 0 ? b - 1 : 0)];
+        hi = (int)bk[2 + (b + 2 < nb ? b + 2 : nb)];
+    }
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (x[mid] < e) lo = mid + 1;
@@ -679,7 +694,7 @@ __device__ __forceinline__ d3 axangle_t(d3 ax, double ang, d3 v) {
 
 // FlatStack.process_photons (marxs/optics/base.py:258-281) for one photon
 // at its intersection with stack element (et, ep, ei): the layer program
-// stage_i[47] runs in order, in float64 for both precisions, each layer with
+// stage_i[47] runs in order, in float64 (the FP32 path: stack_layers32), each layer with
 // the direction and polarization left by the one before it.
 //  grating  membrane (grating.py:233-266): draw 0, order, blaze, probability
 //  qf       exp(-(2 pi sigma/d)^2) ** (order^2) (catgrating.py:147-160)
@@ -800,6 +815,174 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
     }
     if (PREC == 64) { R.d = d; R.pol = pol; }
     else { R.df = tof(d); R.polf = tof(pol); }
+}
+
+// axangle_t in float (the FP32 path).
+__device__ __forceinline__ f3 axangle_t32(f3 ax, float ang, f3 v) {
+#pragma clang fp contract(fast)
+    ax = unitf(ax);
+    const float x = ax.x, y = ax.y, z = ax.z;
+    float s, c;
+    sincosf(ang, &s, &c);
+    const float C = 1.0f - c;
+    const float xs = x * s, ys = y * s, zs = z * s;
+    const float xC = x * C, yC = y * C, zC = z * C;
+    const float xyC = x * yC, yzC = y * zC, zxC = z * xC;
+    const float m00 = x * xC + c, m01 = xyC - zs, m02 = zxC + ys;
+    const float m10 = xyC + zs, m11 = y * yC + c, m12 = yzC - xs;
+    const float m20 = zxC - ys, m21 = yzC + xs, m22 = z * zC + c;
+    return mkf(m00 * v.x + m10 * v.y + m20 * v.z,
+               m01 * v.x + m11 * v.y + m21 * v.z,
+               m02 * v.x + m12 * v.y + m22 * v.z);
+}
+
+// b ** e for an integer e >= 0 by repeated squaring (float64).
+__device__ __forceinline__ double ipow(double b, int e) {
+    double r = 1.0;
+    while (e) {
+        if (e & 1) r *= b;
+        b *= b;
+        e >>= 1;
+    }
+    return r;
+}
+
+// The layer program of stack_layers for the FP32 path: the same layers, draws
+// and decisions, with direction and polarization kept in float (FP32
+// diffraction as for plain gratings, FP32 rotations); order selection, the
+// inverse normal CDF, the quality factor, the bar test and the bar
+// transmission stay in float64.  er: the element's FP32 test record (normal
+// at words 3-5); pl: its FP32 parameter record in LDS or null; eg: global
+// element index.
+//  * L2 absorption: sin(angle to the normal) = |unit(dir) x n|, which keeps
+//    its digits at small angles where arccos of a float dot product does not;
+//  * scatter: the reference's |p_x| < 0.99999 test for the helper axis is
+//    made as (1 - c) p_x^2 < c (p_y^2 + p_z^2), c = 0.99999^2, which float
+//    resolves near the threshold (|p_x| itself rounds to 6e-8 there).
+__device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhotons &P, Ray &R, const cptr<int> si,
+                                               const float *er, const float *pl, size_t eg, const int *ei,
+                                               float ly, int nhit, int64_t i, const Counters &cnt,
+                                               const unsigned char *lds) {
+#pragma clang fp contract(fast)
+    const int prog = si[47];
+    const double *sr = T.sel_d + ei[6];
+    const f3 n = ldf3(er + 3);
+    const double energy = R.energy;
+    const int k0 = MX_STACK_DRAWS * nhit;
+    int m_last = MX_ORDER_NONE;
+    const int nops = prog & 15;
+    for (int k = 0; k < nops; ++k) {
+        const int op = (prog >> (4 + 3 * k)) & 7;
+        if (op == MX_OP_GRATING || op == MX_OP_L1) {
+            const bool l1 = op == MX_OP_L1;
+            // l (0-2), d (3-5), period, blaze centre and slope; sign mode and
+            // transmission as int bits in words 11, 12
+            float prm[13];
+            if (l1) {
+                for (int j = 0; j < 9; ++j) prm[j] = (float)sr[j];
+            } else if (pl) {
+                for (int j = 0; j < 9; ++j) prm[j] = pl[j];
+            } else {
+                for (int j = 0; j < 9; ++j) prm[j] = T.elem_p32[eg * MX_EP + j];
+            }
+            prm[9] = prm[10] = 0.0f;
+            prm[11] = __int_as_float(l1 ? ei[4] : ei[2]);
+            prm[12] = __int_as_float(l1 ? ei[5] : ei[3]);
+            const double u = draw(T, P, R, si[4], si[5], k0 + (l1 ? 1 : 0), i, cnt);
+            int m; double ptot;
+            const f3 d_old = R.df, pol_old = R.polf;
+            if (!select_order<32>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds,
+                                  [&]() { return (double)blaze32(n, prm, ly, d_old); }))
+                count(cnt, MX_ST_ORDER_EMPTY);
+            const float wave = fdiv_fast(MX_E2W_F, (float)energy);
+            const double blaze = (double)diffract32(n, prm, wave, ly, m, R.df, R.polf);
+            if (l1) {
+                const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
+                if (u2 > sr[10]) {
+                    R.df = d_old;
+                    R.polf = pol_old;
+                    m = 0;
+                    const int64_t off = (int64_t)sr[14];
+                    const int nt = (int)sr[15];
+                    bool out;
+                    ptot = trans_interp(T.sel_d + off, T.sel_d + off + nt, nt, energy, out);
+                    if (out) count(cnt, MX_ST_TRANS_RANGE);
+                }
+                put_i(P, si[44], i, m);
+                put_f(P, si[45], i, blaze);
+            } else {
+                m_last = m;
+                put_i(P, si[13], i, m);
+                put_f(P, si[14], i, blaze);
+            }
+            if (ptot < 0.0 || ptot > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * ptot;
+        } else if (op == MX_OP_QF) {
+            double f;
+            if (m_last > -46341 && m_last < 46341) {
+                f = ipow(sr[9], m_last * m_last);
+            } else {
+                const double fm = (double)m_last;
+                f = pow_call(sr[9], fm * fm);
+            }
+            if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * f;
+        } else if (op == MX_OP_L2ABS) {
+            const f3 cr = crossf(unitf(R.df), n);
+            const double f = sr[11] - sr[12] * (double)sqrtf(dotf(cr, cr));
+            if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * f;
+        } else if (op == MX_OP_SCAT_L2 || op == MX_OP_SCAT_CONST) {
+            float sigma;
+            if (op == MX_OP_SCAT_L2) {
+                const float wave = fdiv_fast((float)(MX_KEV_NM * 1e-6), (float)energy);
+                sigma = (1.22f * 0.4f) * asinf(fdiv_fast(wave, (float)sr[13]));
+            } else {
+                sigma = (float)sr[13];
+            }
+            const double u3 = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
+            const double u4 = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
+            const float ang = (float)inv_normal_cdf(u3) * sigma;
+            const f3 pd = unitf(R.df);
+            const float cc = 0.99999f * 0.99999f;
+            const bool xaxis = (1.0f - cc) * (pd.x * pd.x) < cc * (pd.y * pd.y + pd.z * pd.z);
+            const f3 guess = xaxis ? mkf(1.0f, 0.0f, 0.0f) : mkf(0.0f, 1.0f, 0.0f);
+            // the two rotations in closed form: with a = unit(pd x guess),
+            // b = a x pd, the first gives cos(ang) pd - sin(ang) b and the
+            // second, by phi about pd, cos(ang) pd + sin(ang) (sin(phi) a -
+            // cos(phi) b); the deflection is added to pd as a small term
+            const f3 a = unitf(crossf(pd, guess));
+            const f3 b = crossf(a, pd);
+            float sa, ca, sp, cp;
+            sincosf(ang, &sa, &ca);
+            sincosf((float)(u4 * (2 * M_PI)), &sp, &cp);
+            const float ta = sa * sp, tb = sa * cp;
+            const f3 nd = mkf(ca * pd.x + (ta * a.x - tb * b.x), ca * pd.y + (ta * a.y - tb * b.y),
+                              ca * pd.z + (ta * a.z - tb * b.z));
+            R.polf = ptrans32(R.df, nd, R.polf);
+            R.df = nd;
+            put_f(P, si[46], i, (double)ang);
+        } else if (op == MX_OP_RADIAL) {
+            const f3 radial = tof(sub3(R.p, ld3(sr + 17)));
+            const f3 perpplane = crossf(R.df, radial);
+            f3 nd = R.df;
+            float a_in = 0.0f, a_perp = 0.0f;
+            if (sr[13] != 0.0) {
+                a_in = (float)(inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 3, i, cnt)) * sr[13]);
+                nd = axangle_t32(perpplane, a_in, R.df);
+            }
+            if (sr[16] != 0.0) {
+                a_perp = (float)(inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 4, i, cnt)) * sr[16]);
+                nd = axangle_t32(radial, a_perp, nd);
+            }
+            R.polf = ptrans32(R.df, nd, R.polf);
+            R.df = nd;
+            put_f(P, si[46], i, (double)a_in);
+            put_f(P, si[48], i, (double)a_perp);
+        } else {
+            count(cnt, MX_ST_BAD_ARGS);
+        }
+    }
 }
 
 // ------------------------------------------------------------ tube stages
@@ -1109,7 +1292,13 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             put_f(P, si[14], i, blaze);
         } else if (EXT && kind == MX_STAGE_STACK) {
             id = ei[0];
-            stack_layers<PREC>(T, P, R, si, et, ep, ei, ly, nhit, i, cnt, lds);
+            if (PREC == 32) {
+                const float *er = L ? (const float *)rec_l + (size_t)hit * MX_ET : T.elem_t32 + eg * MX_ET;
+                const float *pl = (L && si[22] >= 0) ? (const float *)(lds + si[22]) + (size_t)hit * 16 : nullptr;
+                stack_layers32(T, P, R, si, er, pl, eg, ei, (float)ly, nhit, i, cnt, lds);
+            } else {
+                stack_layers<PREC>(T, P, R, si, et, ep, ei, ly, nhit, i, cnt, lds);
+            }
             put_f(P, si[42], i, ly);
             put_f(P, si[43], i, lz);
         } else if (kind == MX_STAGE_DETECTOR) {
@@ -1430,7 +1619,8 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     }
     if (photons->n == 0 || tables->n_stages == 0) return 0;
     const bool ext = tables->ext_stages != 0;
-    const int block = ext ? MX_BLOCK_EXT : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
+    const int block = ext ? (precision == MX_PREC_FP64 ? MX_BLOCK_EXT : MX_BLOCK_EXT32)
+                          : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
     const int64_t nblk = (photons->n + block - 1) / block;
     hipStream_t st = (hipStream_t)stream;
     const int lds = precision == MX_PREC_FP64 ? tables->lds_bytes64 : tables->lds_bytes32;
@@ -1454,7 +1644,7 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
         hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64, false>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else if (ext)
-        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK_EXT, true>), dim3((unsigned)grid), dim3(block), lds, st,
+        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK_EXT32, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else
         hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32, false>), dim3((unsigned)grid), dim3(block), lds, st,
@@ -1502,7 +1692,8 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
         if (t.ext_stages) ext = true;
     }
     if (b->n_tiles == 0) return 0;
-    const int block = ext ? MX_BLOCK_EXT : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
+    const int block = ext ? (precision == MX_PREC_FP64 ? MX_BLOCK_EXT : MX_BLOCK_EXT32)
+                          : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
     int64_t grid = b->grid;
     if (grid <= 0) {
         int dev = 0, cus = 0;
@@ -1525,7 +1716,7 @@ extern "C" int mx_trace_batched(const MxBatch *b, int32_t precision, int32_t fla
                        (long long)b->n_tiles, (int)flags, b->queue, b->status)
     if (precision == MX_PREC_FP64 && ext) MX_LAUNCH_BATCHED(64, MX_BLOCK_EXT, true);
     else if (precision == MX_PREC_FP64) MX_LAUNCH_BATCHED(64, MX_BLOCK64, false);
-    else if (ext) MX_LAUNCH_BATCHED(32, MX_BLOCK_EXT, true);
+    else if (ext) MX_LAUNCH_BATCHED(32, MX_BLOCK_EXT32, true);
     else MX_LAUNCH_BATCHED(32, MX_BLOCK32, false);
 #undef MX_LAUNCH_BATCHED
     hipError_t err = hipGetLastError();

@@ -60,7 +60,7 @@ def test_cat_stack_line_spectrum_600(gpu):
 
 
 def test_cat_stack_fp32(gpu):
-    """FP32 path: plane tests in float, layers in float64; detector
+    """FP32 path: plane tests and layers in float; detector
     coordinates within 0.5 um RMS of the FP64 oracle."""
     inst, parts = _stack_instrument(600)
     ci = _ci(inst)

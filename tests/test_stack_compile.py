@@ -158,3 +158,31 @@ def test_adjoint_rejects_stacks_and_tubes():
     with pytest.raises(C.CompileError, match='flat grating'):
         AdjointProblem(parts['gratings'], parts['ccds'], z, z, z[0], z[0], zi, zi, zi,
                        device='cpu')
+
+
+def test_transmission_bucket_index_brackets_the_search():
+    """The bucket index stored after an L1 transmission table: the window the
+    kernel searches (the bucket of e, widened by one bucket each side, bucket
+    number from float64 arithmetic) holds the first index whose energy is
+    >= e, for energies on the table, on bucket edges and one ulp either side,
+    outside the table, and random ones; tables from the SiTransmission file,
+    a geometric one and the shortest that gets an index."""
+    rows = [ln.split() for ln in open(os.path.join(os.path.dirname(__file__), 'golden',
+                                                   'SiTransmission.csv')) if not ln.startswith('#')]
+    si = np.array(rows[1:], dtype=float)[:, 0] / 1000.
+    rng = np.random.default_rng(5)
+    for x in (si, np.geomspace(0.1, 10., 300), np.linspace(1., 2., 8)):
+        bk = C._bucket_index(x)
+        nb, inv_w, first = int(bk[0]), bk[1], bk[2:].astype(int)
+        assert nb == 4 * len(x) and len(first) == nb + 1 and first[0] == 0 and first[-1] == len(x)
+        edges = x[0] + np.arange(nb + 1) / inv_w
+        e = np.concatenate([x, edges, np.nextafter(edges, -np.inf), np.nextafter(edges, np.inf),
+                            np.nextafter(x, -np.inf), np.nextafter(x, np.inf),
+                            rng.uniform(x[0] - 1, x[-1] + 1, 20000), [-np.inf, np.inf]])
+        t = (e - x[0]) * inv_w
+        b = np.where(t > 0, np.where(t < nb, np.floor(np.where(np.isfinite(t), t, 0)), nb - 1), 0).astype(int)
+        lo = first[np.maximum(b - 1, 0)]
+        hi = first[np.minimum(b + 2, nb)]
+        want = np.searchsorted(x, e, side='left')
+        assert np.all((lo <= want) & (want <= hi))
+    assert np.array_equal(C._bucket_index(np.arange(7.)), [0., 0.])
