@@ -873,48 +873,58 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
     const int nops = prog & 15;
     for (int k = 0; k < nops; ++k) {
         const int op = (prog >> (4 + 3 * k)) & 7;
-        if (op == MX_OP_GRATING || op == MX_OP_L1) {
-            const bool l1 = op == MX_OP_L1;
-            // l (0-2), d (3-5), period, blaze centre and slope; sign mode and
-            // transmission as int bits in words 11, 12
+        if (op == MX_OP_GRATING) {
+            // parameter record: l (0-2), d (3-5), period, blaze centre and
+            // slope; sign mode and transmission as int bits in words 11, 12
             float prm[13];
-            if (l1) {
-                for (int j = 0; j < 9; ++j) prm[j] = (float)sr[j];
-            } else if (pl) {
-                for (int j = 0; j < 9; ++j) prm[j] = pl[j];
-            } else {
+            const float *pm = pl;
+            if (!pm) {
                 for (int j = 0; j < 9; ++j) prm[j] = T.elem_p32[eg * MX_EP + j];
+                prm[9] = prm[10] = 0.0f;
+                prm[11] = __int_as_float(ei[2]);
+                prm[12] = __int_as_float(ei[3]);
+                pm = prm;
             }
-            prm[9] = prm[10] = 0.0f;
-            prm[11] = __int_as_float(l1 ? ei[4] : ei[2]);
-            prm[12] = __int_as_float(l1 ? ei[5] : ei[3]);
-            const double u = draw(T, P, R, si[4], si[5], k0 + (l1 ? 1 : 0), i, cnt);
+            const double u = draw(T, P, R, si[4], si[5], k0, i, cnt);
             int m; double ptot;
-            const f3 d_old = R.df, pol_old = R.polf;
-            if (!select_order<32>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds,
-                                  [&]() { return (double)blaze32(n, prm, ly, d_old); }))
+            if (!select_order<32>(T, ei[1], energy, u, m, ptot, lds,
+                                  [&]() { return (double)blaze32(n, pm, ly, R.df); }))
                 count(cnt, MX_ST_ORDER_EMPTY);
             const float wave = fdiv_fast(MX_E2W_F, (float)energy);
-            const double blaze = (double)diffract32(n, prm, wave, ly, m, R.df, R.polf);
-            if (l1) {
-                const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
-                if (u2 > sr[10]) {
-                    R.df = d_old;
-                    R.polf = pol_old;
-                    m = 0;
-                    const int64_t off = (int64_t)sr[14];
-                    const int nt = (int)sr[15];
-                    bool out;
-                    ptot = trans_interp(T.sel_d + off, T.sel_d + off + nt, nt, energy, out);
-                    if (out) count(cnt, MX_ST_TRANS_RANGE);
-                }
-                put_i(P, si[44], i, m);
-                put_f(P, si[45], i, blaze);
+            const double blaze = (double)diffract32(n, pm, wave, ly, m, R.df, R.polf);
+            m_last = m;
+            put_i(P, si[13], i, m);
+            put_f(P, si[14], i, blaze);
+            if (ptot < 0.0 || ptot > 1.0) count(cnt, MX_ST_PROB_RANGE);
+            R.prob = R.prob * ptot;
+        } else if (op == MX_OP_L1) {
+            float prm[13];
+            for (int j = 0; j < 9; ++j) prm[j] = (float)sr[j];
+            prm[9] = prm[10] = 0.0f;
+            prm[11] = __int_as_float(ei[4]);
+            prm[12] = __int_as_float(ei[5]);
+            const double u = draw(T, P, R, si[4], si[5], k0 + 1, i, cnt);
+            const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
+            int m; double ptot;
+            if (!select_order<32>(T, ei[7], energy, u, m, ptot, lds,
+                                  [&]() { return (double)blaze32(n, prm, ly, R.df); }))
+                count(cnt, MX_ST_ORDER_EMPTY);
+            double blaze;
+            if (u2 > sr[10]) {
+                // through a bar: direction kept, blaze of the incoming ray
+                blaze = (double)blaze32(n, prm, ly, R.df);
+                m = 0;
+                const int64_t off = (int64_t)sr[14];
+                const int nt = (int)sr[15];
+                bool out;
+                ptot = trans_interp(T.sel_d + off, T.sel_d + off + nt, nt, energy, out);
+                if (out) count(cnt, MX_ST_TRANS_RANGE);
             } else {
-                m_last = m;
-                put_i(P, si[13], i, m);
-                put_f(P, si[14], i, blaze);
+                const float wave = fdiv_fast(MX_E2W_F, (float)energy);
+                blaze = (double)diffract32(n, prm, wave, ly, m, R.df, R.polf);
             }
+            put_i(P, si[44], i, m);
+            put_f(P, si[45], i, blaze);
             if (ptot < 0.0 || ptot > 1.0) count(cnt, MX_ST_PROB_RANGE);
             R.prob = R.prob * ptot;
         } else if (op == MX_OP_QF) {
@@ -1082,11 +1092,42 @@ __device__ __forceinline__ void tube_stage(const MxTables &T, const MxPhotons &P
 #ifndef MX_SCAN_GROUP
 #define MX_SCAN_GROUP 1
 #endif
+#ifndef MX_SCAN_PREFETCH
+#define MX_SCAN_PREFETCH 1
+#endif
 template <int PREC, typename ETP, typename ITP>
 __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, int te,
                                          int last, d3 p, d3 d, f3 pr, f3 df, float kmin32,
                                          d3 &q, double &ly, double &lz, float &kf) {
     constexpr int G = MX_SCAN_GROUP;
+#if MX_SCAN_PREFETCH
+    if (G == 1 && list && PREC == 64) {
+        // the next candidate's index is read while the current one is
+        // tested, so that each test waits for one LDS round trip, not two
+        // (FP64: 3.5 % faster; FP32 unchanged, so it keeps the plain loop)
+        int jn = tb < te ? (int)items[tb] : -1;
+        for (int t = tb; t < te; ++t) {
+            const int j0 = jn;
+            jn = t + 1 < te ? (int)items[t + 1] : -1;
+            const int j = j0 > last ? j0 : 0;
+            if (PREC == 64) {
+                d3 qq;
+                double yy, zz;
+                if (plane_test64((const double *)et + (size_t)j * MX_ET, p, d, qq, yy, zz) && j0 > last) {
+                    q = qq; ly = yy; lz = zz;
+                    return j0;
+                }
+            } else {
+                float kk, yy, zz;
+                if (plane_test32((const float *)et + (size_t)j * MX_ET, pr, df, kmin32, kk, yy, zz) && j0 > last) {
+                    kf = kk; ly = yy; lz = zz;
+                    return j0;
+                }
+            }
+        }
+        return -1;
+    }
+#endif
     for (int t = tb; t < te; t += G) {
         int jj[G];
         bool hh[G];
