@@ -19,7 +19,7 @@ from .optics import (OrderSelector, EfficiencyFile, InterpolateEfficiencyTable, 
                      OpticalElement, FlatStack, CircularDetector)
 from .scatter import RandomGaussianScatter  # noqa: E402
 from .catgrating import CATL1L2Stack  # noqa: E402
-from .simulator import Sequence, Parallel, ParallelCalculated  # noqa: E402
+from .simulator import Sequence, Parallel, ParallelCalculated, propagate  # noqa: E402
 from .rowland import (RowlandTorus, GratingArrayStructure, RectangularGrid,  # noqa: E402
                       CircularMeshGrid, ElementsOnTorus, design_tilted_torus)
 

@@ -2,15 +2,24 @@
 (``marxs/analysis/gratings.py``), computed on the GPU.
 
 ``effectivearea_from_photonlist`` (``gratings.py:333-363``),
-``resolvingpower_from_photonlist`` (``215-270``), ``average_R_Aeff``
-(``187-212``), ``CaptureResAeff`` and ``CaptureResAeff_CCDgaps``
-(``389-553``) keep their names, arguments and results.  The photon list may
+``resolvingpower_from_photonlist`` (``215-270``),
+``resolvingpower_from_photonlist_robust`` (``273-330``),
+``resolvingpower_per_order`` (``26-130``), ``weighted_per_order``
+(``133-178``), ``average_R_Aeff`` (``187-212``),
+``identify_photon_in_subaperture`` (``366-386``), ``CaptureResAeff`` and
+``CaptureResAeff_CCDgaps`` (``389-553``), and from
+``marxs/analysis/analysis.py`` ``sigma_clipped_std``, ``mean_width_2d``,
+``find_best_detector_position`` and ``detected_fraction`` keep their names,
+arguments and results.  The photon list may
 be a :class:`~marxs_amd.tracer.DevicePhotons` (columns stay in HBM), a
 :class:`~marxs_amd.table.PhotonTable` or a dict of arrays (columns are copied
 to the GPU).  The per-order sums and the sigma clipping (astropy's
 ``sigma_clipped_stats`` with the reference's defaults: 3 sigma, at most 5
 rounds, median centre) run in ``mx_order_stats``; only the few per-order
-numbers come back to the host.
+numbers come back to the host.  The functions that trace (the per-order
+loop of ``resolvingpower_per_order``, the detector positions tried by
+``find_best_detector_position``) keep the photons in HBM between traces and
+run the FP64 kernels.
 """
 import numpy as np
 import torch
@@ -20,7 +29,11 @@ from .compiler import ORDER_NONE
 
 __all__ = ['AnalysisError', 'order_stats', 'sigma_clipped_stats',
            'effectivearea_from_photonlist', 'resolvingpower_from_photonlist',
-           'average_R_Aeff', 'CaptureResAeff', 'CaptureResAeff_CCDgaps']
+           'resolvingpower_from_photonlist_robust', 'resolvingpower_per_order',
+           'weighted_per_order', 'identify_photon_in_subaperture',
+           'average_R_Aeff', 'CaptureResAeff', 'CaptureResAeff_CCDgaps',
+           'sigma_clipped_std', 'mean_width_2d', 'find_best_detector_position',
+           'detected_fraction']
 
 RES_FWHM = 2.3548     # FWHM / sigma (gratings.py:268)
 
@@ -254,3 +267,180 @@ class CaptureResAeff_CCDgaps(CaptureResAeff):
 
     def aeff_filter(self, photons):
         return _column(photons, self.aeff_filter_col, _device()) >= 0
+
+
+def resolvingpower_from_photonlist_robust(lphotons, orders, cols, zeropositions, ordercol='order'):
+    """Resolving power of each order from several photon lists / columns, the
+    lowest per order with its position and width (gratings.py:273-330)."""
+    if len(cols) != len(zeropositions):
+        raise ValueError('Number of elements in cols and zeropositions is not the same.')
+    if len(cols) != len(lphotons):
+        raise ValueError('Number of elements in cols and photon lists is not the same.')
+    results = np.zeros((len(cols), len(orders)))
+    positions = np.zeros_like(results)
+    stds = np.zeros_like(results)
+    for i, (photons, col, zeropos) in enumerate(zip(lphotons, cols, zeropositions)):
+        results[i], positions[i], stds[i] = resolvingpower_from_photonlist(
+            photons, orders, col=col, zeropos=zeropos, ordercol=ordercol)
+    ind = np.argmin(results, axis=0)
+    ind_n = np.arange(results.shape[1])
+    return results[ind, ind_n], positions[ind, ind_n], stds[ind, ind_n]
+
+
+def weighted_per_order(data, orders, energy, gratingeff):
+    """Average of a per-order, per-energy quantity weighted by the order
+    probabilities of ``gratingeff`` interpolated to ``energy``
+    (gratings.py:133-178; host arithmetic on a few numbers)."""
+    if len(orders) != data.shape[0]:
+        raise ValueError('First dimension of "data" must match length of "orders".')
+    if len(energy) != data.shape[1]:
+        raise ValueError('Second dimension of "data" must match length of "energy".')
+    weights = np.zeros_like(data)
+    for i, o in enumerate(orders):
+        ind_o = (gratingeff.orders == o).nonzero()[0]
+        if len(ind_o) != 1:
+            raise KeyError('No data for order {0} in gratingeff'.format(o))
+        en_sort = np.argsort(gratingeff.energy)
+        weights[i, :] = np.interp(energy, gratingeff.energy[en_sort],
+                                  gratingeff.prob[:, ind_o[0]][en_sort])
+    return np.ma.average(data, axis=0, weights=weights)
+
+
+def identify_photon_in_subaperture(angle, max_ang, ang_0=np.pi / 2):
+    """Photons in the two mirrored sectors ang_0 +- max_ang and
+    2 pi - ang_0 +- max_ang (gratings.py:366-386; numpy's isclose with its
+    default relative tolerance, as the reference)."""
+    a = np.mod(angle, 2 * np.pi)
+    return np.isclose(a, ang_0, atol=max_ang) | np.isclose(a, 2 * np.pi - ang_0, atol=max_ang)
+
+
+# ------------------------------------------------------ traces for analysis
+def _device_photons(photons):
+    from .tracer import DevicePhotons
+    if isinstance(photons, DevicePhotons):
+        return photons
+    return DevicePhotons.from_table(photons, device=_device())
+
+
+def _trace(tree, src, precision=64):
+    """A copy of device photons ``src`` traced through ``tree`` (the
+    reference's ``tree(photons.copy())``); columns already on ``src`` keep
+    their values where the tree does not write them."""
+    from .tracer import CompiledInstrument
+    ci = CompiledInstrument(tree, device=src.device)
+    out = src.clone()
+    t = ci.tables
+    existing = [c for c in list(out.fcols) + list(out.icols) if c in t.fcols or c in t.icols]
+    ci.trace_device(out, precision=precision, existing=existing)
+    return out
+
+
+def _select(dp, mask):
+    """The photons of ``dp`` where ``mask`` (a device bool tensor) holds."""
+    from .tracer import DevicePhotons
+    idx = torch.nonzero(mask).squeeze(1)
+    out = DevicePhotons.__new__(DevicePhotons)
+    out.device, out.n, out.has_pos = dp.device, int(idx.numel()), dp.has_pos
+    for a in ('pos', 'dir', 'pol'):
+        setattr(out, a, getattr(dp, a)[:, idx].contiguous())
+    out.energy = dp.energy[idx].contiguous()
+    out.prob = dp.prob[idx].contiguous()
+    out.fcols = {k: v[idx].contiguous() for k, v in dp.fcols.items()}
+    out.icols = {k: v[idx].contiguous() for k, v in dp.icols.items()}
+    return out
+
+
+def sigma_clipped_std(photons, colname='det_x', sigma=3., maxiters=5):
+    """Standard deviation of the sigma-clipped column (analysis.py:9-24)."""
+    return sigma_clipped_stats(_column(photons, colname, _device()), sigma=sigma,
+                               maxiters=maxiters)[2]
+
+
+def mean_width_2d(photons):
+    """Mean distance of (det_x, det_y) from their means (analysis.py:27-39)."""
+    dev = _device()
+    x = _column(photons, 'det_x', dev).to(torch.float64)
+    y = _column(photons, 'det_y', dev).to(torch.float64)
+    r = torch.sqrt((x - x.mean()) ** 2 + (y - y.mean()) ** 2)
+    return float(r.sum() / r.numel())
+
+
+def find_best_detector_position(photons, objective_func=sigma_clipped_std,
+                                objective_func_args={'colname': 'det_x'},
+                                orientation=np.eye(3), **kwargs):
+    """Position of best focus (analysis.py:42-81): ``minimize_scalar`` over x
+    of ``objective_func`` on the photons traced (on the GPU, photons kept in
+    HBM) to a flat detector at ``orientation . (x, 0, 0)``.  The objective
+    receives a :class:`~marxs_amd.tracer.DevicePhotons`."""
+    import scipy.optimize
+    from .optics import FlatDetector
+    src = _device_photons(photons)
+
+    def width(x):
+        det = FlatDetector(position=np.dot(orientation, np.array([x, 0, 0])),
+                           orientation=orientation, zoom=1e5, pixsize=1.)
+        return objective_func(_trace(det, src), **objective_func_args)
+
+    return scipy.optimize.minimize_scalar(width, **kwargs)
+
+
+def detected_fraction(photons, labels, col='order'):
+    """Summed probability per label / number of photons (analysis.py:84-107)."""
+    dev = _device()
+    lab = _column(photons, col, dev)
+    r = order_stats(lab, _column(photons, 'probability', dev), labels)
+    return r['prob_sum'] / lab.numel()
+
+
+def resolvingpower_per_order(gratings, photons, orders, detector=None, colname='det_x'):
+    """Resolving power and FWHM of every order (gratings.py:26-130): the same
+    photons (ready to hit the gratings) are sent into one order at a time
+    (the order selectors of ``gratings`` are replaced, as in the reference),
+    traced to the detector and measured with sigma-clipped statistics.
+    ``detector``: an element, a ``RowlandTorus`` (a tube on its Rowland
+    circle, column ``detpix_x``) or None (a flat detector at the x of best
+    focus of each order, found by ``find_best_detector_position``).  Returns
+    (res, fwhm, info)."""
+    from .geometry import Cylinder
+    from .optics import CircularDetector, FlatDetector, OrderSelector
+    from .rowland import RowlandTorus
+    orders = np.asarray(orders)
+    res = np.zeros_like(orders, dtype=float)
+    fwhm = np.zeros_like(orders, dtype=float)
+    info = {}
+    src = _device_photons(photons)
+    if detector is None:
+        info['method'] = 'Detector position numerically optimized'
+        info['fit_results'] = []
+        col = 'det_x'
+        zeropos = 0.
+    elif isinstance(detector, RowlandTorus):
+        det = CircularDetector(geometry=Cylinder.from_rowland(
+            detector, width=1e5, rotation=np.pi, kwargs={'phi_lim': [-np.pi / 2, np.pi / 2]}))
+        info['method'] = 'Circular detector on Rowland circle'
+        col = 'detpix_x'
+        pg = _trace(det, src)
+        pg = _select(pg, pg.prob > 0.)
+        zeropos = sigma_clipped_stats(_column(pg, col, src.device))[0]
+    else:
+        det = detector
+        info['method'] = 'User defined detector'
+        col = colname
+        zeropos = sigma_clipped_stats(_column(_trace(det, src), col, src.device))[0]
+    for i, order in enumerate(orders):
+        gratingeff = OrderSelector([order])
+        gratings.elem_args['order_selector'] = gratingeff
+        for elem in gratings.elements:
+            elem.order_selector = gratingeff
+        pg = _trace(gratings, src)
+        keep = (_orders_i32(_column(pg, 'order', src.device)) == int(order)) & (pg.prob > 0.)
+        pg = _select(pg, keep)
+        if detector is None:
+            xbest = find_best_detector_position(pg, objective_func=sigma_clipped_std)
+            info['fit_results'].append(xbest)
+            det = FlatDetector(position=np.array([xbest.x, 0, 0]), zoom=1e5)
+        pg = _trace(det, pg)
+        meanpos, medianpos, stdpos = sigma_clipped_stats(_column(pg, col, src.device))
+        fwhm[i] = RES_FWHM * stdpos
+        res[i] = np.abs((meanpos - zeropos) / fwhm[i])
+    return res, fwhm, info

@@ -25,7 +25,7 @@ from .geometry import Geometry
 from .mathutils import (decompose44, translation2aff, mat2aff, zoom2aff,
                         compose, e2h, h2e, ex2vec_fix)
 
-__all__ = ['BaseContainer', 'Sequence', 'Parallel', 'ParallelCalculated']
+__all__ = ['BaseContainer', 'Sequence', 'Parallel', 'ParallelCalculated', 'propagate']
 
 
 class BaseContainer(SimulationSequenceElement):
@@ -190,3 +190,13 @@ class ParallelCalculated(Parallel):
                 return np.tile(spec, (xyzw.shape[0], 1))
             return e2h(h2e(xyzw) - h2e(spec)[None, :], 1)
         raise ValueError(specname + 'must be callable or homogeneous coordinate.')
+
+
+def propagate(photons, d):
+    """Move photons by ``d`` along their rays (``simulator.py:644-667``):
+    ``pos += d * dir``; a photon table or device photons."""
+    if hasattr(photons, 'fcols'):
+        photons.pos += d * photons.dir
+        return photons
+    photons['pos'] = photons['pos'] + d * photons['dir']
+    return photons

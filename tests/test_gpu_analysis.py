@@ -153,3 +153,160 @@ def test_order_stats_more_than_64_orders(gpu):
     ref = _oracle_stats(order, value, prob, orders)
     assert np.array_equal(got['count'], ref['count'])
     assert np.allclose(got['mean'], ref['mean'], rtol=1e-9)
+
+
+def test_find_best_detector_position_known_answers(gpu):
+    """marxs/analysis/tests/test_analysis.py:6-41: rays converging at x = 1
+    (or z = 0.5 with the rotated detector) give the best focus there."""
+    import marxs_amd as mx
+    from marxs_amd import analysis as A
+    rng = np.random.default_rng(2)
+    p = mx.generate_test_photons(100)
+    d = np.asarray(p['dir']).copy()
+    d[:, 1] = rng.random(100)
+    p['dir'] = d
+    pos = np.asarray(p['pos']).copy()
+    pos[:, 1] = 0.5
+    p['pos'] = pos
+    mx.propagate(p, -10)
+    out = A.find_best_detector_position(p, objective_func_args={'colname': 'det_x'})
+    assert np.isclose(out.x, 1.)
+
+    p = mx.generate_test_photons(100)
+    d = np.asarray(p['dir']).copy()
+    d[:, 0] = rng.random(100)
+    d[:, 2] = -1
+    p['dir'] = d
+    pos = np.asarray(p['pos']).copy()
+    pos[:, 2] = 0.5
+    p['pos'] = pos
+    mx.propagate(p, -10)
+    out = A.find_best_detector_position(p, objective_func_args={'colname': 'det_x'},
+                                        orientation=np.array([[0., 1., 0.], [0., 0., 1.],
+                                                              [1., 0., 0.]]))
+    assert np.isclose(out.x, 0.5)
+
+    p = mx.generate_test_photons(100)
+    d = np.asarray(p['dir']).copy()
+    d[:, 2] = rng.random(100)
+    p['dir'] = d
+    pos = np.asarray(p['pos']).copy()
+    pos[:, 1] = 0.5
+    p['pos'] = pos
+    mx.propagate(p, -10)
+    out = A.find_best_detector_position(p, A.mean_width_2d, {})
+    assert np.isclose(out.x, 1.0)
+
+
+def test_resolvingpower_from_photonlist_robust_known_answers(gpu):
+    """marxs/analysis/tests/test_grating.py:47-84 (statistical answers,
+    10 % relative)."""
+    from marxs_amd import analysis as A
+    rng = np.random.default_rng(3)
+    size = 10000
+    orders = [1, 2, 3]
+    ph = {'col1': 2000 + rng.normal(size=size), 'col2': 1000 + rng.normal(size=size),
+          'order': rng.choice(orders, size=size)}
+    res, pos, std = A.resolvingpower_from_photonlist_robust([ph, ph], orders, ['col1', 'col2'], [0, 0])
+    assert np.allclose(res, 1000 / 2.3548, rtol=0.1) and np.allclose(std, 1, rtol=0.1)
+    assert np.allclose(pos, 1000, rtol=0.1)
+    res, pos, std = A.resolvingpower_from_photonlist_robust([ph, ph], orders, ['col1', 'col2'],
+                                                            [0, -1000])
+    assert np.allclose(res, 2000 / 2.3548, rtol=0.1)
+    ph = {'col1': 2000 + rng.normal(size=size), 'col2': 1000 + rng.normal(size=size),
+          'ord': rng.choice([0, 1, 2], size=size)}
+    ph['col1'][ph['ord'] == 0] = 0
+    res, pos, std = A.resolvingpower_from_photonlist_robust([ph, ph], [1, 2], ['col1', 'col2'],
+                                                            [None, 10000], ordercol='ord')
+    assert np.allclose(res, 2000 / 2.3548, rtol=0.1) and np.allclose(pos, 2000, rtol=0.1)
+    with pytest.raises(ValueError, match='zeropositions'):
+        A.resolvingpower_from_photonlist_robust([ph, ph], [1], ['col1'], [0, 0])
+
+
+def test_detected_fraction(gpu):
+    """marxs/analysis/analysis.py:84-107 on the photon list of the
+    reference's effective-area test (test_grating.py:87-96)."""
+    from marxs_amd import analysis as A
+    from marxs_amd.table import PhotonTable
+    p = PhotonTable({'a': np.array([0, 1, 2, 2, 1]), 'probability': [.1, 1., 1., .3, .5]})
+    assert np.allclose(A.detected_fraction(p, [2, 1, 0, -1], col='a'), [1.3 / 5, 1.5 / 5, .1 / 5, 0.])
+
+
+def _pre_grating_photons(parts, n, seed, energy):
+    """Photons after aperture and lens (FP64 GPU trace), monochromatic."""
+    import marxs_amd as mx
+    from tests.helpers import make_inputs, gpu_trace
+    from marxs_amd.tracer import CompiledInstrument
+    ci = CompiledInstrument(mx.Sequence(elements=[parts['aperture'], parts['lens']]))
+    ph, u = make_inputs(n, seed, ci.n_uniform_slots)
+    ph['energy'] = np.full(n, energy)
+    out, st = gpu_trace(ci, ph, u)
+    return {k: out[k] for k in ('pos', 'dir', 'pol', 'energy', 'prob')}
+
+
+def _table(ph):
+    from marxs_amd.table import PhotonTable
+    h = lambda a, w: np.hstack([a, np.full((len(a), 1), w)])
+    return PhotonTable({'pos': h(ph['pos'], 1.), 'dir': h(ph['dir'], 0.),
+                        'polarization': h(ph['pol'], 0.), 'energy': ph['energy'],
+                        'probability': ph['prob']})
+
+
+def test_resolvingpower_per_order_rowland_against_oracle(gpu):
+    """gratings.py:26-130 with a RowlandTorus: every photon sent into one
+    order at a time, traced to a tube on the Rowland circle; the oracle runs
+    the same loop (order selectors with one order, the tube, sigma-clipped
+    statistics) on the same photons.  R and FWHM within 1e-9 relative."""
+    import marxs_amd as mx
+    from marxs_amd import configs, analysis as A
+    from marxs_amd.geometry import Cylinder
+    from tests.helpers import oracle_trace
+    inst, parts = configs.cat_rowland_instrument(200)
+    ph = _pre_grating_photons(parts, 20000, 21, 0.8)
+    orders = np.array([-7, -4, 0])
+    res, fwhm, info = A.resolvingpower_per_order(parts['gratings'], _table(ph), orders,
+                                                 detector=parts['rowland'])
+    assert info['method'] == 'Circular detector on Rowland circle'
+    det = mx.CircularDetector(geometry=Cylinder.from_rowland(
+        parts['rowland'], width=1e5, rotation=np.pi, kwargs={'phi_lim': [-np.pi / 2, np.pi / 2]}))
+    n = len(ph['prob'])
+    z = oracle_trace(mx.Sequence(elements=[det]), ph, np.zeros((0, n)))
+    zeropos = OA.sigma_clipped_stats(z['detpix_x'][z['prob'] > 0])[0]
+    rng = np.random.default_rng(0)
+    for i, o in enumerate(orders):
+        sel = mx.OrderSelector([o])
+        for e in parts['gratings'].elements:
+            e.order_selector = sel
+        g = oracle_trace(mx.Sequence(elements=[parts['gratings']]), ph, rng.random((2, n)))
+        keep = (g['order'] == o) & (g['prob'] > 0)
+        sub = {k: g[k][keep] for k in ('pos', 'dir', 'pol', 'energy', 'prob')}
+        t = oracle_trace(mx.Sequence(elements=[det]), sub, np.zeros((0, keep.sum())))
+        mean, med, std = OA.sigma_clipped_stats(t['detpix_x'])
+        assert np.isclose(fwhm[i], 2.3548 * std, rtol=1e-9, atol=0)
+        assert np.isclose(res[i], abs((mean - zeropos) / (2.3548 * std)), rtol=1e-9, atol=0)
+    assert res[0] > 100 and res[1] > 100 and res[2] < 1
+
+
+def test_resolvingpower_per_order_flat_detectors(gpu):
+    """The two other detector modes of gratings.py:26-130: a user detector
+    (the instrument's own CCD array, column det_x) and None (a flat detector
+    at the best focus of each order)."""
+    from marxs_amd import configs, analysis as A
+    inst, parts = configs.cat_rowland_instrument(200)
+    ph = _pre_grating_photons(parts, 20000, 22, 0.8)
+    orders = np.array([-6, 0])
+    res, fwhm, info = A.resolvingpower_per_order(parts['gratings'], _table(ph), orders, detector=None)
+    assert info['method'] == 'Detector position numerically optimized'
+    assert len(info['fit_results']) == 2
+    # the best focus lies near the Rowland circle, about 6 m behind the gratings
+    assert all(-500 < r.x < 500 for r in info['fit_results'])
+    assert np.all(np.isfinite(fwhm)) and res[0] > 100
+    import marxs_amd as mx
+    res2, fwhm2, info2 = A.resolvingpower_per_order(parts['gratings'], _table(ph), orders,
+                                                    detector=mx.FlatDetector(zoom=1e5),
+                                                    colname='det_x')
+    assert info2['method'] == 'User defined detector'
+    assert np.all(np.isfinite(fwhm2)) and res2[0] > 100
+    # a detector at the focus of order 0: narrower there than the best focus
+    # found for the dispersed order, wider for that order
+    assert fwhm2[1] <= fwhm[1] * 1.01 and fwhm2[0] >= fwhm[0]
