@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 16
+#define MX_ABI_VERSION 17
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -202,7 +202,8 @@ typedef struct MxAdjointArgs {
     const double *ccd_t;       /* [n_ccd][MX_ET] detector frames (centre, e_x, e_y, e_z, ...) */
     const double *seed;        /* dL/dy per photon; NULL: see seed_moments */
     double *y_out;             /* coordinate per photon (NaN if unused); may be NULL */
-    double *grad;              /* [n_facet][MX_AJ_NGRAD], accumulated; NULL: forward only */
+    double *grad;              /* [n_facet][MX_AJ_NGRAD], accumulated; NULL: forward only;
+                                  with MX_AJ_SUMS [n_facet][2 MX_AJ_NGRAD] */
     double *moments;           /* [3] accumulated sum w, sum w (y - y_shift), sum w (y - y_shift)^2;
                                   may be NULL */
     const double *seed_moments; /* [3] moments of the forward pass: with grad and no seed the
@@ -213,9 +214,17 @@ typedef struct MxAdjointArgs {
     int32_t n_ccd;
     int32_t coord;             /* coordinate y: 0 global x, 1 global y, 2 global z */
     int32_t precision;         /* MX_PREC_FP64 or MX_PREC_FP32 (accumulation is FP64) */
-    int32_t lds_accumulate;    /* 1: per-workgroup LDS partial sums (n_facet <= 630) */
-    int32_t reserved;
+    int32_t lds_accumulate;    /* 1: per-workgroup LDS partial sums (n_facet <= 630, with
+                                  MX_AJ_SUMS <= 315) */
+    int32_t flags;             /* MX_AJ_SUMS or 0 */
 } MxAdjointArgs;
+
+/* Single pass for L = centroid + RMS width: with grad and moments and no seed,
+ * grad receives per facet A = sum w dy/dg (13 words) and then
+ * B = sum w (y - y_shift) dy/dg (13 words), and moments the shifted moments;
+ * mx_facet_pose mode 2 combines them into dL/dg = A / W + (B - a A) / (W sigma),
+ * a = sum w (y - y_shift) / W, sigma^2 = sum w (y - y_shift)^2 / W - a^2. */
+#define MX_AJ_SUMS 1
 
 int mx_adjoint(const MxAdjointArgs *args, void *stream);
 
@@ -229,7 +238,8 @@ int mx_adjoint(const MxAdjointArgs *args, void *stream);
  * with G, g the 3x3 part and translation of the 4x4 `gu`. */
 typedef struct MxPoseArgs {
     int32_t n_facet;
-    int32_t mode;              /* 0: theta, period -> geom; 1: grad13 -> grad7 (+ loss) */
+    int32_t mode;              /* 0: theta, period -> geom; 1: grad13 -> grad7 (+ loss);
+                                  2: as 1 from the MX_AJ_SUMS sums of mx_adjoint */
     const double *theta;       /* [n_facet][6]: t_x, t_y, t_z, r_x, r_y, r_z */
     const double *period;      /* [n_facet] (mode 0) */
     const double *gu;          /* [16] row-major 4x4 container pose . uncertainty */
@@ -238,9 +248,10 @@ typedef struct MxPoseArgs {
     const double *zoom;        /* [n_facet][3] */
     const double *extra;       /* [n_facet][3] groove angle, sign mode, transmission (mode 0) */
     double *geom;              /* [n_facet][MX_AJ_GEOM] (mode 0) */
-    const double *grad13;      /* [n_facet][MX_AJ_NGRAD] (mode 1) */
+    const double *grad13;      /* [n_facet][MX_AJ_NGRAD] (mode 1); [n_facet][2 MX_AJ_NGRAD]
+                                  sums A, B (mode 2) */
     double *grad7;             /* [n_facet][7]: d/d theta (6), d/d period (mode 1) */
-    const double *moments;     /* [3] shifted moments (mode 1, for loss) */
+    const double *moments;     /* [3] shifted moments (mode 1 for the loss; mode 2) */
     double *loss;              /* [1] mu + sigma (mode 1), may be NULL */
     double y_shift;
 } MxPoseArgs;

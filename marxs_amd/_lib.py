@@ -16,7 +16,7 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseAr
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 16
+MX_ABI_VERSION = 17
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -50,7 +50,7 @@ class MxAdjointArgs(ctypes.Structure):
                 ('seed_moments', P), ('y_shift', ctypes.c_double),
                 ('n_facet', ctypes.c_int32), ('n_ccd', ctypes.c_int32),
                 ('coord', ctypes.c_int32), ('precision', ctypes.c_int32),
-                ('lds_accumulate', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+                ('lds_accumulate', ctypes.c_int32), ('flags', ctypes.c_int32)]
 
 
 class MxBatch(ctypes.Structure):
@@ -77,6 +77,7 @@ class MxOrderStats(ctypes.Structure):
 
 MX_AJ_GEOM = 16
 MX_AJ_NGRAD = 13
+MX_AJ_SUMS = 1
 MX_MAX_ORDERS = 64
 MX_OS_FILTER = 1
 

@@ -221,9 +221,13 @@ class AdjointProblem:
               'zoom': b['zoom'].contiguous().clone(),
               'extra': torch.stack([self.groove, self.sign_mode, self.transmission], 1).contiguous(),
               'theta': torch.zeros((F, 6), **f64), 'period': torch.zeros(F, **f64),
-              'geom': torch.zeros((F, 16), **f64), 'grad13': torch.zeros((F, 13), **f64),
-              'grad7': torch.zeros((F, 7), **f64), 'moments': torch.zeros(3, **f64),
+              'geom': torch.zeros((F, 16), **f64), 'grad7': torch.zeros((F, 7), **f64),
               'loss': torch.zeros(1, **f64)}
+        # the single pass accumulates per facet the sums A, B (26 words) and the
+        # three moments: one buffer, cleared by one fill
+        pb['accum'] = torch.zeros(F * 26 + 3, **f64)
+        pb['sums'] = pb['accum'][:F * 26].view(F, 26)
+        pb['moments'] = pb['accum'][F * 26:]
         # shift of the moments: the weighted mean coordinate at the base pose
         y = self.coordinates(self.geometry(self.theta0, self.period0))
         ok = ~torch.isnan(y)
@@ -256,32 +260,33 @@ class AdjointProblem:
         a = _lib.MxPoseArgs()
         a.n_facet = self.F
         a.mode = mode
-        for k in ('theta', 'period', 'gu', 't0', 'rpost', 'zoom', 'extra', 'geom', 'grad13',
-                  'grad7', 'moments', 'loss'):
+        for k in ('theta', 'period', 'gu', 't0', 'rpost', 'zoom', 'extra', 'geom', 'grad7',
+                  'moments', 'loss'):
             setattr(a, k, pb[k].data_ptr())
+        a.grad13 = pb['accum'].data_ptr()
         a.y_shift = self.y_shift
         return a
 
     def launch_loss_and_grad(self, precision=32, stream=None):
         """Launch the fused evaluation on the parameters held in
-        ``buffers['theta']``/``['period']`` (asynchronous): pose chain, forward
-        replay with moments, adjoint with seeds from those moments, chain rule
-        back to the poses.  Results: ``buffers['loss']`` and
-        ``buffers['grad7']`` (d/d theta, d/d period)."""
+        ``buffers['theta']``/``['period']`` (asynchronous): one fill of the
+        accumulators, the pose chain, one replay of every photon that
+        accumulates the moments and the per-facet sums A = sum w dy/dg,
+        B = sum w (y - s) dy/dg (``MX_AJ_SUMS``), then the chain rule back to
+        the poses with dL/dg = A / W + (B - a A) / (W sigma) (L = mu + sigma,
+        dL/dy_i = w_i / W (1 + (y_i - mu) / sigma)).  Results:
+        ``buffers['loss']`` and ``buffers['grad7']`` (d/d theta, d/d period)."""
         pb = self._pose_buffers()
         if stream is None:
             stream = torch.cuda.current_stream(self.device).cuda_stream
-        pb['moments'].zero_()
-        pb['grad13'].zero_()
+        pb['accum'].zero_()
         _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(self._pose_args(0)), ctypes.c_void_p(stream)))
-        fwd = self._args(pb['geom'], precision, moments=pb['moments'], src=pb['src'])
-        fwd.y_shift = self.y_shift
-        _launch(fwd, stream)
-        bwd = self._args(pb['geom'], precision, grad=pb['grad13'], src=pb['src'])
-        bwd.seed_moments = pb['moments'].data_ptr()
-        bwd.y_shift = self.y_shift
-        _launch(bwd, stream)
-        _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(self._pose_args(1)), ctypes.c_void_p(stream)))
+        one = self._args(pb['geom'], precision, grad=pb['accum'], moments=pb['moments'], src=pb['src'])
+        one.flags = _lib.MX_AJ_SUMS
+        one.lds_accumulate = int(self.F <= 315)
+        one.y_shift = self.y_shift
+        _launch(one, stream)
+        _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(self._pose_args(2)), ctypes.c_void_p(stream)))
 
     @property
     def buffers(self):

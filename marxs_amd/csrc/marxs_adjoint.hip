@@ -136,10 +136,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 template <int PREC>
 __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
-    extern __shared__ __align__(16) double acc[];     // [n_facet][AJ_NG] when A.grad
+    // [n_facet][AJ_NG] when A.grad; single pass (MX_AJ_SUMS): [n_facet][2 AJ_NG],
+    // the sums A = sum w dy/dg and B = sum w (y - y_shift) dy/dg
+    extern __shared__ __align__(16) double acc[];
+    const bool sums = A.grad && (A.flags & MX_AJ_SUMS);
+    const int gs = sums ? 2 * AJ_NG : AJ_NG;
     const bool lds_acc = A.grad && A.lds_accumulate;
     if (lds_acc) {
-        for (int k = threadIdx.x; k < A.n_facet * AJ_NG; k += AJ_BLOCK) acc[k] = 0.0;
+        for (int k = threadIdx.x; k < A.n_facet * gs; k += AJ_BLOCK) acc[k] = 0.0;
         __syncthreads();
     }
     double m0 = 0, m1 = 0, m2 = 0;
@@ -165,17 +169,20 @@ __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
         const double *C = A.ccd_t + (size_t)c * MX_ET;
         const double cu = A.coord == 0 ? G[0] : (A.coord == 1 ? G[1] : G[2]);
         const double ang = G[13];
-        const double sgd = sin(-ang), cgd = cos(-ang);
+        double sgd, cgd;
+        sincos(-ang, &sgd, &cgd);
         const int sign_mode = (int)G[14];
         const double trans = G[15] != 0.0 ? 1.0 : -1.0;
         const double wave = AJ_E2W / A.energy[i];
         const int m = A.order[i];
-        const bool bw = (A.seed != nullptr || A.seed_moments != nullptr) && A.grad != nullptr;
+        const bool bw = (A.seed != nullptr || A.seed_moments != nullptr || sums) && A.grad != nullptr;
         // seed: explicit dL/dy, or that of L = centroid + RMS width from the
         // forward pass's moments: w/W (1 + (y - mu) / sigma)
         double yb0 = 0.0, yb1 = 0.0, yoff = 0.0;
         if (bw) {
-            if (A.seed) {
+            if (sums) {
+                yb0 = wgt;
+            } else if (A.seed) {
                 yb0 = A.seed[i];
             } else {
                 yb0 = wgt * sm_iw;
@@ -214,11 +221,10 @@ __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
         m1 += wgt * ys;
         m2 += wgt * ys * ys;
         if (bw) {
-            if (lds_acc) {
-                for (int k = 0; k < AJ_NG; ++k) atomicAdd(&acc[f * AJ_NG + k], g[k]);
-            } else {
-                for (int k = 0; k < AJ_NG; ++k) atomicAdd(&A.grad[(size_t)f * AJ_NG + k], g[k]);
-            }
+            double *dst = lds_acc ? acc + (size_t)f * gs : A.grad + (size_t)f * gs;
+            for (int k = 0; k < AJ_NG; ++k) atomicAdd(dst + k, g[k]);
+            if (sums)
+                for (int k = 0; k < AJ_NG; ++k) atomicAdd(dst + AJ_NG + k, ys * g[k]);
         }
     }
     if (A.moments) {
@@ -233,7 +239,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
     }
     if (lds_acc) {
         __syncthreads();
-        for (int k = threadIdx.x; k < A.n_facet * AJ_NG; k += AJ_BLOCK)
+        for (int k = threadIdx.x; k < A.n_facet * gs; k += AJ_BLOCK)
             if (acc[k] != 0.0) atomicAdd(&A.grad[k], acc[k]);
     }
 }
@@ -253,7 +259,14 @@ extern "C" int mx_adjoint(const MxAdjointArgs *a, void *stream) {
         return 1;
     }
     if ((a->seed || a->seed_moments) && !a->grad) { mx_set_error("seed given without grad"); return 1; }
-    const size_t lds = (a->grad && a->lds_accumulate) ? (size_t)a->n_facet * AJ_NG * sizeof(double) : 0;
+    if (a->flags & ~MX_AJ_SUMS) { mx_set_error("unknown flags"); return 1; }
+    const bool sums = (a->flags & MX_AJ_SUMS) != 0;
+    if (sums && (!a->grad || !a->moments || a->seed || a->seed_moments)) {
+        mx_set_error("MX_AJ_SUMS needs grad and moments and no seed");
+        return 1;
+    }
+    const size_t lds = (a->grad && a->lds_accumulate)
+                           ? (size_t)a->n_facet * (sums ? 2 : 1) * AJ_NG * sizeof(double) : 0;
     if (lds > 64 * 1024) { mx_set_error("too many facets for LDS accumulation"); return 1; }
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
@@ -287,10 +300,13 @@ extern "C" int mx_adjoint(const MxAdjointArgs *a, void *stream) {
 // with Re(r) = Rz(r_z) Ry(r_y) Rx(r_x) and G, g the 3x3 part and translation of
 // container pose . container uncertainty.  Forward writes the geometry record
 // of mx_adjoint; backward maps its 13 gradients to the 6 pose parameters and
-// the period, and writes the loss L = mu + sigma from the moments.
+// the period, and writes the loss L = mu + sigma from the moments; mode 2 first
+// forms those 13 gradients from the single-pass sums (MX_AJ_SUMS).
 __device__ __forceinline__ void euler_sxyz(const double *r, double R[9], double dR[3][9]) {
-    const double ca = cos(r[0]), sa = sin(r[0]), cb = cos(r[1]), sb = sin(r[1]);
-    const double cc = cos(r[2]), sc = sin(r[2]);
+    double ca, sa, cb, sb, cc, sc;
+    sincos(r[0], &sa, &ca);
+    sincos(r[1], &sb, &cb);
+    sincos(r[2], &sc, &cc);
     const double X[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
     const double Y[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
     const double Z[9] = {cc, -sc, 0, sc, cc, 0, 0, 0, 1};
@@ -314,7 +330,7 @@ __device__ __forceinline__ void euler_sxyz(const double *r, double R[9], double 
 
 __global__ void mx_facet_pose_kernel(MxPoseArgs A) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (A.mode == 1 && A.loss && f == 0) {
+    if (A.mode != 0 && A.loss && f == 0) {
         const double W = A.moments[0];
         const double a = A.moments[1] / W;
         A.loss[0] = A.y_shift + a + sqrt(A.moments[2] / W - a * a);
@@ -327,7 +343,7 @@ __global__ void mx_facet_pose_kernel(MxPoseArgs A) {
     double Re[9], dRe[3][9];
     euler_sxyz(th + 3, Re, dRe);
     // rot = Re Rpost; columns k
-    double rot[9], v[3][3], nv[3];
+    double rot[9], v[3][3], nv[3], inv[3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             rot[3 * i + j] = Re[3 * i] * Rp[j] + Re[3 * i + 1] * Rp[3 + j] + Re[3 * i + 2] * Rp[6 + j];
@@ -335,30 +351,41 @@ __global__ void mx_facet_pose_kernel(MxPoseArgs A) {
         for (int i = 0; i < 3; ++i)
             v[k][i] = (G[4 * i] * rot[k] + G[4 * i + 1] * rot[3 + k] + G[4 * i + 2] * rot[6 + k]) * z[k];
         nv[k] = sqrt(v[k][0] * v[k][0] + v[k][1] * v[k][1] + v[k][2] * v[k][2]);
+        inv[k] = 1.0 / nv[k];
     }
     if (A.mode == 0) {
         double *g = A.geom + (size_t)f * MX_AJ_GEOM;
         const double t[3] = {th[0] + A.t0[3 * f], th[1] + A.t0[3 * f + 1], th[2] + A.t0[3 * f + 2]};
         for (int i = 0; i < 3; ++i) g[i] = G[4 * i] * t[0] + G[4 * i + 1] * t[1] + G[4 * i + 2] * t[2] + G[4 * i + 3];
         for (int k = 0; k < 3; ++k)
-            for (int i = 0; i < 3; ++i) g[3 + 3 * k + i] = v[k][i] / nv[k];
+            for (int i = 0; i < 3; ++i) g[3 + 3 * k + i] = v[k][i] * inv[k];
         g[12] = A.period[f];
         g[13] = A.extra[3 * f];
         g[14] = A.extra[3 * f + 1];
         g[15] = A.extra[3 * f + 2];
         return;
     }
-    // backward
+    // backward; mode 2: dL/dg from the single-pass sums A, B and the moments
+    double gsum[MX_AJ_NGRAD];
     const double *gg = A.grad13 + (size_t)f * MX_AJ_NGRAD;
+    if (A.mode == 2) {
+        const double iw = 1.0 / A.moments[0];
+        const double a = A.moments[1] * iw;
+        const double iws = iw / sqrt(A.moments[2] * iw - a * a);
+        const double *S = A.grad13 + (size_t)f * 2 * MX_AJ_NGRAD;
+        for (int k = 0; k < MX_AJ_NGRAD; ++k)
+            gsum[k] = S[k] * iw + (S[MX_AJ_NGRAD + k] - a * S[k]) * iws;
+        gg = gsum;
+    }
     double *out = A.grad7 + (size_t)f * 7;
     for (int j = 0; j < 3; ++j) out[j] = G[j] * gg[0] + G[4 + j] * gg[1] + G[8 + j] * gg[2];   // G^T g_c
     double ga[3] = {0, 0, 0};
     for (int k = 0; k < 3; ++k) {
-        const double e[3] = {v[k][0] / nv[k], v[k][1] / nv[k], v[k][2] / nv[k]};
+        const double e[3] = {v[k][0] * inv[k], v[k][1] * inv[k], v[k][2] * inv[k]};
         const double *ge = gg + 3 + 3 * k;
         const double eg = e[0] * ge[0] + e[1] * ge[1] + e[2] * ge[2];
         double gv[3];
-        for (int i = 0; i < 3; ++i) gv[i] = (ge[i] - e[i] * eg) / nv[k];
+        for (int i = 0; i < 3; ++i) gv[i] = (ge[i] - e[i] * eg) * inv[k];
         double h[3];   // G^T gv zoom_k
         for (int j = 0; j < 3; ++j) h[j] = (G[j] * gv[0] + G[4 + j] * gv[1] + G[8 + j] * gv[2]) * z[k];
         for (int a = 0; a < 3; ++a) {
@@ -378,15 +405,19 @@ __global__ void mx_facet_pose_kernel(MxPoseArgs A) {
 extern "C" int mx_facet_pose(const MxPoseArgs *a, void *stream) {
     if (!a) { mx_set_error("args is NULL"); return 1; }
     if (a->n_facet < 0) { mx_set_error("negative facet count"); return 1; }
-    if (a->mode != 0 && a->mode != 1) { mx_set_error("mode must be 0 (forward) or 1 (backward)"); return 1; }
+    if (a->mode < 0 || a->mode > 2) {
+        mx_set_error("mode must be 0 (forward), 1 (backward) or 2 (backward from sums)");
+        return 1;
+    }
     if (a->n_facet == 0) return 0;
     if (!a->theta || !a->gu || !a->t0 || !a->rpost || !a->zoom) {
         mx_set_error("a required pose pointer is NULL");
         return 1;
     }
     if (a->mode == 0 && (!a->geom || !a->period || !a->extra)) { mx_set_error("forward needs geom, period, extra"); return 1; }
-    if (a->mode == 1 && (!a->grad13 || !a->grad7)) { mx_set_error("backward needs grad13 and grad7"); return 1; }
+    if (a->mode != 0 && (!a->grad13 || !a->grad7)) { mx_set_error("backward needs grad13 and grad7"); return 1; }
     if (a->mode == 1 && a->loss && !a->moments) { mx_set_error("loss needs moments"); return 1; }
+    if (a->mode == 2 && !a->moments) { mx_set_error("mode 2 needs moments"); return 1; }
     const int block = 64;
     const int grid = (a->n_facet + block - 1) / block;
     hipLaunchKernelGGL(mx_facet_pose_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, *a);

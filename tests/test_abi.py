@@ -109,3 +109,31 @@ def test_order_stats_argument_errors_need_no_gpu(lib):
     assert lib.lib.mx_order_stats(ctypes.byref(a), None) != 0
     assert lib.lib.mx_order_stats_workspace(-1, 3, 0) == -1
     assert lib.lib.mx_order_stats_workspace(10, 65, 0) == -1
+
+
+def test_adjoint_argument_errors_need_no_gpu(lib):
+    """mx_adjoint / mx_facet_pose reject bad flags and modes on the host
+    (the pointers below are never dereferenced: validation fails first)."""
+    a = lib.MxAdjointArgs()
+    a.n, a.n_facet, a.n_ccd, a.precision = 10, 2, 1, 64
+    fake = 4096
+    for k in range(3):
+        a.pos[k] = a.dir[k] = fake
+    a.energy = a.facet = a.order = a.ccd = a.geom = a.ccd_t = fake
+    a.flags = 2
+    assert lib.lib.mx_adjoint(ctypes.byref(a), None) != 0
+    assert b'flags' in lib.lib.mx_last_error()
+    a.flags = lib.MX_AJ_SUMS
+    assert lib.lib.mx_adjoint(ctypes.byref(a), None) != 0       # no grad, no moments
+    assert b'MX_AJ_SUMS' in lib.lib.mx_last_error()
+    a.grad = fake
+    assert lib.lib.mx_adjoint(ctypes.byref(a), None) != 0       # no moments
+    assert b'MX_AJ_SUMS' in lib.lib.mx_last_error()
+    p = lib.MxPoseArgs()
+    p.n_facet, p.mode = 2, 3
+    assert lib.lib.mx_facet_pose(ctypes.byref(p), None) != 0
+    assert b'mode' in lib.lib.mx_last_error()
+    p.mode = 2
+    p.theta = p.gu = p.t0 = p.rpost = p.zoom = p.grad13 = p.grad7 = fake
+    assert lib.lib.mx_facet_pose(ctypes.byref(p), None) != 0
+    assert b'moments' in lib.lib.mx_last_error()
