@@ -591,8 +591,9 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
 
 
 # LDS image budgets per workgroup (160 KiB of LDS per CU; one workgroup of
-# 1024 (FP32) or 768 (FP64) work-items per CU).
-LDS_BUDGET32 = 150 * 1024
+# 1024 (FP32) or 768 (FP64) work-items per CU; the kernel's own LDS is a few
+# dozen bytes).
+LDS_BUDGET32 = 158 * 1024
 LDS_BUDGET64 = 150 * 1024
 
 
@@ -650,17 +651,22 @@ def _build_lds(tables, budget32=None, budget64=None):
             parts.append((o, b))
             off = _align(off + b.nbytes)
             return o
-        # selectors
+        # selectors (all or none); an interpolated table goes to the FP32
+        # image only, as float32 values with float64 grids
         sel_ok = True
         sel_offs = []
         need = 0
+        interp = (tables.sel_i[:, 8] & 8) != 0
         for k in range(tables.sel_i.shape[0]):
             nr, no = int(tables.sel_i[k, 0]), int(tables.sel_i[k, 1])
-            need += _align(8 * (2 * nr + nr * no)) + _align(4 * no)
+            if interp[k]:
+                nt = int(tables.sel_i[k, 9])
+                need += _align(8 * (nr + nt) + 4 * no * nr * nt) + _align(4 * no)
+            else:
+                need += _align(8 * (2 * nr + nr * no)) + _align(4 * no)
         n_sel = tables.sel_i.shape[0]
         need += _align(4 * SELI * n_sel)
-        if need > budget or np.any(tables.sel_i[:, 8] & 8):
-            # interpolated tables stay in global memory (and with them all)
+        if need > budget or (prec == 64 and np.any(interp)):
             sel_ok = False
         col = 6 if prec == 32 else 7
         tables.sel_i[:, col] = -1
@@ -670,9 +676,17 @@ def _build_lds(tables, budget32=None, budget64=None):
             assert hdr == 0
             for k in range(n_sel):
                 nr, no, oe, oc, ot, oo = (int(x) for x in tables.sel_i[k, :6])
-                blk = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[oc:oc + nr * no],
-                                      tables.sel_d[ot:ot + nr]])
-                o = add(blk)
+                if interp[k]:
+                    # wavelengths, angles (float64), values (float32,
+                    # [order][wave][angle]), then the orders
+                    nt = int(tables.sel_i[k, 9])
+                    grids = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[ot:ot + nt]])
+                    vals = np.asarray(tables.sel_d[oc:oc + no * nr * nt], dtype=np.float32)
+                    o = add(np.concatenate([grids.view(np.uint8), vals.view(np.uint8)]))
+                else:
+                    blk = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[oc:oc + nr * no],
+                                          tables.sel_d[ot:ot + nr]])
+                    o = add(blk)
                 add(tables.sel_o[oo:oo + no].astype(np.int32))
                 sel_offs.append(o)
             for k, o in enumerate(sel_offs):

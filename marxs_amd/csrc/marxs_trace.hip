@@ -337,11 +337,12 @@ __device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, d
     h1 = f * (x - g[i]);
 }
 
-template <typename HP>
-__device__ __forceinline__ bool select_interp(const MxTables &T, HP hd, double energy, double blaze,
-                                              double u, int &order, double &ptot) {
-    const int nw = hd[0], no = hd[1], nt = hd[9];
-    const double *wg = T.sel_d + hd[2], *tg = T.sel_d + hd[4], *dat = T.sel_d + hd[3];
+// wg, tg: wavelength and angle grids (float64); dat: values (float64, or
+// float32 in the FP32 path's LDS copy), [order][wave][angle]; ords: orders.
+template <typename GP, typename DP, typename OP>
+__device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP dat, OP ords, int no,
+                                              double energy, double blaze, double u, int &order,
+                                              double &ptot) {
     int iw, it;
     double hx0, hx1, hy0, hy1;
     // the empty asm keeps the compiler from hoisting this division out of
@@ -354,34 +355,34 @@ __device__ __forceinline__ bool select_interp(const MxTables &T, HP hd, double e
     const size_t stride = (size_t)nw * nt;
     double total = 0.0;
     for (int m = 0; m < no; ++m) {
-        const double *d = dat + (size_t)m * stride;
+        const DP d = dat + (size_t)m * stride;
         double sp = 0.0;
-        sp = sp + d[o00] * hx0 * hy0;
-        sp = sp + d[o01] * hx0 * hy1;
-        sp = sp + d[o10] * hx1 * hy0;
-        sp = sp + d[o11] * hx1 * hy1;
+        sp = sp + (double)d[o00] * hx0 * hy0;
+        sp = sp + (double)d[o01] * hx0 * hy1;
+        sp = sp + (double)d[o10] * hx1 * hy0;
+        sp = sp + (double)d[o11] * hx1 * hy1;
         total = total + sp;
     }
     int j = 0;
     double run = 0.0;
     for (int m = 0; m < no; ++m) {
-        const double *d = dat + (size_t)m * stride;
+        const DP d = dat + (size_t)m * stride;
         double sp = 0.0;
-        sp = sp + d[o00] * hx0 * hy0;
-        sp = sp + d[o01] * hx0 * hy1;
-        sp = sp + d[o10] * hx1 * hy0;
-        sp = sp + d[o11] * hx1 * hy1;
+        sp = sp + (double)d[o00] * hx0 * hy0;
+        sp = sp + (double)d[o01] * hx0 * hy1;
+        sp = sp + (double)d[o10] * hx1 * hy0;
+        sp = sp + (double)d[o11] * hx1 * hy1;
         run = run + sp;
         if (run / total > u) { j = m; break; }
     }
-    order = T.sel_o[hd[5] + j];
+    order = ords[j];
     ptot = total;
     return true;
 }
 
 // Selector sel of the tables.  When the selectors are staged in LDS (all or
-// none; never with an interpolated table), their header rows sit at the start
-// of the LDS image.  blaze_fn() gives the photon's blaze angle (only the
+// none; an interpolated table only in the FP32 image, its values as float32),
+// their header rows sit at the start of the LDS image.  blaze_fn() gives the photon's blaze angle (only the
 // interpolated tables need it).
 template <int PREC, typename BF>
 __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double energy, double u,
@@ -392,11 +393,20 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
         const int nr = hd[0], no = hd[1];
         const int loff = hd[PREC == 64 ? 7 : 6];
         const double *b = (const double *)(lds + loff);
+        if (PREC == 32 && (hd[8] & 8)) {
+            // FP32 image: grids (float64), values (float32), orders
+            const int nt = hd[9];
+            const float *v = (const float *)(b + nr + nt);
+            const int *o = (const int *)(lds + loff + ((8 * (nr + nt) + 4 * no * nr * nt + 15) & ~15));
+            return select_interp(b, nr, b + nr, nt, v, o, no, energy, blaze_fn(), u, order, ptot);
+        }
         const int *o = (const int *)(lds + loff + ((8 * (2 * nr + nr * no) + 15) & ~15));
         return select_order_t(hd, b, b + nr, b + nr + nr * no, o, energy, u, order, ptot);
     }
     const int *hd = T.sel_i + (size_t)sel * MX_SELI;
-    if (hd[8] & 8) return select_interp(T, hd, energy, blaze_fn(), u, order, ptot);
+    if (hd[8] & 8)
+        return select_interp(T.sel_d + hd[2], hd[0], T.sel_d + hd[4], hd[9], T.sel_d + hd[3], T.sel_o + hd[5],
+                             hd[1], energy, blaze_fn(), u, order, ptot);
     return select_order_t(hd, T.sel_d + hd[2], T.sel_d + hd[3], T.sel_d + hd[4], T.sel_o + hd[5],
                           energy, u, order, ptot);
 }
