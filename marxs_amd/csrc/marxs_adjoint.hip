@@ -134,18 +134,15 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// The photon loop of mx_adjoint: geometry records from `geom`, gradients
+// into the LDS accumulator `acc` when it is not null, else straight into
+// A.grad; moments into A.moments.
 template <int PREC>
-__global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
+__device__ __forceinline__ void adjoint_photons(const MxAdjointArgs &A, const double *geom, double *acc) {
     // [n_facet][AJ_NG] when A.grad; single pass (MX_AJ_SUMS): [n_facet][2 AJ_NG],
     // the sums A = sum w dy/dg and B = sum w (y - y_shift) dy/dg
-    extern __shared__ __align__(16) double acc[];
     const bool sums = A.grad && (A.flags & MX_AJ_SUMS);
     const int gs = sums ? 2 * AJ_NG : AJ_NG;
-    const bool lds_acc = A.grad && A.lds_accumulate;
-    if (lds_acc) {
-        for (int k = threadIdx.x; k < A.n_facet * gs; k += AJ_BLOCK) acc[k] = 0.0;
-        __syncthreads();
-    }
     double m0 = 0, m1 = 0, m2 = 0;
     // moments of the forward pass (shifted by y_shift): W, mu, 1/W, 1/sigma
     double sm_iw = 0.0, sm_isig = 0.0, sm_mu = 0.0;
@@ -165,7 +162,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
             if (A.y_out) A.y_out[i] = __builtin_nan("");
             continue;
         }
-        const double *G = A.geom + (size_t)f * MX_AJ_GEOM;
+        const double *G = geom + (size_t)f * MX_AJ_GEOM;
         const double *C = A.ccd_t + (size_t)c * MX_ET;
         const double cu = A.coord == 0 ? G[0] : (A.coord == 1 ? G[1] : G[2]);
         const double ang = G[13];
@@ -221,7 +218,7 @@ __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
         m1 += wgt * ys;
         m2 += wgt * ys * ys;
         if (bw) {
-            double *dst = lds_acc ? acc + (size_t)f * gs : A.grad + (size_t)f * gs;
+            double *dst = acc ? acc + (size_t)f * gs : A.grad + (size_t)f * gs;
             for (int k = 0; k < AJ_NG; ++k) atomicAdd(dst + k, g[k]);
             if (sums)
                 for (int k = 0; k < AJ_NG; ++k) atomicAdd(dst + AJ_NG + k, ys * g[k]);
@@ -237,6 +234,18 @@ __global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
             atomicAdd(A.moments + 2, m2);
         }
     }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(AJ_BLOCK) void mx_adjoint_kernel(MxAdjointArgs A) {
+    extern __shared__ __align__(16) double acc[];
+    const int gs = (A.grad && (A.flags & MX_AJ_SUMS)) ? 2 * AJ_NG : AJ_NG;
+    const bool lds_acc = A.grad && A.lds_accumulate;
+    if (lds_acc) {
+        for (int k = threadIdx.x; k < A.n_facet * gs; k += AJ_BLOCK) acc[k] = 0.0;
+        __syncthreads();
+    }
+    adjoint_photons<PREC>(A, A.geom, lds_acc ? acc : nullptr);
     if (lds_acc) {
         __syncthreads();
         for (int k = threadIdx.x; k < A.n_facet * gs; k += AJ_BLOCK)
@@ -328,56 +337,67 @@ __device__ __forceinline__ void euler_sxyz(const double *r, double R[9], double 
     mul(T1, X, dR[2]);
 }
 
-__global__ void mx_facet_pose_kernel(MxPoseArgs A) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (A.mode != 0 && A.loss && f == 0) {
-        const double W = A.moments[0];
-        const double a = A.moments[1] / W;
-        A.loss[0] = A.y_shift + a + sqrt(A.moments[2] / W - a * a);
-    }
-    if (f >= A.n_facet) return;
+// Facet f's axes v_k = G (Re Rpost)[:, k] zoom_k, 1 / |v_k| and the derivatives
+// of Re with respect to the three angles.
+__device__ __forceinline__ void pose_axes(const MxPoseArgs &A, int f, double v[3][3], double inv[3],
+                                          double dRe[3][9]) {
     const double *th = A.theta + (size_t)f * 6;
     const double *G = A.gu;            // 4x4 row-major
     const double *Rp = A.rpost + (size_t)f * 9;
     const double *z = A.zoom + (size_t)f * 3;
-    double Re[9], dRe[3][9];
+    double Re[9];
     euler_sxyz(th + 3, Re, dRe);
     // rot = Re Rpost; columns k
-    double rot[9], v[3][3], nv[3], inv[3];
+    double rot[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             rot[3 * i + j] = Re[3 * i] * Rp[j] + Re[3 * i + 1] * Rp[3 + j] + Re[3 * i + 2] * Rp[6 + j];
     for (int k = 0; k < 3; ++k) {
         for (int i = 0; i < 3; ++i)
             v[k][i] = (G[4 * i] * rot[k] + G[4 * i + 1] * rot[3 + k] + G[4 * i + 2] * rot[6 + k]) * z[k];
-        nv[k] = sqrt(v[k][0] * v[k][0] + v[k][1] * v[k][1] + v[k][2] * v[k][2]);
-        inv[k] = 1.0 / nv[k];
+        inv[k] = 1.0 / sqrt(v[k][0] * v[k][0] + v[k][1] * v[k][1] + v[k][2] * v[k][2]);
     }
-    if (A.mode == 0) {
-        double *g = A.geom + (size_t)f * MX_AJ_GEOM;
-        const double t[3] = {th[0] + A.t0[3 * f], th[1] + A.t0[3 * f + 1], th[2] + A.t0[3 * f + 2]};
-        for (int i = 0; i < 3; ++i) g[i] = G[4 * i] * t[0] + G[4 * i + 1] * t[1] + G[4 * i + 2] * t[2] + G[4 * i + 3];
-        for (int k = 0; k < 3; ++k)
-            for (int i = 0; i < 3; ++i) g[3 + 3 * k + i] = v[k][i] * inv[k];
-        g[12] = A.period[f];
-        g[13] = A.extra[3 * f];
-        g[14] = A.extra[3 * f + 1];
-        g[15] = A.extra[3 * f + 2];
-        return;
-    }
-    // backward; mode 2: dL/dg from the single-pass sums A, B and the moments
-    double gsum[MX_AJ_NGRAD];
-    const double *gg = A.grad13 + (size_t)f * MX_AJ_NGRAD;
-    if (A.mode == 2) {
-        const double iw = 1.0 / A.moments[0];
-        const double a = A.moments[1] * iw;
-        const double iws = iw / sqrt(A.moments[2] * iw - a * a);
-        const double *S = A.grad13 + (size_t)f * 2 * MX_AJ_NGRAD;
-        for (int k = 0; k < MX_AJ_NGRAD; ++k)
-            gsum[k] = S[k] * iw + (S[MX_AJ_NGRAD + k] - a * S[k]) * iws;
-        gg = gsum;
-    }
-    double *out = A.grad7 + (size_t)f * 7;
+}
+
+// Geometry record of facet f (centre, e_x, e_y, e_z, period, groove angle,
+// sign mode, transmission).
+__device__ __forceinline__ void pose_forward(const MxPoseArgs &A, int f, double *g) {
+    double v[3][3], inv[3], dRe[3][9];
+    pose_axes(A, f, v, inv, dRe);
+    const double *th = A.theta + (size_t)f * 6;
+    const double *G = A.gu;
+    const double t[3] = {th[0] + A.t0[3 * f], th[1] + A.t0[3 * f + 1], th[2] + A.t0[3 * f + 2]};
+    for (int i = 0; i < 3; ++i) g[i] = G[4 * i] * t[0] + G[4 * i + 1] * t[1] + G[4 * i + 2] * t[2] + G[4 * i + 3];
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 3; ++i) g[3 + 3 * k + i] = v[k][i] * inv[k];
+    g[12] = A.period[f];
+    g[13] = A.extra[3 * f];
+    g[14] = A.extra[3 * f + 1];
+    g[15] = A.extra[3 * f + 2];
+}
+
+// dL/dg of facet f from the single-pass sums S (A then B) and the moments.
+__device__ __forceinline__ void sums_to_grad(const double *S, const double mom[3], double gg[MX_AJ_NGRAD]) {
+    const double iw = 1.0 / mom[0];
+    const double a = mom[1] * iw;
+    const double iws = iw / sqrt(mom[2] * iw - a * a);
+    for (int k = 0; k < MX_AJ_NGRAD; ++k)
+        gg[k] = S[k] * iw + (S[MX_AJ_NGRAD + k] - a * S[k]) * iws;
+}
+
+__device__ __forceinline__ double loss_from_moments(const double mom[3], double y_shift) {
+    const double W = mom[0];
+    const double a = mom[1] / W;
+    return y_shift + a + sqrt(mom[2] / W - a * a);
+}
+
+// The 13 gradients gg of facet f mapped to d/d theta (6) and d/d period.
+__device__ __forceinline__ void pose_backward(const MxPoseArgs &A, int f, const double *gg, double *out) {
+    double v[3][3], inv[3], dRe[3][9];
+    pose_axes(A, f, v, inv, dRe);
+    const double *G = A.gu;
+    const double *Rp = A.rpost + (size_t)f * 9;
+    const double *z = A.zoom + (size_t)f * 3;
     for (int j = 0; j < 3; ++j) out[j] = G[j] * gg[0] + G[4 + j] * gg[1] + G[8 + j] * gg[2];   // G^T g_c
     double ga[3] = {0, 0, 0};
     for (int k = 0; k < 3; ++k) {
@@ -400,6 +420,28 @@ __global__ void mx_facet_pose_kernel(MxPoseArgs A) {
     out[4] = ga[1];
     out[5] = ga[2];
     out[6] = gg[12];
+}
+
+__global__ void mx_facet_pose_kernel(MxPoseArgs A) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A.mode != 0 && A.loss && f == 0) {
+        const double mom[3] = {A.moments[0], A.moments[1], A.moments[2]};
+        A.loss[0] = loss_from_moments(mom, A.y_shift);
+    }
+    if (f >= A.n_facet) return;
+    if (A.mode == 0) {
+        pose_forward(A, f, A.geom + (size_t)f * MX_AJ_GEOM);
+        return;
+    }
+    // backward; mode 2: dL/dg from the single-pass sums A, B and the moments
+    double gsum[MX_AJ_NGRAD];
+    const double *gg = A.grad13 + (size_t)f * MX_AJ_NGRAD;
+    if (A.mode == 2) {
+        const double mom[3] = {A.moments[0], A.moments[1], A.moments[2]};
+        sums_to_grad(A.grad13 + (size_t)f * 2 * MX_AJ_NGRAD, mom, gsum);
+        gg = gsum;
+    }
+    pose_backward(A, f, gg, A.grad7 + (size_t)f * 7);
 }
 
 extern "C" int mx_facet_pose(const MxPoseArgs *a, void *stream) {
