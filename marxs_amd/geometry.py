@@ -49,6 +49,13 @@ class FinitePlane(Geometry):
     shape = 'box'
     loc_coos_name = ['y', 'z']
 
+    def get_local_euklid_bases(self, interpos_local):
+        """(e_y, e_z, e_x) of the plane for each of the N local positions,
+        homogeneous (N, 4) arrays (``geometry.py:263-281``)."""
+        n = np.asarray(interpos_local).shape[0]
+        return (np.tile(self['e_y'], (n, 1)), np.tile(self['e_z'], (n, 1)),
+                np.tile(self['e_x'], (n, 1)))
+
 
 class PlaneWithHole(FinitePlane):
     def __init__(self, kwargs):
@@ -85,6 +92,7 @@ class Cylinder(Geometry):
     one within the limits counts."""
     shape = 'surface'
     loc_coos_name = ['phi', 'z']
+    n_points = 50
 
     def __init__(self, kwargs=None):
         kwargs = {} if kwargs is None else kwargs
@@ -97,6 +105,36 @@ class Cylinder(Geometry):
             from .mathutils import decompose44
             return decompose44(self.pos4d)[2][0]
         return super().__getitem__(key)
+
+    def get_local_euklid_bases(self, interpos_local):
+        """Unit vectors along phi, along the tube axis and normal to the
+        surface at local positions (phi, z), homogeneous (N, 4) arrays
+        (``geometry.py:602-627``)."""
+        phi = np.asarray(interpos_local, dtype=float)[:, 0]
+        zeros = np.zeros_like(phi)
+        e_phi = np.einsum('...ij,...j', self.pos4d,
+                          np.array([-np.sin(phi), np.cos(phi), zeros, zeros]).T)
+        e_n = np.einsum('...ij,...j', self.pos4d,
+                        np.array([np.cos(phi), np.sin(phi), zeros, zeros]).T)
+        e_z = np.tile(self['e_z'], (len(phi), 1))
+        unit = lambda v: v / np.linalg.norm(v, axis=1)[:, None]
+        return unit(e_phi), e_z, unit(e_n)
+
+    def parametric_surface(self, phi=None, z=None):
+        """Points of the tube in global homogeneous coordinates, shape
+        (n_phi, n_z, 4) (``geometry.py:567-600``): azimuths ``phi`` (default
+        ``n_points`` across the limits) and axial positions ``z`` in global
+        units (default both rims)."""
+        from .mathutils import decompose44
+        phi = (np.linspace(self.coos_limits[0][0], self.coos_limits[0][1], self.n_points)
+               if phi is None else np.asanyarray(phi))
+        zoom = decompose44(self.pos4d)[2]
+        z = self.coos_limits[1] if z is None else np.asanyarray(z) / zoom[2]
+        if phi.ndim != 1 or z.ndim != 1:
+            raise ValueError('input parameters have 1-dim shape.')
+        phi, z = np.meshgrid(phi, z)
+        coos = np.array([np.cos(phi), np.sin(phi), z, np.ones_like(z)]).T
+        return np.einsum('...ij,...j', self.pos4d, coos)
 
     @classmethod
     def from_rowland(cls, rowland, width, rotation=0., kwargs=None):

@@ -2,6 +2,7 @@
 CPU, against the reference's known answers
 (marxs/math/tests/test_geometry.py:56-222) and closed forms."""
 import numpy as np
+import pytest
 
 import marxs_amd as mx
 from marxs_amd.geometry import Cylinder
@@ -130,3 +131,42 @@ def test_circular_detector_pixels_and_description():
     assert np.allclose(out['detpix_x'][:2], [phi0 * 2 / 0.01, 0.])
     assert np.allclose(out['detpix_y'][:2], [100., 0.])
     assert np.isnan(out['detpix_x'][2]) and np.isnan(out['det_y'][2])
+
+
+def test_tube_parametric_points_are_hit():
+    """marxs/math/tests/test_geometry.py:226-239: points of the parametric
+    surface, approached along +x from their own position, are their own
+    intersection points."""
+    circ = Cylinder({'zoom': [2., 3., 4.]})
+    par = circ.parametric_surface(phi=[-.1, 0., 1.]).reshape((6, 4))
+    dirs = np.tile([1., 0, 0], (6, 1))
+    hit, q, phi, z = _isect(_cyl(zoom=[2., 3., 4.]), dirs, par[:, :3])
+    assert hit.all() and np.allclose(q, par[:, :3])
+    with pytest.raises(ValueError, match='1-dim'):
+        circ.parametric_surface(phi=np.zeros((2, 2)))
+
+
+def test_circular_detector_from_rowland_lies_on_torus():
+    """marxs/optics/tests/test_detector.py:44-54: a very narrow tube from
+    Cylinder.from_rowland lies on the Rowland torus."""
+    from scipy.spatial.transform import Rotation
+    rowland = mx.RowlandTorus(R=6e4, r=5e4, position=[123., 345., -678.],
+                              orientation=Rotation.from_euler('yxz', [1, 2, 3]).as_matrix())
+    det = mx.CircularDetector(geometry=Cylinder.from_rowland(rowland, width=1e-6))
+    points = det.geometry.parametric_surface(np.mgrid[0:2. * np.pi:360j])
+    assert np.max(np.abs(rowland.quartic(points[..., :3].reshape(-1, 3)))) < 1e5
+
+
+@pytest.mark.parametrize('geom', [mx.FlatDetector, mx.CircularDetector])
+def test_local_coordsys(geom):
+    """marxs/math/tests/test_geometry.py:242-258: local bases are orthonormal
+    and right-handed."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(9)
+    rot = Rotation.from_euler('xyz', 2 * np.pi * rng.random(3)).as_matrix()
+    g = geom(orientation=rot, position=rng.random(3)).geometry
+    x, y, z = g.get_local_euklid_bases(rng.random((5, 2)))
+    assert np.allclose(np.einsum('ij,ij->i', x, y), 0) and np.allclose(np.einsum('ij,ij->i', x, z), 0)
+    for v in (x, y, z):
+        assert np.allclose(np.linalg.norm(v, axis=1), 1.)
+    assert np.allclose(np.cross(x[:, :3], y[:, :3]), z[:, :3])
