@@ -120,9 +120,8 @@ def stack_program(e):
             _check_grating(ly)
             prog.append((OP_GRATING, ly))
         elif isinstance(ly, CG.QualityFactor):
-            if not any(op == OP_GRATING for op, _ in prog):
-                raise CompileError('a QualityFactor layer needs a grating layer before it '
-                                   'in the same stack')
+            # after a grating layer of the same stack it uses that layer's
+            # order; else the photon's 'order' column (stage_i[49])
             prog.append((OP_QF, ly))
         elif isinstance(ly, CG.L2Abs):
             prog.append((OP_L2ABS, ly))
@@ -144,6 +143,19 @@ def stack_program(e):
         if sum(op in group for op in ops) > 1:
             raise CompileError('a stack is compiled with at most one layer of each type')
     return prog
+
+
+def qf_reads_order_column(program):
+    """True when an encoded stack program has a quality-factor layer with no
+    grating layer before it (it then reads the photon's 'order' column,
+    ``catgrating.py:158-160``)."""
+    for k in range(program & 15):
+        op = (program >> (4 + 3 * k)) & 7
+        if op == OP_GRATING:
+            return False
+        if op == OP_QF:
+            return True
+    return False
 
 
 def encode_program(prog):
@@ -262,6 +274,7 @@ class CompiledTables:
     lds_image32: np.ndarray = None
     lds_image64: np.ndarray = None
     first_draw_slot: int = -1
+    required_icols: tuple = ()   # integer columns the photons must bring
 
     @property
     def n_elems(self):
@@ -429,6 +442,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
     cell_start, cell_items = [], []
     succ_start, succ_items = [], []
     fcols, icols, icol_kind = [], [], {}
+    written, required = set(), []
     uslot = 0
     e = 0
     for s, st in enumerate(stages):
@@ -550,6 +564,17 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
                 if name not in fcols:
                     fcols.append(name)
                 rec[slot] = fcols.index(name)
+        if st.kind == STAGE_STACK and qf_reads_order_column(st.program):
+            # the quality factor of this stack reads the 'order' column
+            if icol_kind.get('order', 'order') != 'order':
+                raise CompileError('column order used with two meanings')
+            if 'order' not in icols:
+                icols.append('order')
+                icol_kind['order'] = 'order'
+            rec[49] = icols.index('order')
+            if 'order' not in written:
+                required.append('order')
+        written.update(n for n in st.columns.values() if n is not None)
         rec[19] = 0
         rec[20:28] = -1     # LDS offsets, set by _build_lds
         rec[30:32] = -1
@@ -577,7 +602,7 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
                     else np.zeros(1, dtype=np.int32)),
         succ_items=(np.concatenate(succ_items + [np.zeros(1, np.int32)]).astype(np.int32)
                     if succ_items else np.zeros(1, dtype=np.int32)),
-        fcols=fcols, icols=icols, icol_kind=icol_kind,
+        fcols=fcols, icols=icols, icol_kind=icol_kind, required_icols=tuple(required),
         n_uniform_slots=uslot,
         read_pos=stages[0].kind not in (STAGE_APERTURE_CIRCLE, STAGE_APERTURE_RECT),
         elements=leaves)

@@ -738,7 +738,9 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
     d3 pol = PREC == 64 ? R.pol : tod(R.polf);
     const double energy = R.energy;
     const int k0 = MX_STACK_DRAWS * nhit;
-    int m_last = MX_ORDER_NONE;
+    // the quality factor's order: the membrane's, or (no grating layer before
+    // it) the photon's order column stage_i[49]
+    int m_last = si[49] >= 0 ? P.icol[si[49]][i] : MX_ORDER_NONE;
     const int nops = prog & 15;
     for (int k = 0; k < nops; ++k) {
         const int op = (prog >> (4 + 3 * k)) & 7;
@@ -775,8 +777,9 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             if (ptot < 0.0 || ptot > 1.0) count(cnt, MX_ST_PROB_RANGE);
             R.prob = R.prob * ptot;
         } else if (op == MX_OP_QF) {
+            // a photon without an order (NaN in the reference) gets NaN
             const double fm = (double)m_last;
-            const double f = pow_call(sr[9], fm * fm);
+            const double f = m_last == MX_ORDER_NONE ? __builtin_nan("") : pow_call(sr[9], fm * fm);
             if (f < 0.0 || f > 1.0) count(cnt, MX_ST_PROB_RANGE);
             R.prob = R.prob * f;
         } else if (op == MX_OP_L2ABS) {
@@ -879,7 +882,7 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
     const f3 n = ldf3(er + 3);
     const double energy = R.energy;
     const int k0 = MX_STACK_DRAWS * nhit;
-    int m_last = MX_ORDER_NONE;
+    int m_last = si[49] >= 0 ? P.icol[si[49]][i] : MX_ORDER_NONE;
     const int nops = prog & 15;
     for (int k = 0; k < nops; ++k) {
         const int op = (prog >> (4 + 3 * k)) & 7;
@@ -939,7 +942,9 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             R.prob = R.prob * ptot;
         } else if (op == MX_OP_QF) {
             double f;
-            if (m_last > -46341 && m_last < 46341) {
+            if (m_last == MX_ORDER_NONE) {
+                f = __builtin_nan("");
+            } else if (m_last > -46341 && m_last < 46341) {
                 f = ipow(sr[9], m_last * m_last);
             } else {
                 const double fm = (double)m_last;

@@ -270,6 +270,11 @@ class CompiledInstrument:
     def trace_device(self, dp, uniforms=None, precision=64, generator=None, existing=(),
                      grid=None):
         """Trace device photons in place; returns the status counters."""
+        for name in self.tables.required_icols:
+            # e.g. a QualityFactor outside a stack reads photons['order']
+            # (catgrating.py:158-160), which the reference requires
+            if name not in dp.icols:
+                raise KeyError(name)
         if uniforms is None and self.tables.n_uniform_slots:
             uniforms = self.draw_uniforms(dp.n, generator)
         status = self.launch(dp, uniforms, precision, existing=existing, grid=grid)
@@ -367,6 +372,9 @@ def _has_hooks(tree):
 def trace(tree, photons, uniforms=None, precision=64, generator=None, **kw):
     """Trace ``photons`` through ``tree`` on the GPU (what calling an element
     or container does)."""
+    if not flatten(tree) and not (isinstance(tree, BaseContainer) and _has_hooks(tree)):
+        # an empty container returns the photons unchanged (simulator.py:42-49)
+        return photons
     if isinstance(tree, BaseContainer) and _has_hooks(tree):
         if uniforms is not None:
             raise CompileError('explicit uniforms are not supported for containers with hooks')

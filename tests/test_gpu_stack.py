@@ -236,3 +236,24 @@ def test_radial_scatter_sigmas(gpu, sig):
         compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
         if pos[1] == 1. and sig[0] == 0.:
             assert np.all(got['det_x'] == 1.0)
+
+
+def test_quality_factor_on_its_own_reads_the_order_column(gpu):
+    """catgrating.py:158-160: probability * factor ** (order ** 2) from the
+    photon's 'order' column; NaN order gives NaN; without the column the
+    reference's KeyError."""
+    qf = CG.QualityFactor()
+    n = 1000
+    rng = np.random.default_rng(3)
+    p = mx.generate_test_photons(n)
+    order = rng.integers(-6, 7, n).astype(float)
+    order[::97] = np.nan
+    p['order'] = order
+    p['probability'] = np.full(n, 0.5)
+    out = qf(p)
+    want = 0.5 * qf.factor ** (order ** 2)
+    got = np.asarray(out['probability'])
+    assert np.allclose(got, want, rtol=1e-14, equal_nan=True)
+    assert np.isnan(got[::97]).all()
+    with pytest.raises(KeyError):
+        qf(mx.generate_test_photons(5))

@@ -74,7 +74,6 @@ def test_sigma_zero_scatter_has_no_operation():
 
 
 @pytest.mark.parametrize('layers, match', [
-    ([CG.QualityFactor], 'grating layer before'),
     ([RandomGaussianScatter, RandomGaussianScatter], 'at most one layer'),
 ])
 def test_rejected_stacks(layers, match):
@@ -82,6 +81,26 @@ def test_rejected_stacks(layers, match):
     st = mx.FlatStack(elements=layers, keywords=[kw[x] for x in layers])
     with pytest.raises(C.CompileError, match=match):
         C.compile_tree(st)
+
+
+def test_quality_factor_without_grating_reads_the_order_column():
+    """catgrating.py:158-160: on its own (or before any grating layer of its
+    stack) the quality factor uses the photon's 'order' column, which the
+    photons must bring unless an earlier stage writes it."""
+    qf = CG.QualityFactor()
+    t = C.compile_tree(qf)
+    s = [k for k, st in enumerate(t.stages) if st.kind == C.STAGE_STACK][0]
+    assert t.stage_i[s, 49] == t.icols.index('order') and t.required_icols == ('order',)
+    assert C.qf_reads_order_column(t.stages[s].program)
+    # after a grating stage that writes 'order', nothing is required
+    grat = mx.FlatGrating(d=1e-3, order_selector=mx.OrderSelector([1]), position=[1., 0, 0])
+    t = C.compile_tree(mx.Sequence(elements=[grat, CG.QualityFactor()]))
+    assert t.required_icols == () and 'order' in t.icols
+    # inside a CAT stack it follows the membrane: no column read
+    t = C.compile_tree(CG.CATL1L2Stack(order_selector=mx.OrderSelector([2]),
+                                       l1_transmission=(np.array([0.1, 10.]), np.array([.5, .5]))))
+    s = [k for k, st in enumerate(t.stages) if st.kind == C.STAGE_STACK][0]
+    assert t.stage_i[s, 49] == -1 and t.required_icols == ()
 
 
 def test_callable_scatter_and_unknown_layer_rejected():
