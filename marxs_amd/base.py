@@ -5,6 +5,7 @@ its output-column defaults (``249-286``: new float columns start as NaN, id
 columns as int -1) and ``_parse_position_keywords`` (``292-336``: a pose is
 either ``pos4d`` or ``compose(position, orientation, zoom)``).
 """
+from collections import OrderedDict
 import numpy as np
 
 from .mathutils import compose
@@ -34,6 +35,10 @@ class MarxsElement:
         if kwargs:
             raise ValueError('Initialization arguments {0} not understood'
                              .format(', '.join(kwargs.keys())))
+
+    def describe(self):
+        """Description for the photon list's metadata (``base/base.py:89-90``)."""
+        return OrderedDict(element=self.name)
 
 
 class SimulationSequenceElement(MarxsElement):

@@ -393,3 +393,13 @@ def test_successor_lists_cover_every_later_hit(which):
     else:
         # the CCD row does not overlap along the beam
         assert n_all == 0
+
+
+def test_every_element_describes_itself():
+    """marxs/optics/tests/test_all_optics.py:185-189"""
+    from collections import OrderedDict
+    from marxs_amd import catgrating as CG
+    for e in (mx.FlatDetector(), mx.CircularDetector(), mx.ThinLens(focallength=100),
+              mx.Sequence(elements=[]), CG.QualityFactor(), mx.RectangleAperture()):
+        d = e.describe()
+        assert isinstance(d, OrderedDict) and len(d) > 0
