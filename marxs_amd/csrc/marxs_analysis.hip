@@ -40,7 +40,8 @@ __attribute__((visibility("hidden"))) void mx_set_error(const char *msg);
 
 #define MX_OS_WAVE 64          // work-items per counting / scatter workgroup
 #define MX_OS_CLIP 1024        // work-items per clipping workgroup
-#define MX_OS_UNROLL 8         // independent loads in flight per work-item
+#define MX_OS_UNROLL 8         // independent loads in flight per work-item (clipping sums)
+#define MX_OS_LOADS 8          // photons loaded ahead per lane (counting, scatter)
 #define MX_OS_CHUNK 4096       // photons per counting workgroup (at least)
 #define MX_OS_MAX_BLOCKS 16384
 #define MX_OS_SCAN 256         // work-items per scan workgroup (one per order)
@@ -104,11 +105,6 @@ __device__ __forceinline__ int order_index(const int *ords, int no, int o) {
     return k;
 }
 
-__device__ __forceinline__ bool counted(const MxOrderStats &a, int64_t i, double &v) {
-    v = a.value ? a.value[i] : 0.0;
-    if (!(a.flags & MX_OS_FILTER)) return true;
-    return isfinite(v) && a.prob[i] > 0.0;
-}
 
 // Pass 1: per chunk and order, probability sum, counted photons, finite
 // values.  One wave per workgroup; lane l accumulates into slot [k][l].
@@ -131,14 +127,30 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_count_kernel(MxOrderStats a,
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     int64_t b1 = b0 + chunk;
     if (b1 > a.n) b1 = a.n;
-    for (int64_t i = b0 + lane; i < b1; i += MX_OS_WAVE) {
-        const int k = order_index(ords, no, a.order[i]);
-        if (k < 0) continue;
-        sp[k * MX_OS_WAVE + lane] += a.prob[i];
-        double v;
-        if (counted(a, i, v)) {
-            sc[k * MX_OS_WAVE + lane] += 1;
-            if (a.value && isfinite(v)) sv[k * MX_OS_WAVE + lane] += 1;
+    const bool filter = (a.flags & MX_OS_FILTER) != 0;
+    // MX_OS_LOADS photons of the lane are loaded before any is counted (the
+    // loads overlap); they are counted in index order, as one at a time
+    for (int64_t i0 = b0 + lane; i0 < b1; i0 += (int64_t)MX_OS_WAVE * MX_OS_LOADS) {
+        int ord[MX_OS_LOADS];
+        double pr[MX_OS_LOADS], val[MX_OS_LOADS];
+#pragma unroll
+        for (int u = 0; u < MX_OS_LOADS; ++u) {
+            const int64_t i = i0 + (int64_t)u * MX_OS_WAVE;
+            const bool in = i < b1;
+            ord[u] = in ? a.order[i] : INT32_MIN;
+            pr[u] = in ? a.prob[i] : 0.0;
+            val[u] = (in && a.value) ? a.value[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < MX_OS_LOADS; ++u) {
+            if (i0 + (int64_t)u * MX_OS_WAVE >= b1) break;
+            const int k = order_index(ords, no, ord[u]);
+            if (k < 0) continue;
+            sp[k * MX_OS_WAVE + lane] += pr[u];
+            if (!filter || (isfinite(val[u]) && pr[u] > 0.0)) {
+                sc[k * MX_OS_WAVE + lane] += 1;
+                if (a.value && isfinite(val[u])) sv[k * MX_OS_WAVE + lane] += 1;
+            }
         }
     }
     __syncthreads();
@@ -230,10 +242,24 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     int64_t b1 = b0 + chunk;
     if (b1 > a.n) b1 = a.n;
-    for (int64_t i = b0 + lane; i < b1; i += MX_OS_WAVE) {
-        const int k = order_index(ords, no, a.order[i]);
-        double v;
-        if (k >= 0 && counted(a, i, v) && isfinite(v)) pos[k * MX_OS_WAVE + lane] += 1;
+    const bool filter = (a.flags & MX_OS_FILTER) != 0;
+    for (int64_t i0 = b0 + lane; i0 < b1; i0 += (int64_t)MX_OS_WAVE * MX_OS_LOADS) {
+        int ord[MX_OS_LOADS];
+        double pr[MX_OS_LOADS], val[MX_OS_LOADS];
+#pragma unroll
+        for (int u = 0; u < MX_OS_LOADS; ++u) {
+            const int64_t i = i0 + (int64_t)u * MX_OS_WAVE;
+            const bool in = i < b1;
+            ord[u] = in ? a.order[i] : INT32_MIN;
+            pr[u] = (in && filter) ? a.prob[i] : 1.0;
+            val[u] = in ? a.value[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < MX_OS_LOADS; ++u) {
+            if (i0 + (int64_t)u * MX_OS_WAVE >= b1) break;
+            const int k = order_index(ords, no, ord[u]);
+            if (k >= 0 && isfinite(val[u]) && (!filter || pr[u] > 0.0)) pos[k * MX_OS_WAVE + lane] += 1;
+        }
     }
     __syncthreads();
     // exclusive prefix over lanes, plus the chunk and segment offsets
@@ -246,14 +272,27 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
         }
     }
     __syncthreads();
-    for (int64_t i = b0 + lane; i < b1; i += MX_OS_WAVE) {
-        const int k = order_index(ords, no, a.order[i]);
-        double v;
-        if (k >= 0 && counted(a, i, v) && isfinite(v)) {
-            const int64_t p = pos[k * MX_OS_WAVE + lane];
-            pos[k * MX_OS_WAVE + lane] = p + 1;
-            vals[p] = v;
-            ordk[p] = (uint8_t)k;
+    for (int64_t i0 = b0 + lane; i0 < b1; i0 += (int64_t)MX_OS_WAVE * MX_OS_LOADS) {
+        int ord[MX_OS_LOADS];
+        double pr[MX_OS_LOADS], val[MX_OS_LOADS];
+#pragma unroll
+        for (int u = 0; u < MX_OS_LOADS; ++u) {
+            const int64_t i = i0 + (int64_t)u * MX_OS_WAVE;
+            const bool in = i < b1;
+            ord[u] = in ? a.order[i] : INT32_MIN;
+            pr[u] = (in && filter) ? a.prob[i] : 1.0;
+            val[u] = in ? a.value[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < MX_OS_LOADS; ++u) {
+            if (i0 + (int64_t)u * MX_OS_WAVE >= b1) break;
+            const int k = order_index(ords, no, ord[u]);
+            if (k >= 0 && isfinite(val[u]) && (!filter || pr[u] > 0.0)) {
+                const int64_t p = pos[k * MX_OS_WAVE + lane];
+                pos[k * MX_OS_WAVE + lane] = p + 1;
+                vals[p] = val[u];
+                ordk[p] = (uint8_t)k;
+            }
         }
     }
 }
