@@ -105,6 +105,52 @@ __device__ __forceinline__ int order_index(const int *ords, int no, int o) {
     return k;
 }
 
+// Order number -> index through a table of MX_OS_LUT entries in LDS when the
+// orders span fewer values (the usual case), else the linear search.  Built
+// by the one wave of a counting / scatter workgroup.
+#define MX_OS_LUT 256
+struct OrderLut {
+    const int *t;
+    const int *ords;
+    int lo, no;
+    bool ok;
+    __device__ __forceinline__ int operator()(int o) const {
+        if (!ok) return order_index(ords, no, o);
+        const int64_t d = (int64_t)o - lo;
+        const int k = (d >= 0 && d < MX_OS_LUT) ? t[d] : MX_MAX_ORDERS;
+        return k < MX_MAX_ORDERS ? k : -1;
+    }
+};
+
+__device__ __forceinline__ OrderLut build_lut(const MxOrderStats &a, int *ords, int *lut) {
+    const int no = a.n_orders, lane = threadIdx.x;
+    int mn = 0x7fffffff, mx = INT32_MIN;
+    for (int k = lane; k < no; k += MX_OS_WAVE) {
+        const int o = a.orders[k];
+        ords[k] = o;
+        mn = o < mn ? o : mn;
+        mx = o > mx ? o : mx;
+    }
+    for (int w = 32; w > 0; w >>= 1) {
+        const int omn = __shfl_xor(mn, w, MX_OS_WAVE), omx = __shfl_xor(mx, w, MX_OS_WAVE);
+        mn = omn < mn ? omn : mn;
+        mx = omx > mx ? omx : mx;
+    }
+    OrderLut L;
+    L.t = lut;
+    L.ords = ords;
+    L.lo = mn;
+    L.no = no;
+    L.ok = (int64_t)mx - (int64_t)mn < MX_OS_LUT;
+    for (int j = lane; j < MX_OS_LUT; j += MX_OS_WAVE) lut[j] = MX_MAX_ORDERS;
+    __syncthreads();
+    // the first index of an order number, as the linear search finds it
+    if (L.ok)
+        for (int k = lane; k < no; k += MX_OS_WAVE) atomicMin(&lut[(int64_t)a.orders[k] - mn], k);
+    __syncthreads();
+    return L;
+}
+
 
 // Pass 1: per chunk and order, probability sum, counted photons, finite
 // values.  One wave per workgroup; lane l accumulates into slot [k][l].
@@ -117,13 +163,13 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_count_kernel(MxOrderStats a,
     int *sc = (int *)(sp + no * MX_OS_WAVE);            // [no][64]
     int *sv = sc + no * MX_OS_WAVE;                     // [no][64]
     int *ords = sv + no * MX_OS_WAVE;                   // [no]
-    for (int k = lane; k < no; k += MX_OS_WAVE) ords[k] = a.orders[k];
+    __shared__ int lut[MX_OS_LUT];
     for (int k = 0; k < no; ++k) {
         sp[k * MX_OS_WAVE + lane] = 0.0;
         sc[k * MX_OS_WAVE + lane] = 0;
         sv[k * MX_OS_WAVE + lane] = 0;
     }
-    __syncthreads();
+    const OrderLut index_of = build_lut(a, ords, lut);
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     int64_t b1 = b0 + chunk;
     if (b1 > a.n) b1 = a.n;
@@ -144,7 +190,7 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_count_kernel(MxOrderStats a,
 #pragma unroll
         for (int u = 0; u < MX_OS_LOADS; ++u) {
             if (i0 + (int64_t)u * MX_OS_WAVE >= b1) break;
-            const int k = order_index(ords, no, ord[u]);
+            const int k = index_of(ord[u]);
             if (k < 0) continue;
             sp[k * MX_OS_WAVE + lane] += pr[u];
             if (!filter || (isfinite(val[u]) && pr[u] > 0.0)) {
@@ -236,9 +282,9 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
     const int no = a.n_orders, lane = threadIdx.x;
     int64_t *pos = (int64_t *)sm;                       // [no][64]
     int *ords = (int *)(pos + no * MX_OS_WAVE);         // [no]
-    for (int k = lane; k < no; k += MX_OS_WAVE) ords[k] = a.orders[k];
+    __shared__ int lut[MX_OS_LUT];
     for (int k = 0; k < no; ++k) pos[k * MX_OS_WAVE + lane] = 0;
-    __syncthreads();
+    const OrderLut index_of = build_lut(a, ords, lut);
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     int64_t b1 = b0 + chunk;
     if (b1 > a.n) b1 = a.n;
@@ -257,7 +303,7 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
 #pragma unroll
         for (int u = 0; u < MX_OS_LOADS; ++u) {
             if (i0 + (int64_t)u * MX_OS_WAVE >= b1) break;
-            const int k = order_index(ords, no, ord[u]);
+            const int k = index_of(ord[u]);
             if (k >= 0 && isfinite(val[u]) && (!filter || pr[u] > 0.0)) pos[k * MX_OS_WAVE + lane] += 1;
         }
     }
@@ -286,7 +332,7 @@ __global__ __launch_bounds__(MX_OS_WAVE) void mx_os_scatter_kernel(MxOrderStats 
 #pragma unroll
         for (int u = 0; u < MX_OS_LOADS; ++u) {
             if (i0 + (int64_t)u * MX_OS_WAVE >= b1) break;
-            const int k = order_index(ords, no, ord[u]);
+            const int k = index_of(ord[u]);
             if (k >= 0 && isfinite(val[u]) && (!filter || pr[u] > 0.0)) {
                 const int64_t p = pos[k * MX_OS_WAVE + lane];
                 pos[k * MX_OS_WAVE + lane] = p + 1;
