@@ -386,6 +386,36 @@ __device__ __forceinline__ double block_sum(const double *x, int64_t lo, int64_t
     return r;
 }
 
+// First index j in [lo, hi) with pred(x[j]) (pred false, then true, along
+// the sorted range), or hi; by the whole workgroup: each round every
+// work-item tests one of MX_OS_CLIP evenly spaced probes and the range
+// shrinks to the gap before the first true probe (the same answer as a
+// binary search, in log_1024 rounds of parallel loads instead of log_2
+// dependent ones).
+template <typename F>
+__device__ __forceinline__ int64_t block_first(const double *x, int64_t lo, int64_t hi, int *s_t, F pred) {
+    for (;;) {
+        const int64_t n = hi - lo;
+        if (n <= 0) return lo;
+        const int64_t step = n <= MX_OS_CLIP ? 1 : (n + MX_OS_CLIP - 1) / MX_OS_CLIP;
+        const int64_t j = lo + (int64_t)threadIdx.x * step;
+        if (threadIdx.x == 0) *s_t = MX_OS_CLIP;
+        __syncthreads();
+        if (j < hi && pred(x[j])) atomicMin(s_t, (int)threadIdx.x);
+        __syncthreads();
+        const int t = *s_t;
+        __syncthreads();
+        if (step == 1) return t < MX_OS_CLIP ? lo + t : hi;
+        if (t == 0) return lo;
+        if (t < MX_OS_CLIP) {
+            hi = lo + (int64_t)t * step;                 // pred true here
+            lo = lo + (int64_t)(t - 1) * step + 1;       // false at the probe before
+        } else {
+            lo = lo + (n - 1) / step * step + 1;         // after the last probe (false)
+        }
+    }
+}
+
 __device__ __forceinline__ double median_sorted(const double *x, int64_t lo, int64_t hi) {
     const int64_t m = hi - lo;
     if (m & 1) return x[lo + m / 2];
@@ -396,11 +426,15 @@ __device__ __forceinline__ double median_sorted(const double *x, int64_t lo, int
 __global__ __launch_bounds__(MX_OS_CLIP) void mx_os_clip_kernel(MxOrderStats a, const int32_t *seg,
                                                                  const double *x) {
     __shared__ double red[MX_OS_CLIP];
-    __shared__ int64_t s_lo, s_hi;
+    __shared__ int s_t;
     const int k = blockIdx.x, no = a.n_orders;
     int64_t lo = seg[k], hi = seg[no + k];
     const double nan = __builtin_nan("");
     int64_t nchanged = 1;
+    // statistics of the current range once computed (a round that keeps
+    // every value leaves them final)
+    bool have = false;
+    double h_med = nan, h_mean = nan, h_sd = nan;
     for (int it = 0; nchanged != 0 && it < a.maxiters; ++it) {
         const int64_t m = hi - lo;
         if (m == 0) break;
@@ -408,26 +442,26 @@ __global__ __launch_bounds__(MX_OS_CLIP) void mx_os_clip_kernel(MxOrderStats a, 
         const double mean = block_sum(x, lo, hi, red, [](double v) { return v; }) / (double)m;
         const double var = block_sum(x, lo, hi, red, [mean](double v) { const double d = v - mean; return d * d; }) / (double)m;
         const double sd = sqrt(var);
+        h_med = cen;
+        h_mean = mean;
+        h_sd = sd;
         const double vmin = cen - sd * a.sigma;
         const double vmax = cen + sd * a.sigma;
-        if (threadIdx.x == 0) {
-            int64_t l = lo, h = hi;            // first index with x >= vmin
-            while (l < h) { const int64_t mid = (l + h) >> 1; if (x[mid] >= vmin) h = mid; else l = mid + 1; }
-            const int64_t nlo = l;
-            h = hi;                            // first index with x > vmax
-            while (l < h) { const int64_t mid = (l + h) >> 1; if (x[mid] > vmax) h = mid; else l = mid + 1; }
-            s_lo = nlo;
-            s_hi = l;
-        }
-        __syncthreads();
-        nchanged = m - (s_hi - s_lo);
-        lo = s_lo;
-        hi = s_hi;
-        __syncthreads();
+        // first index with x >= vmin, then first index with x > vmax
+        const int64_t nlo = block_first(x, lo, hi, &s_t, [vmin](double v) { return v >= vmin; });
+        const int64_t nhi = block_first(x, nlo, hi, &s_t, [vmax](double v) { return v > vmax; });
+        nchanged = m - (nhi - nlo);
+        have = nchanged == 0;
+        lo = nlo;
+        hi = nhi;
     }
     const int64_t m = hi - lo;
     double mean = nan, med = nan, sd = nan;
-    if (m > 0) {
+    if (have) {
+        med = h_med;
+        mean = h_mean;
+        sd = h_sd;
+    } else if (m > 0) {
         med = median_sorted(x, lo, hi);
         mean = block_sum(x, lo, hi, red, [](double v) { return v; }) / (double)m;
         sd = sqrt(block_sum(x, lo, hi, red, [mean](double v) { const double d = v - mean; return d * d; }) / (double)m);
