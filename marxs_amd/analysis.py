@@ -123,10 +123,14 @@ def order_stats(order, prob, orders, value=None, res_filter=False, sigma=3., max
     if wsb < 0:
         raise ValueError('order statistics: invalid sizes')
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
-    f64 = dict(dtype=torch.float64, device=dev)
-    out = {'prob_sum': torch.empty(no, **f64), 'count': torch.empty(no, dtype=torch.int64, device=dev),
-           'mean': torch.empty(no, **f64), 'median': torch.empty(no, **f64),
-           'std': torch.empty(no, **f64), 'n_kept': torch.empty(no, dtype=torch.int64, device=dev)}
+    # the six result arrays in one buffer, so that one copy brings them back
+    keys = ('prob_sum', 'count', 'mean', 'median', 'std', 'n_kept')
+    ints = ('count', 'n_kept')
+    buf = torch.empty(len(keys) * no, dtype=torch.float64, device=dev)
+    out = {}
+    for j, k in enumerate(keys):
+        part = buf[j * no:(j + 1) * no]
+        out[k] = part.view(torch.int64) if k in ints else part
     a = _lib.MxOrderStats()
     a.n = n
     a.order = o.data_ptr()
@@ -144,7 +148,11 @@ def order_stats(order, prob, orders, value=None, res_filter=False, sigma=3., max
     if stream is None:
         stream = torch.cuda.current_stream(dev).cuda_stream
     _lib.check(_lib.lib.mx_order_stats(a, stream))
-    res = {k: t.cpu().numpy() for k, t in out.items()}
+    host = buf.cpu().numpy()
+    res = {}
+    for j, k in enumerate(keys):
+        part = host[j * no:(j + 1) * no]
+        res[k] = part.view(np.int64) if k in ints else part
     if v is None:
         for k in ('mean', 'median', 'std', 'n_kept'):
             res.pop(k)
@@ -243,18 +251,45 @@ class CaptureResAeff:
         if n_photons is None:
             n_photons = order.numel()
         sel = self.aeff_filter(photons)
-        r = order_stats(order, prob, self.orders, select=sel)
-        aeff = r['prob_sum'] / n_photons * self.A_geom
         rsel = self.res_filter(photons)
-        try:
-            res, pos, std = _resolving_power(order, value, self.orders, self.zeropos,
-                                             rsel is None, prob, select=rsel)
-        except AnalysisError:
-            res = np.nan * np.ones(len(self.orders))
+        if sel is None and rsel is None:
+            # one pass: the per-order probability sums of the resolving-power
+            # pass run over all photons of the order, as the area needs
+            res, aeff = self._both(order, prob, value, n_photons)
+        else:
+            r = order_stats(order, prob, self.orders, select=sel)
+            aeff = r['prob_sum'] / n_photons * self.A_geom
+            try:
+                res, pos, std = _resolving_power(order, value, self.orders, self.zeropos,
+                                                 rsel is None, prob, select=rsel)
+            except AnalysisError:
+                res = np.nan * np.ones(len(self.orders))
         disporders = self.orders != 0
         avggratres, aeffgrat = average_R_Aeff(res[disporders], aeff[disporders])
+        aeff = np.asarray(aeff)
         aeff0 = np.sum(aeff[~disporders])
         return {'Aeff0': aeff0, 'Aeffgrat': aeffgrat, 'Aeff': aeff, 'Rgrat': avggratres, 'R': res}
+
+
+    def _both(self, order, prob, value, n_photons):
+        """Areas and resolving powers from one mx_order_stats call (default
+        filters)."""
+        orders = np.asarray(self.orders)
+        want = list(dict.fromkeys([int(x) for x in orders] + ([0] if self.zeropos is None else [])))
+        r = order_stats(order, prob, want, value=value, res_filter=True)
+        idx = {o: i for i, o in enumerate(want)}
+        aeff = np.array([r['prob_sum'][idx[int(o)]] for o in orders]) / n_photons * self.A_geom
+        zeropos = self.zeropos
+        if zeropos is None:
+            if r['count'][idx[0]] < 20:
+                return np.nan * np.ones(len(orders)), aeff
+            zeropos = r['mean'][idx[0]]
+        ok = np.array([r['count'][idx[int(o)]] > 20 for o in orders])
+        pos = np.where(ok, [r['mean'][idx[int(o)]] for o in orders], np.nan)
+        std = np.where(ok, [r['std'][idx[int(o)]] for o in orders], np.nan)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            res = np.abs(pos - zeropos) / (std * RES_FWHM)
+        return res, aeff
 
 
 class CaptureResAeff_CCDgaps(CaptureResAeff):
