@@ -57,7 +57,17 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace layout (bytes, each part 256-aligned).
 struct OsLayout {
-    size_t cnt_b, nval_b, psum_b, off_b, tot, seg, vals, sorted, ord, ord2, temp, temp_bytes, total;
+    size_t cnt_b, nval_b, psum_b, off_b, tot, seg, vals, sorted, ord, ord2, clip, part0, part1, temp,
+        temp_bytes, total;
+};
+
+#define MX_OS_CW 16            // workgroups per order in the clipping sums
+
+// Clipping state of one order between the launches of the rounds.
+struct ClipState {
+    int64_t lo, hi;            // current range of the sorted segment
+    double med, mean, sd;      // statistics of that range when `have`
+    int32_t done, have;        // no further round; statistics final
 };
 
 // temporary storage of the two radix sorts (by value with the order index
@@ -89,6 +99,9 @@ static int os_layout(int64_t n, int no, bool values, OsLayout &L) {
     L.sorted = o; o += values ? align256(sizeof(double) * (n > 0 ? n : 1)) : 0;
     L.ord = o;    o += values ? align256(n > 0 ? n : 1) : 0;
     L.ord2 = o;   o += values ? align256(n > 0 ? n : 1) : 0;
+    L.clip = o;   o += values ? align256(sizeof(ClipState) * no) : 0;
+    L.part0 = o;  o += values ? align256(sizeof(double) * no * MX_OS_CW) : 0;
+    L.part1 = o;  o += values ? align256(sizeof(double) * no * MX_OS_CW) : 0;
     L.temp = o;
     L.temp_bytes = 0;
     if (values && os_sort_temp_bytes(n, L.temp_bytes)) return 1;
@@ -422,56 +435,108 @@ __device__ __forceinline__ double median_sorted(const double *x, int64_t lo, int
     return (x[lo + m / 2 - 1] + x[lo + m / 2]) / 2;
 }
 
-// Pass 5: sigma clipping of each sorted segment, one workgroup per order.
-__global__ __launch_bounds__(MX_OS_CLIP) void mx_os_clip_kernel(MxOrderStats a, const int32_t *seg,
-                                                                 const double *x) {
+// Pass 5: sigma clipping of each sorted segment.  Every clipping round keeps
+// a contiguous range, so a round is: the sum of the values over the range,
+// then the sum of squared deviations from their mean (each by MX_OS_CW
+// workgroups per order over consecutive parts of the range, partial sums
+// added in part order), then, one workgroup per order, the median, the new
+// bounds and the state for the next round.  Rounds are launched up to
+// maxiters times; orders that are done return at once.
+__global__ void mx_os_clip_init_kernel(int no, const int32_t *seg, ClipState *cs) {
+    const int k = threadIdx.x;
+    if (k >= no) return;
+    ClipState c;
+    c.lo = seg[k];
+    c.hi = seg[no + k];
+    c.med = c.mean = c.sd = __builtin_nan("");
+    c.done = c.hi == c.lo;
+    c.have = 0;
+    cs[k] = c;
+}
+
+// pass 0: partial sums of x; pass 1: partial sums of (x - mean)^2, mean from
+// pass 0's partials.  final: the passes after the last round, for orders
+// whose statistics are not yet final.
+__global__ __launch_bounds__(MX_OS_CLIP) void mx_os_clip_sum_kernel(const ClipState *cs, const double *x,
+                                                                     double *p0, double *p1, int pass,
+                                                                     int final) {
     __shared__ double red[MX_OS_CLIP];
+    const int k = blockIdx.x / MX_OS_CW, w = blockIdx.x % MX_OS_CW;
+    const ClipState c = cs[k];
+    if (final ? (c.have || c.hi == c.lo) : c.done) return;
+    const int64_t m = c.hi - c.lo;
+    const int64_t b = c.lo + m * w / MX_OS_CW, e = c.lo + m * (w + 1) / MX_OS_CW;
+    double s;
+    if (pass == 0) {
+        s = block_sum(x, b, e, red, [](double v) { return v; });
+    } else {
+        double t = 0.0;
+        for (int q = 0; q < MX_OS_CW; ++q) t += p0[k * MX_OS_CW + q];
+        const double mean = t / (double)m;
+        s = block_sum(x, b, e, red, [mean](double v) { const double d = v - mean; return d * d; });
+    }
+    if (threadIdx.x == 0) (pass == 0 ? p0 : p1)[k * MX_OS_CW + w] = s;
+}
+
+__device__ __forceinline__ void clip_moments(const double *p0, const double *p1, int k, int64_t m,
+                                             double &mean, double &sd) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int q = 0; q < MX_OS_CW; ++q) t0 += p0[k * MX_OS_CW + q];
+    for (int q = 0; q < MX_OS_CW; ++q) t1 += p1[k * MX_OS_CW + q];
+    mean = t0 / (double)m;
+    sd = sqrt(t1 / (double)m);
+}
+
+__global__ __launch_bounds__(MX_OS_CLIP) void mx_os_clip_bounds_kernel(MxOrderStats a, ClipState *cs,
+                                                                        const double *x, const double *p0,
+                                                                        const double *p1) {
     __shared__ int s_t;
-    const int k = blockIdx.x, no = a.n_orders;
-    int64_t lo = seg[k], hi = seg[no + k];
-    const double nan = __builtin_nan("");
-    int64_t nchanged = 1;
-    // statistics of the current range once computed (a round that keeps
-    // every value leaves them final)
-    bool have = false;
-    double h_med = nan, h_mean = nan, h_sd = nan;
-    for (int it = 0; nchanged != 0 && it < a.maxiters; ++it) {
-        const int64_t m = hi - lo;
-        if (m == 0) break;
-        const double cen = median_sorted(x, lo, hi);
-        const double mean = block_sum(x, lo, hi, red, [](double v) { return v; }) / (double)m;
-        const double var = block_sum(x, lo, hi, red, [mean](double v) { const double d = v - mean; return d * d; }) / (double)m;
-        const double sd = sqrt(var);
-        h_med = cen;
-        h_mean = mean;
-        h_sd = sd;
-        const double vmin = cen - sd * a.sigma;
-        const double vmax = cen + sd * a.sigma;
-        // first index with x >= vmin, then first index with x > vmax
-        const int64_t nlo = block_first(x, lo, hi, &s_t, [vmin](double v) { return v >= vmin; });
-        const int64_t nhi = block_first(x, nlo, hi, &s_t, [vmax](double v) { return v > vmax; });
-        nchanged = m - (nhi - nlo);
-        have = nchanged == 0;
-        lo = nlo;
-        hi = nhi;
-    }
-    const int64_t m = hi - lo;
-    double mean = nan, med = nan, sd = nan;
-    if (have) {
-        med = h_med;
-        mean = h_mean;
-        sd = h_sd;
-    } else if (m > 0) {
-        med = median_sorted(x, lo, hi);
-        mean = block_sum(x, lo, hi, red, [](double v) { return v; }) / (double)m;
-        sd = sqrt(block_sum(x, lo, hi, red, [mean](double v) { const double d = v - mean; return d * d; }) / (double)m);
-    }
+    const int k = blockIdx.x;
+    const ClipState c = cs[k];
+    if (c.done) return;
+    const int64_t m = c.hi - c.lo;
+    double mean, sd;
+    clip_moments(p0, p1, k, m, mean, sd);
+    const double cen = median_sorted(x, c.lo, c.hi);
+    const double vmin = cen - sd * a.sigma;
+    const double vmax = cen + sd * a.sigma;
+    // first index with x >= vmin, then first index with x > vmax
+    const int64_t nlo = block_first(x, c.lo, c.hi, &s_t, [vmin](double v) { return v >= vmin; });
+    const int64_t nhi = block_first(x, nlo, c.hi, &s_t, [vmax](double v) { return v > vmax; });
     if (threadIdx.x == 0) {
-        a.mean[k] = mean;
-        a.median[k] = med;
-        a.std[k] = sd;
-        a.n_kept[k] = m;
+        const int64_t nchanged = m - (nhi - nlo);
+        ClipState d;
+        d.lo = nlo;
+        d.hi = nhi;
+        d.med = cen;
+        d.mean = mean;
+        d.sd = sd;
+        d.have = nchanged == 0;
+        d.done = nchanged == 0 || nhi == nlo;
+        cs[k] = d;
     }
+}
+
+__global__ void mx_os_clip_final_kernel(MxOrderStats a, const ClipState *cs, const double *x, const double *p0,
+                                        const double *p1) {
+    const int k = threadIdx.x;
+    if (k >= a.n_orders) return;
+    const ClipState c = cs[k];
+    const int64_t m = c.hi - c.lo;
+    const double nan = __builtin_nan("");
+    double mean = nan, med = nan, sd = nan;
+    if (c.have) {
+        med = c.med;
+        mean = c.mean;
+        sd = c.sd;
+    } else if (m > 0) {
+        med = median_sorted(x, c.lo, c.hi);
+        clip_moments(p0, p1, k, m, mean, sd);
+    }
+    a.mean[k] = mean;
+    a.median[k] = med;
+    a.std[k] = sd;
+    a.n_kept[k] = m;
 }
 
 extern "C" int64_t mx_order_stats_workspace(int64_t n, int32_t n_orders, int32_t with_values) {
@@ -544,9 +609,25 @@ extern "C" int mx_order_stats(const MxOrderStats *a, void *stream) {
             mx_set_error("order stats: sort by order failed");
             return 2;
         }
-        sorted = vals;
-        hipLaunchKernelGGL(mx_os_clip_kernel, dim3((unsigned)no), dim3(MX_OS_CLIP), 0, st, *a, seg,
-                           (const double *)sorted);
+        const double *xs = vals;
+        ClipState *cs = (ClipState *)(ws + L.clip);
+        double *p0 = (double *)(ws + L.part0), *p1 = (double *)(ws + L.part1);
+        hipLaunchKernelGGL(mx_os_clip_init_kernel, dim3(1), dim3(MX_OS_WAVE), 0, st, no, (const int32_t *)seg, cs);
+        const dim3 gsum((unsigned)(no * MX_OS_CW));
+        for (int it = 0; it < a->maxiters; ++it) {
+            hipLaunchKernelGGL(mx_os_clip_sum_kernel, gsum, dim3(MX_OS_CLIP), 0, st, (const ClipState *)cs, xs, p0,
+                               p1, 0, 0);
+            hipLaunchKernelGGL(mx_os_clip_sum_kernel, gsum, dim3(MX_OS_CLIP), 0, st, (const ClipState *)cs, xs, p0,
+                               p1, 1, 0);
+            hipLaunchKernelGGL(mx_os_clip_bounds_kernel, dim3((unsigned)no), dim3(MX_OS_CLIP), 0, st, *a, cs, xs,
+                               (const double *)p0, (const double *)p1);
+        }
+        hipLaunchKernelGGL(mx_os_clip_sum_kernel, gsum, dim3(MX_OS_CLIP), 0, st, (const ClipState *)cs, xs, p0, p1,
+                           0, 1);
+        hipLaunchKernelGGL(mx_os_clip_sum_kernel, gsum, dim3(MX_OS_CLIP), 0, st, (const ClipState *)cs, xs, p0, p1,
+                           1, 1);
+        hipLaunchKernelGGL(mx_os_clip_final_kernel, dim3(1), dim3(MX_OS_WAVE), 0, st, *a, (const ClipState *)cs,
+                           xs, (const double *)p0, (const double *)p1);
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
