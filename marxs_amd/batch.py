@@ -106,6 +106,11 @@ class BatchedTrace:
         for ci in self.instruments:
             if ci.device != self.device:
                 raise ValueError('all variants must live on one device')
+            if ci.tables.required_icols:
+                # e.g. a QualityFactor outside a stack reads the photons'
+                # order column, which the batched outputs do not carry in
+                raise ValueError('batched traces start without photon columns; a variant reads '
+                                 + ', '.join(ci.tables.required_icols))
         self.n_variants = len(self.instruments)
         self.outputs = [outputs_like(src) for src in self.inputs]
         self._photon_structs = [ci.photons_struct(dp, u, src) for ci, dp, u, src in
