@@ -703,10 +703,17 @@ def _build_lds(tables, budget32=None, budget64=None):
                 nr, no, oe, oc, ot, oo = (int(x) for x in tables.sel_i[k, :6])
                 if interp[k]:
                     # wavelengths, angles (float64), values (float32,
-                    # [order][wave][angle]), then the orders
+                    # [order][wave][angle]), then the orders.  With no
+                    # negative value the values are stored summed over the
+                    # orders up to each one (sel_i[k, 10] = 1): the
+                    # interpolated cumulative sum at order j is then one
+                    # bilinear value (interpolation is linear in the table)
                     nt = int(tables.sel_i[k, 9])
                     grids = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[ot:ot + nt]])
-                    vals = np.asarray(tables.sel_d[oc:oc + no * nr * nt], dtype=np.float32)
+                    data = np.asarray(tables.sel_d[oc:oc + no * nr * nt], dtype=float).reshape(no, nr, nt)
+                    cum = bool(np.all(data >= 0))
+                    tables.sel_i[k, 10] = int(cum)
+                    vals = (np.cumsum(data, axis=0) if cum else data).astype(np.float32).ravel()
                     o = add(np.concatenate([grids.view(np.uint8), vals.view(np.uint8)]))
                 else:
                     blk = np.concatenate([tables.sel_d[oe:oe + nr], tables.sel_d[oc:oc + nr * no],

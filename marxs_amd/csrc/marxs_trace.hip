@@ -380,6 +380,44 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
     return true;
 }
 
+// The same from a table summed over the orders (cum[j] = sum of the values of
+// orders 0..j; all values >= 0): the interpolated cumulative sum at order j
+// is one bilinear value, non-decreasing in j, so the first order whose
+// cumulative sum over the total exceeds u is found by bisection (the FP32
+// path's LDS copy; the float64 path keeps the per-order sums above).
+template <typename GP, typename DP, typename OP>
+__device__ __forceinline__ bool select_interp_cum(GP wg, int nw, GP tg, int nt, DP cum, OP ords, int no,
+                                                  double energy, double blaze, double u, int &order,
+                                                  double &ptot) {
+    int iw, it;
+    double hx0, hx1, hy0, hy1;
+    double e = energy;
+    asm volatile("" : "+v"(e));
+    hat1(wg, nw, MX_KEV_NM / e, iw, hx0, hx1);
+    hat1(tg, nt, blaze, it, hy0, hy1);
+    const size_t o00 = (size_t)iw * nt + it, o01 = o00 + 1, o10 = o00 + nt, o11 = o10 + 1;
+    const size_t stride = (size_t)nw * nt;
+    auto at = [&](int j) {
+        const DP d = cum + (size_t)j * stride;
+        double sp = 0.0;
+        sp = sp + (double)d[o00] * hx0 * hy0;
+        sp = sp + (double)d[o01] * hx0 * hy1;
+        sp = sp + (double)d[o10] * hx1 * hy0;
+        sp = sp + (double)d[o11] * hx1 * hy1;
+        return sp;
+    };
+    const double total = at(no - 1);
+    int lo = 0, hi = no;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (at(mid) / total > u) hi = mid;
+        else lo = mid + 1;
+    }
+    order = ords[lo < no ? lo : 0];
+    ptot = total;
+    return true;
+}
+
 // Selector sel of the tables.  When the selectors are staged in LDS (all or
 // none; an interpolated table only in the FP32 image, its values as float32),
 // their header rows sit at the start of the LDS image.  blaze_fn() gives the photon's blaze angle (only the
@@ -398,6 +436,7 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
             const int nt = hd[9];
             const float *v = (const float *)(b + nr + nt);
             const int *o = (const int *)(lds + loff + ((8 * (nr + nt) + 4 * no * nr * nt + 15) & ~15));
+            if (hd[10] == 1) return select_interp_cum(b, nr, b + nr, nt, v, o, no, energy, blaze_fn(), u, order, ptot);
             return select_interp(b, nr, b + nr, nt, v, o, no, energy, blaze_fn(), u, order, ptot);
         }
         const int *o = (const int *)(lds + loff + ((8 * (2 * nr + nr * no) + 15) & ~15));
