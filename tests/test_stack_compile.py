@@ -205,3 +205,57 @@ def test_transmission_bucket_index_brackets_the_search():
         want = np.searchsorted(x, e, side='left')
         assert np.all((lo <= want) & (want <= hi))
     assert np.array_equal(C._bucket_index(np.arange(7.)), [0., 0.])
+
+
+def _interp_block(tables, k):
+    """Grids, values and orders of interpolated selector k in the FP32 LDS
+    image."""
+    nr, no, nt = (int(tables.sel_i[k, c]) for c in (0, 1, 9))
+    off = int(tables.sel_i[k, 6])
+    img = tables.lds_image32
+    g = img[off:off + 8 * (nr + nt)].view(np.float64)
+    v = img[off + 8 * (nr + nt):off + 8 * (nr + nt) + 4 * no * nr * nt].view(np.float32)
+    o0 = off + ((8 * (nr + nt) + 4 * no * nr * nt + 15) & ~15)
+    return g[:nr], g[nr:], v.reshape(no, nr, nt), img[o0:o0 + 4 * no].view(np.int32)
+
+
+def test_interpolated_table_lds_copy_is_summed_over_orders():
+    """With no negative entry the FP32 LDS copy of an interpolated table holds
+    the values summed over the orders (flag sel_i[k, 10] = 1), and the
+    bilinear value of that copy at order j equals the sum over orders <= j of
+    the per-order bilinear values (interpolation is linear in the table), so
+    the first order whose cumulative sum over the total exceeds a draw is the
+    same; a table with a negative entry keeps the per-order values (flag 0)."""
+    eff = configs.blaze_interp_efficiency()
+    inst, parts = configs.cat_rowland_instrument(20, efficiency=eff)
+    t = C.compile_tree(inst)
+    k = int(np.nonzero(t.sel_i[:, 8] & 8)[0][0])
+    assert t.sel_i[k, 6] >= 0 and t.sel_i[k, 10] == 1
+    wg, tg, v, o = _interp_block(t, k)
+    data = np.asarray(eff.data, dtype=float)
+    assert np.array_equal(o, np.asarray(eff.orders))
+    assert np.array_equal(v, np.cumsum(data, axis=0).astype(np.float32))
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(wg[0], wg[-1], 500), rng.uniform(tg[0], tg[-1], 500)
+    i = np.clip(np.searchsorted(wg, x, side='right') - 1, 0, len(wg) - 2)
+    j = np.clip(np.searchsorted(tg, y, side='right') - 1, 0, len(tg) - 2)
+    hx = (x - wg[i]) / (wg[i + 1] - wg[i])
+    hy = (y - tg[j]) / (tg[j + 1] - tg[j])
+
+    def bil(d):
+        return (d[:, i, j] * (1 - hx) * (1 - hy) + d[:, i, j + 1] * (1 - hx) * hy
+                + d[:, i + 1, j] * hx * (1 - hy) + d[:, i + 1, j + 1] * hx * hy)
+    per = np.cumsum(bil(data), axis=0)
+    cum = bil(np.cumsum(data, axis=0))
+    assert np.allclose(per, cum, rtol=1e-12, atol=0)
+    u = rng.uniform(0, 1, 500)
+    first = lambda c: np.argmax(c / c[-1] > u, axis=0)
+    assert np.mean(first(per) == first(cum)) > 0.99
+    neg = data.copy()
+    neg[0, 0, 0] = -1e-3
+    tneg = mx.InterpolateEfficiencyTable.from_arrays(eff.wave, eff.theta, eff.orders, neg)
+    inst, parts = configs.cat_rowland_instrument(20, efficiency=tneg)
+    t = C.compile_tree(inst)
+    k = int(np.nonzero(t.sel_i[:, 8] & 8)[0][0])
+    assert t.sel_i[k, 10] == 0
+    assert np.array_equal(_interp_block(t, k)[2], neg.astype(np.float32))
