@@ -788,6 +788,8 @@ def _interp_row(sel, sel_d, sel_o):
     nw, nt, no = len(wave), len(theta), len(orders)
     if nw < 2 or nt < 2 or np.any(np.diff(wave) <= 0) or np.any(np.diff(theta) <= 0):
         raise CompileError('interpolation grids need >= 2 strictly increasing points')
+    if no < 1:
+        raise CompileError('interpolated efficiency table lists no orders')
     if np.any(np.abs(np.asarray(orders)) >= 2 ** 31 - 1):
         raise CompileError('order numbers must fit in int32')
     row = [nw, no, len(sel_d), 0, 0, len(sel_o), -1, -1, 8, nt, 0, 0]
@@ -810,6 +812,8 @@ def _selector_row(sel, sel_index, sel_rows, sel_d, sel_o):
         return sel_index[key]
     energy, cdf, tot, orders = sel.table()
     nr, no = cdf.shape
+    if nr < 1 or no < 1:
+        raise CompileError('order selector table lists no orders or no rows')
     if energy is None:
         energy = np.zeros(1)
     if np.any(tot > 1) or np.any(tot < 0):

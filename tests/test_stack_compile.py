@@ -259,3 +259,16 @@ def test_interpolated_table_lds_copy_is_summed_over_orders():
     k = int(np.nonzero(t.sel_i[:, 8] & 8)[0][0])
     assert t.sel_i[k, 10] == 0
     assert np.array_equal(_interp_block(t, k)[2], neg.astype(np.float32))
+
+
+def test_selector_tables_without_orders_rejected():
+    """A tabulated or interpolated efficiency table with no orders is a
+    compile error (the kernels read the first order of every table)."""
+    e = np.array([0.3, 1.0, 2.0])
+    empty = mx.EfficiencyFile.from_arrays(e, np.zeros((3, 0)), np.zeros(0, dtype=int))
+    w, t = np.array([1., 2., 3.]), np.array([0., 0.01])
+    empty_i = mx.InterpolateEfficiencyTable.from_arrays(w, t, np.zeros(0, dtype=int), np.zeros((0, 3, 2)))
+    for sel in (empty, empty_i):
+        g = mx.FlatGrating(d=1e-3, order_selector=sel, zoom=[1, 20, 20])
+        with pytest.raises(C.CompileError, match='no orders'):
+            C.compile_tree(mx.Sequence(elements=[g]))
