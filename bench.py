@@ -12,7 +12,9 @@ region; every step reads the same input columns and writes fresh outputs.
 
 ``value`` is the FP32 local-frame tracer's photons/s; the FP64 parity
 kernel's rate, their ratio, the FP32 detector error against FP64 and the CPU
-oracle's rate on a bounded sample are reported beside it.
+oracle's rate on a bounded sample are reported beside it, and
+``fp32_with_draws`` times the FP32 step with its uniform draws generated on
+the device inside the timed region.
 
 Two more sections run after it (``--sections``):
 
@@ -171,16 +173,16 @@ def main():
     def step(out, prec):
         ci.launch(out, uni, precision=prec, src=src, status=status)
 
-    def timed(out, prec):
+    def timed(out, prec, fn=step):
         for _ in range(args.warmup):
-            step(out, prec)
+            fn(out, prec)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.steps)]
         t = time.perf_counter()
         for a, b in ev:
             a.record()
-            step(out, prec)
+            fn(out, prec)
             b.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
@@ -198,6 +200,22 @@ def main():
     log(f'FP32: {wall32 / args.steps * 1e3:.3f} ms/step, kernel {k32 * 1e3:.3f} ms')
     wall64, k64 = timed(out64, 64)
     log(f'FP64: {wall64 / args.steps * 1e3:.3f} ms/step, kernel {k64 * 1e3:.3f} ms')
+    # the same FP32 step with its uniform draws generated on the device
+    # inside the timed region (the reference draws its random numbers while
+    # processing), into buffers of their own
+    uni_d = torch.empty_like(uni)
+    out_d = DevicePhotons(src.dir, src.pol, src.energy, src.prob, device=dev)
+    status_d = torch.zeros_like(status)
+    gd = torch.Generator(device=dev)
+    gd.manual_seed(args.seed + 1)
+
+    def step_draws(out, prec):
+        uni_d.uniform_(0.0, 1.0, generator=gd)
+        ci.launch(out, uni_d, precision=prec, src=src, status=status_d)
+    wall_d, _ = timed(out_d, 32, step_draws)
+    raise_for_status(status_d.cpu().numpy())
+    del out_d, uni_d
+    log(f'FP32 with draws generated in the step: {wall_d / args.steps * 1e3:.3f} ms/step')
     st = status.cpu().numpy()
     raise_for_status(st)
 
@@ -315,6 +333,11 @@ def main():
                      'frac': gbs / HBM_PEAK_GBS,
                      'traffic': traffic},
         'cpu_baseline': cpu,
+        'fp32_with_draws': {'photons_per_s': n / (wall_d / args.steps),
+                            'ms_per_step': wall_d / args.steps * 1e3,
+                            'uniform_slots': ns,
+                            'what': 'the FP32 step with its uniform draws generated on the '
+                                    'device (torch Philox, float64) inside the timed region'},
         'fp64_photons_per_s': pps64,
         'fp64_ms_per_step': wall64 / args.steps * 1e3,
         'fp32_over_fp64': pps32 / pps64,
