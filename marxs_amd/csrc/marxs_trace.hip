@@ -322,7 +322,8 @@ __device__ __forceinline__ bool select_order_t(HP hd, DP eg, DP cdf, DP tot, IP 
 // divided by the total exceeds u, the first order when none does; factor =
 // total.  hd[0] = n_wave, hd[1] = n_orders, hd[9] = n_angle; offsets hd[2]
 // wavelengths, hd[4] angles, hd[3] probabilities [order][wave][angle], hd[5]
-// orders.
+// orders; hd[10] = 1 when the FP32 LDS copy holds the values summed over the
+// orders (no negative value in the table).
 #define MX_KEV_NM 1.2398419843320026
 __device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, double &h0, double &h1) {
     x = x < g[0] ? g[0] : (x > g[n - 1] ? g[n - 1] : x);
