@@ -777,7 +777,9 @@ def _build_lds(tables, budget32=None, budget64=None):
 def _interp_row(sel, sel_d, sel_o):
     """Selector record of an interpolated efficiency table: grid sizes,
     offsets of wavelength (nm), blaze-angle (rad) and probability
-    ([order][wave][angle]) blocks in sel_d, of the orders in sel_o; flag 8."""
+    ([order][wave][angle]) blocks in sel_d, of the orders in sel_o; flag 8;
+    column 11 the offset of the values summed over the orders ([wave][angle],
+    only when no value is negative; else -1)."""
     if getattr(sel, 'k', 1) != 1:
         raise CompileError('interpolated efficiency tables are traced with degree-1 '
                            '(bilinear) interpolation only; got k=%r' % sel.k)
@@ -792,12 +794,17 @@ def _interp_row(sel, sel_d, sel_o):
         raise CompileError('interpolated efficiency table lists no orders')
     if np.any(np.abs(np.asarray(orders)) >= 2 ** 31 - 1):
         raise CompileError('order numbers must fit in int32')
-    row = [nw, no, len(sel_d), 0, 0, len(sel_o), -1, -1, 8, nt, 0, 0]
+    row = [nw, no, len(sel_d), 0, 0, len(sel_o), -1, -1, 8, nt, 0, -1]
     sel_d.extend(wave.tolist())
     row[4] = len(sel_d)
     sel_d.extend(theta.tolist())
     row[3] = len(sel_d)
     sel_d.extend(data.ravel().tolist())
+    if np.all(data >= 0):
+        # the values summed over the orders, [wave][angle]: the kernel's
+        # estimate of the total (sel_i[k, 11])
+        row[11] = len(sel_d)
+        sel_d.extend(data.reshape(no, nw, nt).sum(axis=0).ravel().tolist())
     sel_o.extend(np.asarray(orders, dtype=np.int64).tolist())
     return row
 

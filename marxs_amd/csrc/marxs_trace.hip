@@ -323,7 +323,8 @@ __device__ __forceinline__ bool select_order_t(HP hd, DP eg, DP cdf, DP tot, IP 
 // total.  hd[0] = n_wave, hd[1] = n_orders, hd[9] = n_angle; offsets hd[2]
 // wavelengths, hd[4] angles, hd[3] probabilities [order][wave][angle], hd[5]
 // orders; hd[10] = 1 when the FP32 LDS copy holds the values summed over the
-// orders (no negative value in the table).
+// orders (no negative value in the table); hd[11] offset of the values summed
+// over the orders, [wave][angle] (-1: the table has a negative value).
 #define MX_KEV_NM 1.2398419843320026
 __device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, double &h0, double &h1) {
     x = x < g[0] ? g[0] : (x > g[n - 1] ? g[n - 1] : x);
@@ -340,10 +341,17 @@ __device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, d
 
 // wg, tg: wavelength and angle grids (float64); dat: values (float64, or
 // float32 in the FP32 path's LDS copy), [order][wave][angle]; ords: orders.
+// tsum (or null): the values summed over the orders, [wave][angle] float64,
+// present when no value is negative.  Its bilinear value estimates the total
+// so that the order can be picked during the first pass: with values >= 0
+// the running sums do not decrease and a correctly rounded division by the
+// (positive) total is monotone, so the first order whose exact running sum
+// over the exact total exceeds u is the candidate j when that test is false
+// at j - 1 and true at j; otherwise the second pass decides, as without it.
 template <typename GP, typename DP, typename OP>
 __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP dat, OP ords, int no,
                                               double energy, double blaze, double u, int &order,
-                                              double &ptot) {
+                                              double &ptot, GP tsum = nullptr) {
     int iw, it;
     double hx0, hx1, hy0, hy1;
     // the empty asm keeps the compiler from hoisting this division out of
@@ -354,6 +362,16 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
     hat1(tg, nt, blaze, it, hy0, hy1);
     const size_t o00 = (size_t)iw * nt + it, o01 = o00 + 1, o10 = o00 + nt, o11 = o10 + 1;
     const size_t stride = (size_t)nw * nt;
+    double lim = __builtin_inf(), r0 = 0.0, r1 = 0.0;
+    int jc = -1;
+    if (tsum) {
+        double est = 0.0;
+        est = est + tsum[o00] * hx0 * hy0;
+        est = est + tsum[o01] * hx0 * hy1;
+        est = est + tsum[o10] * hx1 * hy0;
+        est = est + tsum[o11] * hx1 * hy1;
+        lim = u * est;
+    }
     double total = 0.0;
     for (int m = 0; m < no; ++m) {
         const DP d = dat + (size_t)m * stride;
@@ -362,7 +380,14 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
         sp = sp + (double)d[o01] * hx0 * hy1;
         sp = sp + (double)d[o10] * hx1 * hy0;
         sp = sp + (double)d[o11] * hx1 * hy1;
+        const double prev = total;
         total = total + sp;
+        if (jc < 0 && total > lim) { jc = m; r0 = prev; r1 = total; }
+    }
+    if (jc >= 0 && !(r0 / total > u) && r1 / total > u) {
+        order = ords[jc];
+        ptot = total;
+        return true;
     }
     int j = 0;
     double run = 0.0;
@@ -446,7 +471,8 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
     const int *hd = T.sel_i + (size_t)sel * MX_SELI;
     if (hd[8] & 8)
         return select_interp(T.sel_d + hd[2], hd[0], T.sel_d + hd[4], hd[9], T.sel_d + hd[3], T.sel_o + hd[5],
-                             hd[1], energy, blaze_fn(), u, order, ptot);
+                             hd[1], energy, blaze_fn(), u, order, ptot,
+                             hd[11] >= 0 ? T.sel_d + hd[11] : nullptr);
     return select_order_t(hd, T.sel_d + hd[2], T.sel_d + hd[3], T.sel_d + hd[4], T.sel_o + hd[5],
                           energy, u, order, ptot);
 }

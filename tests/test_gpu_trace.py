@@ -315,6 +315,30 @@ def test_cat_rowland_blaze_dependent_table_parity(gpu):
     assert len(set(got['order'][got['facet'] >= 0].tolist())) >= 12
 
 
+@pytest.mark.parametrize('scale', [0.0, 0.5, 2.0])
+def test_interpolated_total_estimate_only_guides(gpu, scale):
+    """The table summed over the orders only estimates the total for picking
+    the order in the first pass; the pick is then checked with the exact
+    running sums and total.  With the summed table scaled by 0, 0.5 or 2 the
+    check fails for (nearly) every photon and the second pass decides: FP64
+    orders, probabilities and positions still equal the oracle's."""
+    from marxs_amd.tracer import DeviceTables
+    inst, parts = configs.cat_rowland_instrument(200, efficiency=configs.blaze_interp_efficiency())
+    ci = _ci(inst)
+    t = ci.tables
+    k = int(np.nonzero(t.sel_i[:, 8] & 8)[0][0])
+    nw, nt, o = int(t.sel_i[k, 0]), int(t.sel_i[k, 9]), int(t.sel_i[k, 11])
+    assert o >= 0
+    t.sel_d[o:o + nw * nt] *= scale
+    ci.dev_tables = DeviceTables(t, ci.device)
+    ph, u = make_inputs(20000, 23, ci.n_uniform_slots, 'lines')
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(ci, ph, u, precision=64)
+    rep = compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    assert rep['order'][1] == 0 and rep['facet'][1] == 0
+    assert len(set(got['order'][got['facet'] >= 0].tolist())) >= 10
+
+
 def test_cat_rowland_blaze_dependent_table_fp32(gpu):
     """The FP32 kernel reads the interpolated table from its LDS copy (values
     rounded to float32): against the oracle, orders agree for all but a few

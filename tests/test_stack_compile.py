@@ -257,8 +257,22 @@ def test_interpolated_table_lds_copy_is_summed_over_orders():
     inst, parts = configs.cat_rowland_instrument(20, efficiency=tneg)
     t = C.compile_tree(inst)
     k = int(np.nonzero(t.sel_i[:, 8] & 8)[0][0])
-    assert t.sel_i[k, 10] == 0
+    assert t.sel_i[k, 10] == 0 and t.sel_i[k, 11] == -1
     assert np.array_equal(_interp_block(t, k)[2], neg.astype(np.float32))
+
+
+def test_interpolated_table_total_estimate():
+    """An interpolated table with no negative value also carries its values
+    summed over the orders ([wave][angle], float64, offset sel_i[k, 11]) for
+    the kernel's estimate of the total."""
+    eff = configs.blaze_interp_efficiency()
+    inst, parts = configs.cat_rowland_instrument(20, efficiency=eff)
+    t = C.compile_tree(inst)
+    k = int(np.nonzero(t.sel_i[:, 8] & 8)[0][0])
+    nw, no, nt, o = (int(t.sel_i[k, c]) for c in (0, 1, 9, 11))
+    data = np.asarray(eff.data, dtype=float)
+    assert o >= 0
+    assert np.allclose(t.sel_d[o:o + nw * nt].reshape(nw, nt), data.sum(axis=0), rtol=1e-15, atol=0)
 
 
 def test_selector_tables_without_orders_rejected():
