@@ -40,8 +40,13 @@ CAT_ORDERS = np.arange(-12, 3)
 LINES = np.array([0.5, 0.65, 0.9, 1.2])
 
 
-def flat_grid_instrument():
-    """11 x 11 FlatGrating grid between a PerfectLens and one large detector."""
+def flat_grid_instrument(det_shift=0.):
+    """11 x 11 FlatGrating grid between a PerfectLens and one large detector.
+
+    The zero order focuses onto the detector centre, which is the corner of
+    four pixels (8192 pixels, centre pixel coordinate 4095.5);
+    ``det_shift`` (mm) moves the detector along y and z, e.g. by a third of a
+    pixel (0.008 mm) so that the focus lies inside a pixel."""
     aper = CircleAperture(position=[10000, 0, 0], zoom=[1, 600, 600], r_inner=300)
     lens = PerfectLens(focallength=10000, position=[10000, 0, 0], zoom=[1, 700, 700])
     yz = np.arange(-300, 301, 60, dtype=float)
@@ -50,7 +55,7 @@ def flat_grid_instrument():
                     elem_args={'zoom': [1, 30, 30], 'd': 2e-4,
                                'order_selector': OrderSelector([-1, 0, 1], [.25, .5, .25])},
                     elem_pos={'position': pos}, id_col='facet')
-    det = FlatDetector(pixsize=0.024, position=[0, 0, 0], zoom=[1, 98.304, 98.304])
+    det = FlatDetector(pixsize=0.024, position=[0, det_shift, det_shift], zoom=[1, 98.304, 98.304])
     return Sequence(elements=[aper, lens, grat, det])
 
 

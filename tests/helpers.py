@@ -45,18 +45,28 @@ def near_pixel_boundary(ref, tol=1e-6):
     return out
 
 
-def compare(ref, got, rtol=1e-9, max_int_mismatch=0, near=None, verbose=True,
-            skip=()):
+def compare(ref, got, rtol=1e-9, atol=1e-11, max_int_mismatch=0, near=None, max_near_frac=1e-3,
+            verbose=True, skip=()):
     """Return a report dict; raise AssertionError when outside tolerance.
 
-    Floats: |a - b| <= rtol * max(1, |a|) (NaN must match NaN).  Vectors are
-    compared per component against the vector norm.  Integers: exact, with at
-    most ``max_int_mismatch`` differing photons outside the near-boundary masks
-    ``near[col]``.
+    Floats: |a - b| <= rtol * |a| + atol per value (NaN must match NaN);
+    vector columns (N, 3) per component against the vector's length,
+    |a_k - b_k| <= rtol * |a| + atol.  The absolute floor atol (1e-11 mm,
+    rad, pixel or probability: about five units in the last place of the
+    1.2e4 mm global coordinates that local ones are differences of) only
+    matters for values that are zero up to rounding, such as the coordinates
+    of a photon focused onto the origin (there the relative bound does not
+    exist; measured differences there reach 3e-12 mm and 7e-12 pixels); the
+    report counts the values that pass only because of it.  Integers: exact, with at most ``max_int_mismatch``
+    differing photons outside the near-boundary masks ``near[col]``; the
+    differing photons inside them (a pixel index decided by the last bit of
+    a coordinate on a pixel edge) are counted and must stay at most
+    ``max_near_frac`` of the photons.
     """
     near = near or {}
     report = {}
     names = [k for k in ref if k in got and k not in skip]
+    n_ph = max((len(np.asarray(ref[k])) for k in names), default=0)
     for k in names:
         a, b = np.asarray(ref[k]), np.asarray(got[k])
         if a.dtype.kind in 'iu':
@@ -65,19 +75,29 @@ def compare(ref, got, rtol=1e-9, max_int_mismatch=0, near=None, verbose=True,
             report[k] = ('int', int(diff.sum()) - nb, nb)
             continue
         if a.ndim == 2:
-            scale = np.maximum(1., np.linalg.norm(a, axis=1))[:, None]
+            scale = np.linalg.norm(a, axis=1)[:, None] * np.ones_like(a)
         else:
-            scale = np.maximum(1., np.abs(a))
+            scale = np.abs(a)
         nan_a, nan_b = np.isnan(a), np.isnan(b)
         nan_mism = int((nan_a != nan_b).sum())
         with np.errstate(invalid='ignore'):
-            err = np.where(nan_a | nan_b, 0., np.abs(a - b) / scale)
-        report[k] = ('float', float(err.max()) if err.size else 0., nan_mism)
+            d = np.where(nan_a | nan_b, 0., np.abs(a - b))
+            rel_bound = rtol * np.where(nan_a | nan_b, 0., scale)
+            fail = d > rel_bound + atol
+            floor_only = (d > rel_bound) & ~fail
+            rel = np.where(scale > 0, d / np.where(scale > 0, scale, 1.), np.where(d > 0, np.inf, 0.))
+        report[k] = ('float', float(rel.max()) if rel.size else 0., nan_mism, int(fail.sum()),
+                     int(floor_only.sum()))
+        if fail.any():
+            # the largest absolute difference among the failing values and
+            # its oracle value, for the failure message
+            j = np.unravel_index(np.argmax(np.where(fail, d, -1.)), d.shape)
+            report[k] = report[k] + (float(d[j]), float(a[j]))
     bad = []
     for k, r in report.items():
-        if r[0] == 'int' and r[1] > max_int_mismatch:
+        if r[0] == 'int' and (r[1] > max_int_mismatch or r[2] > max_near_frac * n_ph):
             bad.append((k, r))
-        if r[0] == 'float' and (r[1] > rtol or r[2] > max_int_mismatch):
+        if r[0] == 'float' and (r[3] > 0 or r[2] > max_int_mismatch):
             bad.append((k, r))
     if verbose:
         print({k: r for k, r in report.items()})
