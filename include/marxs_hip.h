@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 17
+#define MX_ABI_VERSION 18
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -46,6 +46,32 @@ extern "C" {
 #define MX_EP 16   /* double per element, interaction parameters */
 #define MX_EI 8    /* int32 per element */
 #define MX_SELI 12 /* int32 per order selector */
+
+/* Order-selector rows sel_i[k][0 .. MX_SELI-1].  Offsets count doubles in
+ * sel_d (ints in sel_o); LDS offsets count bytes in that precision's LDS
+ * image, -1 = not staged (read from sel_d).
+ *
+ * Tabulated selector (OrderSelector, EfficiencyFile; flag bit 3 clear):
+ *   0 n_rows (energies)   1 n_orders           2 energies offset
+ *   3 cdf rows offset ([row][order], cumulative probability / row total)
+ *   4 row totals offset   5 orders offset (sel_o)
+ *   6 LDS offset FP32     7 LDS offset FP64
+ *   8 flags: bit 0 energies strictly increasing, bit 1 every cdf row
+ *     non-decreasing, bit 2 energies evenly spaced within 10 % of the step
+ *   9 float bits of the first energy (bit 2)   10 float bits of 1 / step (bit 2)
+ *  11 0 (unused)
+ * Interpolated efficiency table (InterpolateEfficiencyTable, degree 1;
+ * flag bit 3 set):
+ *   0 n_wave (nm grid)    1 n_orders           2 wavelengths offset
+ *   3 values offset ([order][wave][angle], float64)
+ *   4 blaze angles offset 5 orders offset (sel_o)
+ *   6 LDS offset FP32 (float32 values)
+ *   7 LDS offset FP64 (float64 values, then the summed values of 11)
+ *   8 flags = 8           9 n_angle
+ *  10 1 when the FP32 LDS copy holds the values summed over the orders up
+ *     to each one (no negative value in the table), else 0
+ *  11 offset of the values summed over all orders ([wave][angle], float64),
+ *     the FP64 path's estimate of the total; -1 = absent (a negative value) */
 
 /* stage kinds */
 #define MX_STAGE_APERTURE_CIRCLE 1
@@ -77,9 +103,15 @@ extern "C" {
 
 /* mx_trace flags */
 #define MX_FLAG_NO_GRID 1  /* test every element of every array stage */
+#define MX_FLAG_REGROUP 2  /* regroup the rays of a workgroup before every culled array stage
+                              (results are identical either way; measured slower on the
+                              600-facet workload, see mx_trace_regroup_pairs) */
 /* bits 8..15: workgroups launched per CU (persistent grid); 0 = as many as
  * fit at once (occupancy query) */
 #define MX_FLAG_WG_PER_CU(n) (((n) & 0xff) << 8)
+/* bits 16..23: regroup a workgroup's rays before a stage only when at most
+ * n/256 of them can interact with it (1..255; 0 = the library default) */
+#define MX_FLAG_REGROUP_MAX_FRAC(n) (((n) & 0xff) << 16)
 
 /* status counters (unsigned long long, device memory, accumulated) */
 #define MX_ST_PROB_RANGE    0  /* probability factor outside [0, 1]          */
@@ -89,6 +121,8 @@ extern "C" {
 #define MX_ST_MULTI_HIT     4  /* >1 interaction of one photon in one stage  */
 #define MX_ST_BAD_ARGS      5
 #define MX_ST_TRANS_RANGE   6  /* energy outside the L1 transmission table     */
+#define MX_ST_TESTS         7  /* photon-element plane tests made (candidates of
+                                  the culling lists, or every element) */
 
 typedef struct MxTables {
     int32_t n_stages;
@@ -114,6 +148,11 @@ typedef struct MxTables {
     const unsigned char *lds_image64;  /* the same for FP64 */
     int32_t first_draw_slot;   /* uniform slot of the first grating draw (-1: none), prefetched */
     int32_t ext_stages;        /* 1: some stage is MX_STAGE_STACK or MX_STAGE_CYLINDER (selects the kernels that run them) */
+    int32_t regroup_stages;    /* number of grating, stack or detector stages with a culling grid
+                                  (stage_i[3] bit 0): with MX_FLAG_REGROUP, mx_trace regroups the
+                                  rays of a workgroup before each so that waves are all
+                                  interacting or all passing; 0 = never regroup */
+    int32_t reserved;
 } MxTables;
 
 typedef struct MxPhotons {
@@ -144,6 +183,14 @@ typedef struct MxPhotons {
 int mx_trace(const MxTables *tables, const MxPhotons *photons,
              int32_t precision, int32_t flags, void *stream,
              unsigned long long *status);
+
+/* Ray pairs per exchange round of the regrouping kernel that mx_trace would
+ * launch for these tables, precision and flags (MX_FLAG_REGROUP set; the
+ * LDS left after the image, MX_SLOT32 = 80 or MX_SLOT64 = 112 bytes per ray,
+ * at most half a workgroup of pairs); 0 when mx_trace runs the kernel
+ * without regrouping, -1 on an argument error.  No reference counterpart: a
+ * query for callers that report which kernel ran. */
+int mx_trace_regroup_pairs(const MxTables *tables, int32_t precision, int32_t flags);
 
 /* ------------------------------------------------------------ batched trace
  * Many instrument variants, each with its own tables (ragged element counts)

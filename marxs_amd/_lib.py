@@ -16,7 +16,7 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseAr
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 17
+MX_ABI_VERSION = 18
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -31,7 +31,8 @@ class MxTables(ctypes.Structure):
                 ('elem_i', P), ('elem_t32', P), ('elem_p32', P), ('sel_i', P),
                 ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P),
                 ('succ_start', P), ('succ_items', P), ('lds_image32', P), ('lds_image64', P),
-                ('first_draw_slot', ctypes.c_int32), ('ext_stages', ctypes.c_int32)]
+                ('first_draw_slot', ctypes.c_int32), ('ext_stages', ctypes.c_int32),
+                ('regroup_stages', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
 class MxPhotons(ctypes.Structure):
@@ -93,6 +94,8 @@ def _load():
     lib.mx_trace.restype = ctypes.c_int
     lib.mx_trace.argtypes = [ctypes.POINTER(MxTables), ctypes.POINTER(MxPhotons),
                              ctypes.c_int32, ctypes.c_int32, P, P]
+    lib.mx_trace_regroup_pairs.restype = ctypes.c_int
+    lib.mx_trace_regroup_pairs.argtypes = [ctypes.POINTER(MxTables), ctypes.c_int32, ctypes.c_int32]
     lib.mx_fill_i32.restype = ctypes.c_int
     lib.mx_fill_i32.argtypes = [P, ctypes.c_int64, ctypes.c_int32, P]
     lib.mx_trace_batched.restype = ctypes.c_int

@@ -27,7 +27,7 @@ from . import _lib
 from .compiler import _build_lds
 from .tracer import (CompiledInstrument, DevicePhotons, DeviceTables, PREC, raise_for_status,
                      TraceStatus, ST_PROB_RANGE, ST_ORDER_EMPTY, ST_SLOT_REUSE,
-                     ST_GRID_FALLBACK, ST_MULTI_HIT, ST_TRANS_RANGE)
+                     ST_GRID_FALLBACK, ST_MULTI_HIT, ST_TRANS_RANGE, ST_TESTS)
 
 __all__ = ['BatchedTrace', 'variant_cost', 'outputs_like']
 
@@ -240,7 +240,8 @@ class BatchedTrace:
             dp.has_pos = True
         return [TraceStatus(prob_range=int(r[ST_PROB_RANGE]), order_empty=int(r[ST_ORDER_EMPTY]),
                             slot_reuse=int(r[ST_SLOT_REUSE]), grid_fallback=int(r[ST_GRID_FALLBACK]),
-                            multi_hit=int(r[ST_MULTI_HIT]), trans_range=int(r[ST_TRANS_RANGE]))
+                            multi_hit=int(r[ST_MULTI_HIT]), trans_range=int(r[ST_TRANS_RANGE]),
+                            tests=int(r[ST_TESTS]))
                 for r in raw]
 
     def raw_status(self):

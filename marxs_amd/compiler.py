@@ -177,7 +177,7 @@ def leaf_kind(e):
         return STAGE_APERTURE_RECT
     if isinstance(e, O.PerfectLens):
         if e.reflectivity_interpolator is not None:
-            raise CompileError('PerfectLens with a reflectivity_interpolator is not compiled')
+            reflectivity_table(e.reflectivity_interpolator)
         return STAGE_LENS_PERFECT
     if isinstance(e, O.ThinLens):
         return STAGE_LENS_THIN
@@ -189,6 +189,32 @@ def leaf_kind(e):
     if isinstance(e, O.CircularDetector):
         return STAGE_CYLINDER
     raise CompileError('{0} has no GPU implementation'.format(type(e).__name__))
+
+
+def reflectivity_table(interp):
+    """Grid and values of a PerfectLens ``reflectivity_interpolator``
+    (mirror.py:47-49): a degree-1 tensor spline in (energy, angle) such as
+    scipy's ``RectBivariateSpline(..., kx=1, ky=1)``, the type the
+    reference's documentation and tests use (test_mirror.py:62-80).  For
+    degree 1 the knots are the grid points (doubled at the ends) and the
+    coefficients are the values at the grid points, [energy][angle].
+    Returns (energy grid, angle grid, values)."""
+    deg = getattr(interp, 'degrees', None)
+    if deg is None or not hasattr(interp, 'get_knots') or not hasattr(interp, 'get_coeffs'):
+        raise CompileError('PerfectLens reflectivity_interpolator: only a degree-1 tensor spline '
+                           '(scipy RectBivariateSpline with kx=ky=1) is compiled; got '
+                           + type(interp).__name__)
+    if tuple(deg) != (1, 1):
+        raise CompileError('PerfectLens reflectivity_interpolator: degree-1 (bilinear) splines '
+                           'only; got degrees %r' % (tuple(deg),))
+    tx, ty = (np.asarray(t, dtype=float) for t in interp.get_knots())
+    x, y = tx[1:-1], ty[1:-1]
+    c = np.asarray(interp.get_coeffs(), dtype=float)
+    if len(x) < 2 or len(y) < 2 or c.size != len(x) * len(y) or tx[0] != tx[1] or \
+            tx[-1] != tx[-2] or ty[0] != ty[1] or ty[-1] != ty[-2] or \
+            np.any(np.diff(x) <= 0) or np.any(np.diff(y) <= 0):
+        raise CompileError('PerfectLens reflectivity_interpolator: unexpected spline knots')
+    return x, y, c.reshape(len(x), len(y))
 
 
 def leaf_columns(e, kind):
@@ -617,9 +643,15 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
 
 # LDS image budgets per workgroup (160 KiB of LDS per CU; one workgroup of
 # 1024 (FP32) or 768 (FP64) work-items per CU; the kernel's own LDS is a few
-# dozen bytes).
+# hundred bytes).
 LDS_BUDGET32 = 158 * 1024
 LDS_BUDGET64 = 150 * 1024
+# The loose tier's cell lists (read only by rays more oblique than the tight
+# tier allows) are staged last and only while the image leaves this much of
+# the 160 KiB free for the regrouping kernels' ray exchange buffer (about 600
+# rays in FP32, 440 in FP64); otherwise they are read from global memory.
+LDS_RESERVE = 48 * 1024
+LDS_TOTAL = 160 * 1024
 
 
 def _align(n, a=16):
@@ -653,10 +685,14 @@ def _build_lds(tables, budget32=None, budget64=None):
     float64, then int32 orders), then per array stage of two or more
     elements, while it fits: element test records (16 x f32 / f64), for FP32
     also the interaction parameter records, then the culling grid as uint16
-    cell offsets and uint16 element indices.  Offsets (bytes, -1 = global
-    memory): stage_i[20]/[21] records FP32/FP64, [22] FP32 parameters,
-    [24]/[25] FP32 cell offsets/items, [26]/[27] FP64 cell offsets/items;
-    sel_i[6]/[7] selector block FP32/FP64.
+    cell offsets and uint16 element indices (tight tier) and the successor
+    blocks of both tiers; last, the loose tier's cell offsets and items of
+    each stage while the image leaves ``LDS_RESERVE`` bytes of LDS free.
+    Offsets (bytes, -1 = global memory): stage_i[20]/[21] records FP32/FP64,
+    [22] FP32 parameters, [24]/[25]/[30] FP32 tight cell offsets/items/
+    successors, [26]/[27]/[31] the same FP64, [36]/[37]/[38] FP32 loose cell
+    offsets/items/successors, [39]/[40]/[41] the same FP64; sel_i[6]/[7]
+    selector block FP32/FP64.
     """
     budget32 = LDS_BUDGET32 if budget32 is None else budget32
     budget64 = LDS_BUDGET64 if budget64 is None else budget64
@@ -686,12 +722,16 @@ def _build_lds(tables, budget32=None, budget64=None):
             nr, no = int(tables.sel_i[k, 0]), int(tables.sel_i[k, 1])
             if interp[k]:
                 nt = int(tables.sel_i[k, 9])
-                need += _align(8 * (nr + nt) + 4 * no * nr * nt) + _align(4 * no)
+                if prec == 32:
+                    need += _align(8 * (nr + nt) + 4 * no * nr * nt) + _align(4 * no)
+                else:
+                    ts = nr * nt if tables.sel_i[k, 11] >= 0 else 0
+                    need += _align(8 * (nr + nt + no * nr * nt + ts)) + _align(4 * no)
             else:
                 need += _align(8 * (2 * nr + nr * no)) + _align(4 * no)
         n_sel = tables.sel_i.shape[0]
         need += _align(4 * SELI * n_sel)
-        if need > budget or (prec == 64 and np.any(interp)):
+        if need > budget:
             sel_ok = False
         col = 6 if prec == 32 else 7
         tables.sel_i[:, col] = -1
@@ -701,7 +741,19 @@ def _build_lds(tables, budget32=None, budget64=None):
             assert hdr == 0
             for k in range(n_sel):
                 nr, no, oe, oc, ot, oo = (int(x) for x in tables.sel_i[k, :6])
-                if interp[k]:
+                if interp[k] and prec == 64:
+                    # wavelengths, angles, values [order][wave][angle] and,
+                    # when the table has one (sel_i[k, 11] >= 0), the values
+                    # summed over all orders [wave][angle], all float64 as in
+                    # sel_d; then the orders
+                    nt = int(tables.sel_i[k, 9])
+                    parts64 = [tables.sel_d[oe:oe + nr], tables.sel_d[ot:ot + nt],
+                               tables.sel_d[oc:oc + no * nr * nt]]
+                    o11 = int(tables.sel_i[k, 11])
+                    if o11 >= 0:
+                        parts64.append(tables.sel_d[o11:o11 + nr * nt])
+                    o = add(np.concatenate(parts64).astype(float))
+                elif interp[k]:
                     # wavelengths, angles (float64), values (float32,
                     # [order][wave][angle]), then the orders.  With no
                     # negative value the values are stored summed over the
@@ -725,6 +777,7 @@ def _build_lds(tables, budget32=None, budget64=None):
                 tables.sel_i[k, col] = o
             # fill the header with this image's rows
             parts[0] = (0, np.ascontiguousarray(tables.sel_i).astype(np.int32).view(np.uint8).ravel().copy())
+        staged = []
         for s, st in enumerate(tables.stages):
             n = len(st.elements)
             if st.kind not in PLANE_ARRAYS or n < 2:
@@ -738,8 +791,8 @@ def _build_lds(tables, budget32=None, budget64=None):
                 g = None
             size = _align(recs.nbytes) + (_align(params.nbytes) if params is not None else 0)
             if g is not None:
+                size += _align(2 * len(g['cell_start'])) + _align(2 * len(g['items']))
                 for t in ('', '_l'):
-                    size += _align(2 * len(g['cell_start' + t])) + _align(2 * len(g['items' + t]))
                     size += _align(2 * len(g['succ_start' + t])) + _align(2 * len(g['succ_items' + t]) + 2)
             if off + size > budget or (st.grid is not None and g is None):
                 continue
@@ -748,20 +801,28 @@ def _build_lds(tables, budget32=None, budget64=None):
             if params is not None:
                 si[s, 22] = add(params)
             if g is not None:
-                # both tiers: cell offsets and items (uint16), and successor
-                # blocks of (n + 1) uint16 offsets, padded to 16 bytes, then
-                # the uint16 element indices
-                cols = {'': ((24, 25, 30) if prec == 32 else (26, 27, 31)),
-                        '_l': ((36, 37, 38) if prec == 32 else (39, 40, 41))}
-                for t, (c0, c1, c2) in cols.items():
-                    si[s, c0] = add(g['cell_start' + t].astype(np.uint16))
-                    si[s, c1] = add(g['items' + t].astype(np.uint16))
+                # tight cell offsets and items (uint16), and the successor
+                # blocks of both tiers: (n + 1) uint16 offsets, padded to 16
+                # bytes, then the uint16 element indices
+                c0, c1 = (24, 25) if prec == 32 else (26, 27)
+                si[s, c0] = add(g['cell_start'].astype(np.uint16))
+                si[s, c1] = add(g['items'].astype(np.uint16))
+                for t, c2 in (('', 30 if prec == 32 else 31), ('_l', 38 if prec == 32 else 41)):
                     ss, sit = g['succ_start' + t], g['succ_items' + t]
                     blk = np.zeros(_align(2 * len(ss)) // 2 + len(sit) + 1, dtype=np.uint16)
                     blk[:len(ss)] = ss
                     o0 = _align(2 * len(ss)) // 2
                     blk[o0:o0 + len(sit)] = sit
                     si[s, c2] = add(blk)
+                staged.append((s, g))
+        # the loose tier's cell lists, while LDS_RESERVE stays free
+        for s, g in staged:
+            size = _align(2 * len(g['cell_start_l'])) + _align(2 * len(g['items_l']))
+            if off + size > min(budget, LDS_TOTAL - LDS_RESERVE):
+                continue
+            c0, c1 = (36, 37) if prec == 32 else (39, 40)
+            si[s, c0] = add(g['cell_start_l'].astype(np.uint16))
+            si[s, c1] = add(g['items_l'].astype(np.uint16))
         img = np.zeros(_align(max(off, 16)), dtype=np.uint8)
         for o, b in parts:
             img[o:o + b.nbytes] = b
@@ -871,6 +932,16 @@ def _element_params(kind, x, f, ep, ei, sel_index, sel_rows, sel_d, sel_o):
     elif kind == STAGE_LENS_PERFECT:
         ep[0:3] = (g['center'] - x.d_center_optax * g['e_z'])[:3]
         ep[6] = x.focallength
+        # reflectivity table in sel_d: n_energy, n_angle, energies, angles,
+        # values [energy][angle] (-1: reflectivity 1)
+        ei[6] = -1
+        if x.reflectivity_interpolator is not None:
+            ge, ga, c = reflectivity_table(x.reflectivity_interpolator)
+            ei[6] = len(sel_d)
+            sel_d.extend([float(len(ge)), float(len(ga))])
+            sel_d.extend(ge.tolist())
+            sel_d.extend(ga.tolist())
+            sel_d.extend(c.ravel().tolist())
     elif kind == STAGE_LENS_THIN:
         ep[6] = x.focallength
     elif kind == STAGE_GRATING:

@@ -52,6 +52,9 @@
 #define MX_BLOCK_EXT32 768      /* the FP32 kernels with stack or tube stages */
 #endif
 #define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
+#ifndef MX_REGROUP_DEFAULT_FRAC
+#define MX_REGROUP_DEFAULT_FRAC 256  /* exchange when <= this/256 of the rays interact */
+#endif
 #ifndef MX_CDF_COUNT
 #define MX_CDF_COUNT 0          /* 1: count cdf entries <= u instead of bisecting */
 #endif
@@ -232,7 +235,7 @@ template <typename T> __device__ __forceinline__ cptr<T> cp(const T *p) { return
 struct Counters {
     unsigned *c;
 };
-#define MX_NCOUNT 7
+#define MX_NCOUNT 8
 __device__ __forceinline__ void count(const Counters &cnt, int k) {
     const unsigned long long m = __ballot(1);
     if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
@@ -337,6 +340,26 @@ __device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, d
     const double f = 1.0 / (g[i + 1] - g[i]);
     h0 = f * (g[i + 1] - x);
     h1 = f * (x - g[i]);
+}
+
+// PerfectLens reflectivity (marxs/optics/mirror.py:47-49,72-82): the
+// degree-1 tensor spline of record rt = n_e, n_a, energy grid, angle grid,
+// values [energy][angle] at (energy, angle), arguments clamped to the grid
+// and the four terms added in FITPACK's order.
+__device__ __forceinline__ double lens_reflectivity(const double *rt, double energy, double angle) {
+    const int ne = (int)rt[0], na = (int)rt[1];
+    const double *ge = rt + 2, *ga = ge + ne, *c = ga + na;
+    int ie, ia;
+    double hx0, hx1, hy0, hy1;
+    hat1(ge, ne, energy, ie, hx0, hx1);
+    hat1(ga, na, angle, ia, hy0, hy1);
+    const double *c0 = c + (size_t)ie * na + ia, *c1 = c0 + na;
+    double sp = 0.0;
+    sp = sp + c0[0] * hx0 * hy0;
+    sp = sp + c0[1] * hx0 * hy1;
+    sp = sp + c1[0] * hx1 * hy0;
+    sp = sp + c1[1] * hx1 * hy1;
+    return sp;
 }
 
 // wg, tg: wavelength and angle grids (float64); dat: values (float64, or
@@ -445,7 +468,8 @@ __device__ __forceinline__ bool select_interp_cum(GP wg, int nw, GP tg, int nt, 
 }
 
 // Selector sel of the tables.  When the selectors are staged in LDS (all or
-// none; an interpolated table only in the FP32 image, its values as float32),
+// none; an interpolated table's values as float32 in the FP32 image, as
+// float64 with the order-summed values in the FP64 image),
 // their header rows sit at the start of the LDS image.  blaze_fn() gives the photon's blaze angle (only the
 // interpolated tables need it).
 template <int PREC, typename BF>
@@ -465,6 +489,16 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
             if (hd[10] == 1) return select_interp_cum(b, nr, b + nr, nt, v, o, no, energy, blaze_fn(), u, order, ptot);
             return select_interp(b, nr, b + nr, nt, v, o, no, energy, blaze_fn(), u, order, ptot);
         }
+        if (PREC == 64 && (hd[8] & 8)) {
+            // FP64 image: grids, values [order][wave][angle], the values
+            // summed over the orders when hd[11] >= 0 (all float64), orders
+            const int nt = hd[9];
+            const double *v = b + nr + nt;
+            const int ts = hd[11] >= 0 ? nr * nt : 0;
+            const int *o = (const int *)(lds + loff + ((8 * (nr + nt + no * nr * nt + ts) + 15) & ~15));
+            return select_interp(b, nr, b + nr, nt, v, o, no, energy, blaze_fn(), u, order, ptot,
+                                 ts ? v + (size_t)no * nr * nt : nullptr);
+        }
         const int *o = (const int *)(lds + loff + ((8 * (2 * nr + nr * no) + 15) & ~15));
         return select_order_t(hd, b, b + nr, b + nr + nr * no, o, energy, u, order, ptot);
     }
@@ -483,10 +517,10 @@ __device__ __forceinline__ bool select_order(const MxTables &T, int sel, double 
 // elements.  g: a, b1, b2, S_lo, S_hi, U0, W0, 1/cu, 1/cw, U1, W1, delta_t,
 // tan_t, delta_l, tan_l (stage_f / stage_d layout, see compiler).  cs: the
 // stage's tight cell offsets (cell c lists items [cs[c], cs[c+1])), csl the
-// loose ones (both in LDS or both in global memory).
-template <typename R, typename V, typename CS, typename SIP, typename GP>
+// loose ones (each in LDS or in global memory).
+template <typename R, typename V, typename CS, typename CSL, typename SIP, typename GP>
 __device__ __forceinline__ int cull_cell(SIP si, GP g, V r, V d, R kmin, int &b, int &e,
-                                         CS cs, CS csl) {
+                                         CS cs, CSL csl) {
     R pa = r.x * g[0] + r.y * g[1] + r.z * g[2];
     R da = d.x * g[0] + d.y * g[1] + d.z * g[2];
     R pb1 = r.x * g[3] + r.y * g[4] + r.z * g[5];
@@ -571,6 +605,7 @@ struct Ray {
     f3 df, polf;         // FP32 direction / polarization
     double prob, energy;
     double u_first;      // prefetched first grating draw
+    int64_t i;           // photon index (-1: no photon)
 };
 
 // Uniform slot of the k-th interaction of a photon inside one stage.
@@ -1179,7 +1214,7 @@ __device__ __forceinline__ void tube_stage(const MxTables &T, const MxPhotons &P
 template <int PREC, typename ETP, typename ITP>
 __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, int te,
                                          int last, d3 p, d3 d, f3 pr, f3 df, float kmin32,
-                                         d3 &q, double &ly, double &lz, float &kf) {
+                                         d3 &q, double &ly, double &lz, float &kf, const Counters &cnt) {
     constexpr int G = MX_SCAN_GROUP;
 #if MX_SCAN_PREFETCH
     if (G == 1 && list && PREC == 64) {
@@ -1188,6 +1223,7 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
         // (FP64: 3.5 % faster; FP32 unchanged, so it keeps the plain loop)
         int jn = tb < te ? (int)items[tb] : -1;
         for (int t = tb; t < te; ++t) {
+            count(cnt, MX_ST_TESTS);
             const int j0 = jn;
             jn = t + 1 < te ? (int)items[t + 1] : -1;
             const int j = j0 > last ? j0 : 0;
@@ -1210,6 +1246,8 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
     }
 #endif
     for (int t = tb; t < te; t += G) {
+        // candidate tests made (photon-element tests of the path)
+        if (G == 1) count(cnt, MX_ST_TESTS);
         int jj[G];
         bool hh[G];
 #pragma unroll
@@ -1259,12 +1297,15 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
     const d3 O = ld3(sd);
     // tables of this stage
     const unsigned char *rec_l = lds + (L ? si[PREC == 64 ? 21 : 20] : 0);
+    // tight cell lists and both tiers' successors in LDS (or all lists in
+    // global memory); the loose cell lists may stay in global memory alone
     const bool lds_lists = L && si[PREC == 64 ? 26 : 24] >= 0;
+    const bool lds_loose = lds_lists && si[PREC == 64 ? 39 : 36] >= 0;
     const unsigned short *cs_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 26 : 24] : 0));
     const unsigned short *it_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 27 : 25] : 0));
     const unsigned short *su_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 31 : 30] : 0));
-    const unsigned short *csl_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 39 : 36] : 0));
-    const unsigned short *itl_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 40 : 37] : 0));
+    const unsigned short *csl_l = (const unsigned short *)(lds + (lds_loose ? si[PREC == 64 ? 39 : 36] : 0));
+    const unsigned short *itl_l = (const unsigned short *)(lds + (lds_loose ? si[PREC == 64 ? 40 : 37] : 0));
     const unsigned short *sul_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 41 : 38] : 0));
     const int su_o = ((2 * (n_el + 1) + 15) & ~15) >> 1;   // items after the offsets
     const int *cs_g = T.cell_start + si[8];
@@ -1324,10 +1365,12 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             }
         } else if (grid) {
             if (PREC == 32) {
-                if (lds_lists) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l, csl_l);
+                if (lds_loose) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l, csl_l);
+                else if (lds_lists) mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_l, csl_g);
                 else mode = cull_cell<float, f3>(si, sf, pr, R.df, kmin32, cb, ce, cs_g, csl_g);
             } else {
-                if (lds_lists) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l, csl_l);
+                if (lds_loose) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l, csl_l);
+                else if (lds_lists) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l, csl_g);
                 else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g, csl_g);
             }
             if (mode == 0) break;
@@ -1345,19 +1388,19 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         // cell_items (tight and loose cell lists hold absolute indices)
         const int *items_g = mode == 3 ? su_g : (mode == 5 ? sul_g : T.cell_items);
         if (L) {
-            if (lds_lists)
+            if (lds_lists && (lds_loose || mode != 4))
                 hit = scan_next<PREC>(rec_l, list,
                                       mode == 3 ? su_l + su_o : (mode == 5 ? sul_l + su_o : (mode == 4 ? itl_l : it_l)),
-                                      tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+                                      tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf, cnt);
             else
                 hit = scan_next<PREC>(rec_l, list, items_g, tb, te, last,
-                                      R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+                                      R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf, cnt);
         } else if (PREC == 64) {
             hit = scan_next<PREC>(T.elem_t + (size_t)e0 * MX_ET, list, items_g,
-                                  tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+                                  tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf, cnt);
         } else {
             hit = scan_next<PREC>(T.elem_t32 + (size_t)e0 * MX_ET, list, items_g,
-                                  tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf);
+                                  tb, te, last, R.p, R.d, pr, R.df, kmin32, q, ly, lz, kf, cnt);
         }
         if (hit < 0) break;
         if (PREC == 32) {
@@ -1447,18 +1490,33 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             id = ei[0];
             d3 popt = ld3(ep);
             double f = ep[6];
+            double ang;   // angle between the old and the new direction
             if (PREC == 64) {
                 d3 dn = unit3(R.d);
                 d3 fp = mk(popt.x + f * dn.x, popt.y + f * dn.y, popt.z + f * dn.z);
                 d3 nd = unit3(sub3(fp, q));
+                if (ei[6] >= 0) ang = acos64<EXT>(fabs(dot3(nd, dn)));
                 R.pol = ptrans64(R.d, nd, R.pol);
                 R.d = nd;
             } else {
                 f3 dn = unitf(R.df);
                 d3 fp = mk(popt.x + f * (double)dn.x, popt.y + f * (double)dn.y, popt.z + f * (double)dn.z);
                 f3 nd = unitf(tof(sub3(fp, q)));
+                if (ei[6] >= 0) {
+                    // atan2(|a x b|, |a . b|) keeps the digits of small angles
+                    const f3 cr = crossf(nd, dn);
+                    ang = (double)atan2f(sqrtf(dotf(cr, cr)), fabsf(dotf(nd, dn)));
+                }
                 R.polf = ptrans32(R.df, nd, R.polf);
                 R.df = nd;
+            }
+            if (ei[6] >= 0) {
+                // probability *= reflectivity(energy, angle / 4) ** 2
+                // (mirror.py:72-82), a degree-1 tensor spline
+                const double r = lens_reflectivity(T.sel_d + ei[6], R.energy, ang / 4);
+                const double pf = r * r;
+                if (pf < 0.0 || pf > 1.0) count(cnt, MX_ST_PROB_RANGE);
+                R.prob = R.prob * pf;
             }
         } else {  // MX_STAGE_LENS_THIN
             id = ei[0];
@@ -1484,12 +1542,9 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
 }
 
 // ======================================================= the trace kernel
-// One photon through all stages.
-template <int PREC, bool EXT>
-__device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
-                                          const bool use_grid, const Counters &cnt,
-                                          const unsigned char *lds) {
-    Ray R;
+// Photon i's input columns into the ray state.
+__device__ __forceinline__ void init_ray(const MxTables &T, const MxPhotons &P, const int64_t i, Ray &R) {
+    R.i = i;
     R.p = mk(0.0, 0.0, 0.0);
     if (P.read_pos) {
         const double *const *pin = P.pos_in[0] ? P.pos_in : (const double *const *)P.pos;
@@ -1505,11 +1560,15 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
     R.energy = P.energy ? P.energy[i] : 1.0;
     R.u_first = (T.first_draw_slot >= 0 && T.first_draw_slot < P.n_uniform_slots)
                     ? P.uniforms[(int64_t)T.first_draw_slot * P.uniform_stride + i] : 0.0;
+}
 
-    for (int s_ = 0; s_ < T.n_stages; ++s_) {
-        // the stage index is the same in every lane: keep it (and all stage
-        // metadata derived from it) in scalar registers
-        const int s = __builtin_amdgcn_readfirstlane(s_);
+// Stage s (wave-uniform) for one photon.
+template <int PREC, bool EXT>
+__device__ __forceinline__ void run_stage(const MxTables &T, const MxPhotons &P, const int s,
+                                          const bool use_grid, const Counters &cnt,
+                                          const unsigned char *lds, Ray &R) {
+    const int64_t i = R.i;
+    {
         const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
         const int kind = si[0], e0 = si[2];
 
@@ -1547,7 +1606,7 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             put_f(P, si[10], i, x);
             put_f(P, si[11], i, y);
             put_i(P, si[12], i, cp(T.elem_i)[(size_t)e0 * MX_EI]);
-            continue;
+            return;
         }
         if (kind == MX_STAGE_CATSUPPORT) {
             // catsupportbars (marxs/missions/mitsnl/catgrating.py:314-324):
@@ -1555,25 +1614,235 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
             // the facet id column, written by an earlier stage of this trace
             const int c = si[6];
             if (c < 0 || P.icol[c][i] < 0) R.prob = 0.0;
-            continue;
+            return;
         }
         if (EXT && kind == MX_STAGE_CYLINDER) {
             tube_stage<PREC>(T, P, si, i, cnt, R);
-            continue;
+            return;
         }
         if (kind < MX_STAGE_LENS_PERFECT || kind > (EXT ? MX_STAGE_STACK : MX_STAGE_DETECTOR)) {
             count(cnt, MX_ST_BAD_ARGS);
-            continue;
+            return;
         }
         if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, EXT>(T, P, s, i, use_grid, cnt, lds, R);
         else array_stage<PREC, false, EXT>(T, P, s, i, use_grid, cnt, lds, R);
     }
+}
 
+// The ray's final position, direction, polarization and probability.
+template <int PREC>
+__device__ __forceinline__ void finish_ray(const MxPhotons &P, Ray &R) {
+    const int64_t i = R.i;
     if (PREC == 32) { R.d = tod(R.df); R.pol = tod(R.polf); }
     P.pos[0][i] = R.p.x; P.pos[1][i] = R.p.y; P.pos[2][i] = R.p.z;
     P.dir[0][i] = R.d.x; P.dir[1][i] = R.d.y; P.dir[2][i] = R.d.z;
     P.pol[0][i] = R.pol.x; P.pol[1][i] = R.pol.y; P.pol[2][i] = R.pol.z;
     P.prob[i] = R.prob;
+}
+
+// One photon through all stages.
+template <int PREC, bool EXT>
+__device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
+                                          const bool use_grid, const Counters &cnt,
+                                          const unsigned char *lds) {
+    Ray R;
+    init_ray(T, P, i, R);
+    for (int s_ = 0; s_ < T.n_stages; ++s_) {
+        // the stage index is the same in every lane: keep it (and all stage
+        // metadata derived from it) in scalar registers
+        const int s = __builtin_amdgcn_readfirstlane(s_);
+        run_stage<PREC, EXT>(T, P, s, use_grid, cnt, lds, R);
+    }
+    finish_ray<PREC>(P, R);
+}
+
+// ------------------------------------------------- regrouping between stages
+// A stage whose culling grid can rule out every element for a ray is a
+// regrouping point: before it, the workgroup's rays are exchanged so that
+// the rays that may interact occupy the first threads.  Waves are then
+// either all interacting (the element scan and the interaction run at full
+// width) or all passing (the stage only writes its default columns), instead
+// of nearly every wave running the interaction for a few of its lanes.
+// Photons are independent, so which thread carries a ray changes nothing in
+// its result; outputs are written at the ray's own index.
+template <int PREC>
+__device__ __forceinline__ bool regroup_stage(const MxTables &T, const int s, const bool use_grid) {
+    const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
+    const int kind = si[0];
+    return use_grid && (si[3] & 1) && (kind == MX_STAGE_GRATING || kind == MX_STAGE_DETECTOR ||
+                                       kind == MX_STAGE_STACK);
+}
+
+// Whether stage s (a regrouping stage) can interact with the ray: the first
+// culling decision of array_stage (mode != 0), computed the same way.
+template <int PREC>
+__device__ __forceinline__ bool stage_may_hit(const MxTables &T, const int s, const unsigned char *lds,
+                                              const Ray &R) {
+    const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
+    const cptr<double> sd = cp(T.stage_d) + (size_t)s * MX_SD;
+    const cptr<float> sf = cp(T.stage_f) + (size_t)s * MX_SD;
+    const bool L = si[PREC == 64 ? 21 : 20] >= 0;
+    const bool lds_lists = L && si[PREC == 64 ? 26 : 24] >= 0;
+    const bool lds_loose = lds_lists && si[PREC == 64 ? 39 : 36] >= 0;
+    const unsigned short *cs_l = (const unsigned short *)(lds + (lds_lists ? si[PREC == 64 ? 26 : 24] : 0));
+    const unsigned short *csl_l = (const unsigned short *)(lds + (lds_loose ? si[PREC == 64 ? 39 : 36] : 0));
+    const int *cs_g = T.cell_start + si[8];
+    const int *csl_g = T.cell_start + si[32];
+    const d3 O = ld3(sd);
+    int cb = 0, ce = 0, mode;
+    if (PREC == 32) {
+#pragma clang fp contract(fast)
+        f3 r0 = tof(sub3(R.p, O));
+        f3 a = ldf3(sf);
+        float da = dotf(R.df, a);
+        float kadv = 0.0f;
+        if (da != 0.0f && da == da) {
+            kadv = fdiv_fast(-dotf(r0, a), da);
+            if (!(fabsf(kadv) < 1e30f)) kadv = 0.0f;
+        }
+        const f3 pr = mkf(r0.x + kadv * R.df.x, r0.y + kadv * R.df.y, r0.z + kadv * R.df.z);
+        if (lds_loose) mode = cull_cell<float, f3>(si, sf, pr, R.df, -kadv, cb, ce, cs_l, csl_l);
+        else if (lds_lists) mode = cull_cell<float, f3>(si, sf, pr, R.df, -kadv, cb, ce, cs_l, csl_g);
+        else mode = cull_cell<float, f3>(si, sf, pr, R.df, -kadv, cb, ce, cs_g, csl_g);
+    } else {
+        if (lds_loose) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l, csl_l);
+        else if (lds_lists) mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_l, csl_g);
+        else mode = cull_cell<double, d3>(si, sd + 3, sub3(R.p, O), R.d, 0.0, cb, ce, cs_g, csl_g);
+    }
+    return mode != 0;
+}
+
+// Ray state in an LDS exchange slot (MX_SLOT32 / MX_SLOT64 bytes, 16-byte
+// aligned): the FP32 path carries its float direction and polarization, the
+// FP64 path its double ones.
+#define MX_SLOT32 80
+#define MX_SLOT64 112
+template <int PREC>
+__device__ __forceinline__ void store_ray(unsigned char *slot, const Ray &R) {
+    double2 *v = (double2 *)slot;
+    v[0] = make_double2(R.p.x, R.p.y);
+    v[1] = make_double2(R.p.z, R.prob);
+    v[2] = make_double2(R.energy, R.u_first);
+    if (PREC == 32) {
+        float4 *f = (float4 *)slot;
+        f[3] = make_float4(R.df.x, R.df.y, R.df.z, R.polf.x);
+        f[4] = make_float4(R.polf.y, R.polf.z, __int_as_float((int)(R.i & 0xffffffff)),
+                           __int_as_float((int)(R.i >> 32)));
+    } else {
+        v[3] = make_double2(R.d.x, R.d.y);
+        v[4] = make_double2(R.d.z, R.pol.x);
+        v[5] = make_double2(R.pol.y, R.pol.z);
+        v[6] = make_double2(__longlong_as_double(R.i), 0.0);
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void load_ray(const unsigned char *slot, Ray &R) {
+    const double2 *v = (const double2 *)slot;
+    const double2 a = v[0], b = v[1], c = v[2];
+    R.p = mk(a.x, a.y, b.x);
+    R.prob = b.y;
+    R.energy = c.x;
+    R.u_first = c.y;
+    if (PREC == 32) {
+        const float4 *f = (const float4 *)slot;
+        const float4 f3_ = f[3], f4_ = f[4];
+        R.df = mkf(f3_.x, f3_.y, f3_.z);
+        R.polf = mkf(f3_.w, f4_.x, f4_.y);
+        R.i = (int64_t)(uint32_t)__float_as_int(f4_.z) | ((int64_t)__float_as_int(f4_.w) << 32);
+    } else {
+        const double2 d0 = v[3], d1 = v[4], d2 = v[5], d3_ = v[6];
+        R.d = mk(d0.x, d0.y, d1.x);
+        R.pol = mk(d1.y, d2.x, d2.y);
+        R.i = __double_as_longlong(d3_.x);
+    }
+}
+
+// Partition of the workgroup's BS rays by `act`: afterwards threads
+// [0, n_act) hold the rays with act set (returned: n_act; -1 without an
+// exchange when more than max_act rays are flagged, where waves would be
+// mostly interacting anyway).  Only misplaced rays move: the k-th
+// thread below n_act without the flag and the k-th thread at or above n_act
+// with it exchange their rays through two slots of the buffer, `half` pairs
+// per round (a round: stores, barrier, loads, barrier).  smask: BS / 64
+// wave ballots in LDS (alternate two arrays between consecutive calls, so
+// that a slow wave's reads of one call cannot meet the next call's writes).
+// Every thread of the workgroup calls it (it contains barriers).
+template <int PREC, int BS>
+__device__ __forceinline__ int partition_rays(Ray &R, const bool act, unsigned long long *smask,
+                                              unsigned char *buf, const int half, const int max_act) {
+    constexpr int NW = BS / 64;
+    constexpr int SLOT = PREC == 64 ? MX_SLOT64 : MX_SLOT32;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const unsigned long long m = __ballot(act);
+    if (lane == 0) smask[w] = m;
+    __syncthreads();
+    int before = 0, n_act = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const int c = __popcll(smask[k]);
+        before += k < w ? c : 0;
+        n_act += c;
+    }
+    // flagged rays among threads [0, n_act)
+    const int wn = n_act >> 6, ln = n_act & 63;
+    int a_lo = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) a_lo += k < wn ? __popcll(smask[k]) : 0;
+    if (ln) a_lo += __popcll(smask[wn] & ((1ULL << ln) - 1));
+    // more flagged rays than max_act: no exchange (-1 tells the caller
+    // that the rays stayed where they were)
+    if (n_act > max_act) return -1;
+    const int npairs = __builtin_amdgcn_readfirstlane(n_act - a_lo);
+    const int r_a = before + __popcll(m & ((1ULL << lane) - 1));   // flagged rays below t
+    int pair = -1, side = 0;
+    if (t < n_act && !act) { pair = t - r_a; side = 0; }
+    else if (t >= n_act && act) { pair = r_a - a_lo; side = 1; }
+    for (int r0 = 0; r0 < npairs; r0 += half) {
+        const bool mine = pair >= r0 && pair < r0 + half;
+        const int k = 2 * (pair - r0);
+        if (mine) store_ray<PREC>(buf + (size_t)(k + side) * SLOT, R);
+        __syncthreads();
+        if (mine) load_ray<PREC>(buf + (size_t)(k + 1 - side) * SLOT, R);
+        __syncthreads();
+    }
+    return n_act;
+}
+
+// Chunks of BS photons (one per thread) through all stages, regrouped
+// before every regrouping stage.
+template <int PREC, int BS, bool EXT>
+__device__ __forceinline__ void trace_chunk(const MxTables &T, const MxPhotons &P, const int64_t base,
+                                            const bool use_grid, const Counters &cnt,
+                                            const unsigned char *lds, unsigned long long *smask,
+                                            unsigned char *xbuf, const int half, const int max_act) {
+    Ray R;
+    const int64_t i0 = base + threadIdx.x;
+    if (i0 < P.n) {
+        init_ray(T, P, i0, R);
+    } else {
+        // no photon: a ray that only moves through the exchanges
+        R.p = R.d = R.pol = mk(0.0, 0.0, 0.0);
+        R.df = R.polf = mkf(0.0f, 0.0f, 0.0f);
+        R.prob = R.energy = R.u_first = 0.0;
+        R.i = -1;
+    }
+    int parity = 0;
+    for (int s_ = 0; s_ < T.n_stages; ++s_) {
+        const int s = __builtin_amdgcn_readfirstlane(s_);
+        bool run = R.i >= 0;
+        if (regroup_stage<PREC>(T, s, use_grid)) {
+            const bool act = run && stage_may_hit<PREC>(T, s, lds, R);
+            const int n_act = partition_rays<PREC, BS>(R, act, smask + parity * (BS / 64), xbuf, half,
+                                                       max_act);
+            parity ^= 1;
+            run = n_act >= 0 ? (int)threadIdx.x < n_act : act;
+            // no element of the stage can be hit: its default columns
+            if (!run && R.i >= 0) put_defaults(P, cp(T.stage_i) + (size_t)s * MX_SI, R.i);
+        }
+        if (run) run_stage<PREC, EXT>(T, P, s, use_grid, cnt, lds, R);
+    }
+    if (R.i >= 0) finish_ray<PREC>(P, R);
 }
 
 // Persistent kernel: each workgroup copies the LDS image (element records,
@@ -1618,6 +1887,46 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons 
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
         trace_one<PREC, EXT>(T, P, i, use_grid, cnt, lds);
+    __syncthreads();
+    if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
+        atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
+}
+
+// The same trace with the rays regrouped before every regrouping stage:
+// each workgroup takes chunks of BS consecutive photons (one per thread,
+// grid stride) and exchanges rays through `half` slot pairs of LDS placed
+// after the LDS image at byte offset xoff.
+template <int PREC, int BS, bool EXT>
+__global__ __launch_bounds__(BS) void mx_trace_regroup_kernel(MxTables T_arg, MxPhotons P_arg, int flags,
+                                                               unsigned long long *status, int xoff,
+                                                               int half, int max_act) {
+    static_assert(sizeof(MxTables) % alignof(MxPhotons) == 0, "kernel argument layout");
+    const __attribute__((address_space(4))) unsigned char *ka =
+        (const __attribute__((address_space(4))) unsigned char *)__builtin_amdgcn_kernarg_segment_ptr();
+    const MxTables &T = *(const MxTables *)((const __attribute__((address_space(4))) MxTables *)ka);
+    const MxPhotons &P = *(const MxPhotons *)((const __attribute__((address_space(4))) MxPhotons *)(ka + sizeof(MxTables)));
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned s_count[MX_NCOUNT];
+    __shared__ unsigned long long s_mask[2 * (BS / 64)];
+    if (P.n != P_arg.n || P.prob != P_arg.prob || T.n_stages != T_arg.n_stages ||
+        T.stage_i != T_arg.stage_i) {
+        if (status && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(status + MX_ST_BAD_ARGS, 1ULL);
+        return;
+    }
+    const int lds_bytes = PREC == 64 ? T.lds_bytes64 : T.lds_bytes32;
+    if (threadIdx.x < MX_NCOUNT) s_count[threadIdx.x] = 0;
+    if (lds_bytes > 0) {
+        const uint4 *src = (const uint4 *)(PREC == 64 ? T.lds_image64 : T.lds_image32);
+        uint4 *dst = (uint4 *)lds;
+        const int nvec = (lds_bytes + 15) >> 4;
+        for (int v = threadIdx.x; v < nvec; v += BS) dst[v] = src[v];
+    }
+    __syncthreads();
+    const Counters cnt = {s_count};
+    const bool use_grid = !(flags & MX_FLAG_NO_GRID);
+    const int64_t stride = (int64_t)gridDim.x * BS;
+    for (int64_t base = (int64_t)blockIdx.x * BS; base < P.n; base += stride)
+        trace_chunk<PREC, BS, EXT>(T, P, base, use_grid, cnt, lds, s_mask, lds + xoff, half, max_act);
     __syncthreads();
     if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
@@ -1732,6 +2041,40 @@ static int check_photons(const MxPhotons *p) {
     return 0;
 }
 
+// Ray pairs exchanged per round by the regrouping kernel for these tables
+// (its exchange buffer fills the LDS left after the image, at most half a
+// workgroup of pairs); 0 when the plain kernel runs: no regrouping stage,
+// regrouping not asked for (MX_FLAG_REGROUP), culling switched off, or room
+// for fewer than 64 pairs.  Regrouping is off by default: on the 600-facet
+// workload the kernel that regroups takes 1.49 ms against 1.13 ms (FP32),
+// and 1.46 ms even when it never exchanges a ray, so the cost is the
+// workgroup-wide barriers of the chunk structure (the waves then run in
+// lockstep phases, and in a phase only the interacting waves hide each
+// other's latency), not the exchange itself.
+static int regroup_pairs(const MxTables *t, int32_t precision, int32_t flags) {
+    if (t->regroup_stages <= 0 || !(flags & MX_FLAG_REGROUP) || (flags & MX_FLAG_NO_GRID)) return 0;
+    const bool ext = t->ext_stages != 0;
+    const int block = ext ? (precision == MX_PREC_FP64 ? MX_BLOCK_EXT : MX_BLOCK_EXT32)
+                          : (precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32);
+    const int lds = precision == MX_PREC_FP64 ? t->lds_bytes64 : t->lds_bytes32;
+    const int xoff = (lds + 15) & ~15;
+    const int slot = precision == MX_PREC_FP64 ? MX_SLOT64 : MX_SLOT32;
+    const int static_lds = 1024;   // counters and wave masks, with margin
+    const int room = 160 * 1024 - static_lds - xoff;
+    int half = room > 0 ? room / (2 * slot) : 0;
+    if (half > block / 2) half = block / 2;
+    return half >= 64 ? half : 0;
+}
+
+extern "C" int mx_trace_regroup_pairs(const MxTables *tables, int32_t precision, int32_t flags) {
+    if (check_tables(tables)) return -1;
+    if (precision != MX_PREC_FP64 && precision != MX_PREC_FP32) {
+        set_error("precision must be 64 or 32");
+        return -1;
+    }
+    return regroup_pairs(tables, precision, flags);
+}
+
 extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_t precision,
                         int32_t flags, void *stream, unsigned long long *status) {
     if (check_tables(tables) || check_photons(photons)) return 1;
@@ -1759,6 +2102,36 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
     int64_t grid = (int64_t)cus * per_cu;
     if (grid > nblk) grid = nblk;
+    // regrouping: the exchange buffer after the LDS image, at least 64 ray
+    // pairs (else the plain kernel runs)
+    const int half = regroup_pairs(tables, precision, flags);
+    if (half > 0) {
+        {
+            const int xoff = (lds + 15) & ~15;
+            const int slot = precision == MX_PREC_FP64 ? MX_SLOT64 : MX_SLOT32;
+            const int dyn = xoff + 2 * half * slot;
+            // bits 16..23: exchange only when at most this many 256ths of a
+            // workgroup's rays interact (0: the default)
+            const int frac = (flags >> 16) & 0xff;
+            const int max_act = (frac ? frac : MX_REGROUP_DEFAULT_FRAC) * block / 256;
+#define MX_LAUNCH_REGROUP(PR, BSZ, E)                                                                  \
+    hipLaunchKernelGGL((mx_trace_regroup_kernel<PR, BSZ, E>), dim3((unsigned)grid), dim3(block), dyn, st,  \
+                       *tables, *photons, (int)flags, status, xoff, half, max_act)
+            if (precision == MX_PREC_FP64 && ext) MX_LAUNCH_REGROUP(64, MX_BLOCK_EXT, true);
+            else if (precision == MX_PREC_FP64) MX_LAUNCH_REGROUP(64, MX_BLOCK64, false);
+            else if (ext) MX_LAUNCH_REGROUP(32, MX_BLOCK_EXT32, true);
+            else MX_LAUNCH_REGROUP(32, MX_BLOCK32, false);
+#undef MX_LAUNCH_REGROUP
+            hipError_t err = hipGetLastError();
+            if (err != hipSuccess) {
+                char buf[256];
+                snprintf(buf, sizeof(buf), "kernel launch failed: %s", hipGetErrorString(err));
+                set_error(buf);
+                return 2;
+            }
+            return 0;
+        }
+    }
     if (precision == MX_PREC_FP64 && ext)
         hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK_EXT, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);

@@ -357,8 +357,12 @@ class CircleAperture(FlatAperture):
 class PerfectLens(FlatOpticalElement):
     """Redirect every ray to ``center - d_off e_z + f * dir`` (``mirror.py:11-82``).
 
-    Only the default reflectivity (1) is compiled; a custom
-    ``reflectivity_interpolator`` is rejected at compile time.
+    ``reflectivity_interpolator(energy, angle, grid=False)`` gives the
+    probability of one reflection; the photon's probability is multiplied by
+    its square at a quarter of the angle between the old and new directions
+    (default: 1).  The trace compiles degree-1 tensor splines (scipy's
+    ``RectBivariateSpline`` with ``kx=ky=1``, as in the reference's tests);
+    other callables raise ``CompileError``.
     """
     loc_coos_name = ['mirror_x', 'mirror_y']
 

@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from .compiler import (compile_tree, flatten, CompileError, ORDER_NONE, STAGE_STACK,
-                       STAGE_CYLINDER)
+                       STAGE_CYLINDER, STAGE_GRATING, STAGE_DETECTOR)
 from .simulator import BaseContainer, Sequence
 from . import random as mxrandom
 
@@ -32,7 +32,7 @@ __all__ = ['DeviceTables', 'DevicePhotons', 'CompiledInstrument', 'trace',
            'TraceStatus']
 
 (ST_PROB_RANGE, ST_ORDER_EMPTY, ST_SLOT_REUSE, ST_GRID_FALLBACK, ST_MULTI_HIT, ST_BAD,
- ST_TRANS_RANGE) = range(7)
+ ST_TRANS_RANGE, ST_TESTS) = range(8)
 
 PREC = {64: 64, 32: 32, 'fp64': 64, 'fp32': 32}
 
@@ -75,6 +75,9 @@ class DeviceTables:
             setattr(s, name, k[name].data_ptr())
         s.first_draw_slot = self.host.first_draw_slot
         s.ext_stages = int(any(st.kind in (STAGE_STACK, STAGE_CYLINDER) for st in self.host.stages))
+        s.regroup_stages = int(sum(1 for k in range(len(self.host.stages))
+                                   if self.host.stage_i[k, 0] in (STAGE_GRATING, STAGE_DETECTOR, STAGE_STACK)
+                                   and self.host.stage_i[k, 3] & 1))
         return s
 
     def struct_for(self, existing):
@@ -160,14 +163,20 @@ class CompiledInstrument:
         Build culling grids for array stages (exact; disable to test every
         element).
     device : torch device (default: current CUDA device)
+    regroup : bool
+        Regroup the rays of a workgroup before every culled array stage so
+        that waves are all interacting or all passing (identical results;
+        off by default: slower on the workloads measured, because of the
+        workgroup barriers it needs).
     """
 
-    def __init__(self, tree, n_draws=2, grid=True, device=None, **grid_kw):
+    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, **grid_kw):
         self.tree = tree
         self.tables = compile_tree(tree, n_draws=n_draws, grid=grid, **grid_kw)
         self.device = _dev(device)
         self.dev_tables = DeviceTables(self.tables, self.device)
         self.grid = grid
+        self.regroup = regroup
         # persistent-grid size per precision (workgroups per CU, 0 = library
         # default); marxs_amd.autotune.tune_trace sets it from measurements
         self.wg_per_cu = {}
@@ -194,11 +203,13 @@ class CompiledInstrument:
                           generator=generator or mxrandom.generator(self.device))
 
     def launch(self, dp, uniforms=None, precision=64, status=None, stream=None,
-               existing=(), grid=None, src=None, wg_per_cu=None):
+               existing=(), grid=None, src=None, wg_per_cu=None, regroup=None,
+               regroup_max_frac=None):
         """Launch the trace kernel on device photons (asynchronous).
 
         ``src``: optional DevicePhotons to read pos/dir/pol/prob from (left
         unchanged); results go to ``dp``.  Without it the trace is in place.
+        ``regroup``: override of the instrument's ``regroup`` setting.
         """
         prec = PREC[precision]
         if status is None:
@@ -211,12 +222,29 @@ class CompiledInstrument:
             stream = torch.cuda.current_stream(self.device).cuda_stream
         use_grid = self.grid if grid is None else grid
         flags = 0 if use_grid else 1
+        if self.regroup if regroup is None else regroup:
+            flags |= 2
+        if regroup_max_frac is not None:
+            # regroup only when at most this fraction of a workgroup's rays
+            # can interact with the stage
+            flags |= (max(1, min(255, int(round(regroup_max_frac * 256)))) & 0xff) << 16
         wpc = self.wg_per_cu.get(prec, 0) if wg_per_cu is None else wg_per_cu
         flags |= (int(wpc) & 0xff) << 8
         _lib.check(_lib.lib.mx_trace(ctypes.byref(tables), ctypes.byref(P), prec, flags,
                                      ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
         dp.has_pos = True
         return status
+
+    def regroup_pairs(self, precision=64, grid=None, regroup=None):
+        """Ray pairs per exchange round of the regrouping kernel that a launch
+        with these settings runs (0: the kernel without regrouping)."""
+        flags = 0 if (self.grid if grid is None else grid) else 1
+        if self.regroup if regroup is None else regroup:
+            flags |= 2
+        n = _lib.lib.mx_trace_regroup_pairs(ctypes.byref(self.dev_tables.struct), PREC[precision], flags)
+        if n < 0:
+            _lib.check(1)
+        return n
 
     def photons_struct(self, dp, uniforms=None, src=None):
         """Check the photon columns against the tables, allocate missing
@@ -268,7 +296,7 @@ class CompiledInstrument:
         return P
 
     def trace_device(self, dp, uniforms=None, precision=64, generator=None, existing=(),
-                     grid=None):
+                     grid=None, regroup=None):
         """Trace device photons in place; returns the status counters."""
         for name in self.tables.required_icols:
             # e.g. a QualityFactor outside a stack reads photons['order']
@@ -277,12 +305,13 @@ class CompiledInstrument:
                 raise KeyError(name)
         if uniforms is None and self.tables.n_uniform_slots:
             uniforms = self.draw_uniforms(dp.n, generator)
-        status = self.launch(dp, uniforms, precision, existing=existing, grid=grid)
+        status = self.launch(dp, uniforms, precision, existing=existing, grid=grid, regroup=regroup)
         st = status.cpu().numpy()
         raise_for_status(st)
         return TraceStatus(prob_range=int(st[ST_PROB_RANGE]), order_empty=int(st[ST_ORDER_EMPTY]),
                            slot_reuse=int(st[ST_SLOT_REUSE]), grid_fallback=int(st[ST_GRID_FALLBACK]),
-                           multi_hit=int(st[ST_MULTI_HIT]), trans_range=int(st[ST_TRANS_RANGE]))
+                           multi_hit=int(st[ST_MULTI_HIT]), trans_range=int(st[ST_TRANS_RANGE]),
+                           tests=int(st[ST_TESTS]))
 
     # ------------------------------------------------------------- table
     def __call__(self, photons, uniforms=None, precision=64, generator=None, grid=None):

@@ -113,6 +113,9 @@ def describe(tree):
             d['kind'] = 'lens_perfect'
             d['focallength'] = e.focallength
             d['d_center_optax'] = e.d_center_optax
+            # the callable itself: the reference evaluates it (mirror.py:78-80)
+            if getattr(e, 'reflectivity_interpolator', None) is not None:
+                d['reflectivity'] = e.reflectivity_interpolator
         elif name == 'ThinLens':
             d['kind'] = 'lens_thin'
             d['focallength'] = e.focallength

@@ -226,6 +226,30 @@ def select_order(tab, energy, u, blaze=None):
     return order, tab['tot'][row], ok
 
 
+def select_order_loop(tab, energy, u):
+    """EfficiencyFile.__call__ as the reference writes it (grating.py:90-96):
+    one photon at a time, the nearest energy row by argmin, then the smallest
+    index whose cumulative probability exceeds the photon's draw (the
+    injected uniform in place of np.random.rand()).  Same results as
+    :func:`select_order` for a tabulated selector; the CPU baseline times
+    both."""
+    n = len(energy)
+    order = np.empty(n, dtype=np.int64)
+    tot = np.empty(n)
+    ok = np.ones(n, dtype=bool)
+    eg = tab['energy']
+    for i, e in enumerate(energy):
+        r = 0 if eg is None or len(eg) == 1 else int(np.argmin(np.abs(eg - e)))
+        nz = np.nonzero(tab['cdf'][r] > u[i])[0]
+        if len(nz) == 0:
+            ok[i] = False
+            order[i] = ORDER_NONE
+        else:
+            order[i] = tab['orders'][np.min(nz)]
+        tot[i] = tab['tot'][r]
+    return order, tot, ok
+
+
 # ----------------------------------------------------------------- elements
 def groove_frame(fr, groove_angle):
     """FlatGrating.e_groove_coos, grating.py:186-207."""
@@ -294,15 +318,26 @@ def aperture_points(el, fr, u0, u1):
     return x, y, q
 
 
-def perfect_lens(el, fr, dir, q, pol):
-    """PerfectLens.specific_process_photons, mirror.py:53-82 (reflectivity 1)."""
+def perfect_lens(el, fr, dir, q, pol, energy=None):
+    """PerfectLens.specific_process_photons, mirror.py:53-82.  Returns the
+    new direction, polarization and probability factor: the element's
+    ``reflectivity`` callable (the reference's ``reflectivity_interpolator``,
+    called as the reference calls it, with ``grid=False``) at the energy and
+    a quarter of the angle between the new and the old direction, squared;
+    1 without one."""
     pos4d = np.asarray(el['pos4d'], dtype=float)
     e_z4 = pos4d[:, 2] / np.linalg.norm(pos4d[:, 2])
     popt = (pos4d[:, 3] - el.get('d_center_optax', 0.) * e_z4)[:3]
     dn = unit(dir)
     fp = popt[None, :] + el['focallength'] * dn
     new = unit(fp - q)
-    return new, parallel_transport(dir, new, pol)
+    refl = el.get('reflectivity')
+    if refl is None:
+        prob = np.ones(len(new))
+    else:
+        angle = np.arccos(np.abs(new[:, 0] * dn[:, 0] + new[:, 1] * dn[:, 1] + new[:, 2] * dn[:, 2]))
+        prob = np.asarray(refl(energy, angle / 4, grid=False), dtype=float) ** 2
+    return new, parallel_transport(dir, new, pol), prob
 
 
 def thin_lens(el, fr, dir, q, pol):
@@ -671,12 +706,14 @@ def stack_layers(el, fr, ph, hit, ly, uniforms, base, nslot, k, cols_out):
             raise ValueError(op)
 
 
-def trace(elements, photons, uniforms, n_draws=2):
+def trace(elements, photons, uniforms, n_draws=2, selector_loop=False):
     """Element-major trace (simulator.py:42-49 + optics/base.py:144-211).
 
     Returns a new dict of columns.  Raises ValueError where the reference
     raises (probability factor outside [0, 1]; EfficiencyFile without a
-    valid order).
+    valid order).  ``selector_loop``: tabulated order selectors run one
+    photon at a time as the reference's EfficiencyFile does
+    (:func:`select_order_loop`; identical results, for timing).
     """
     ph = {k: np.array(v, dtype=float) for k, v in photons.items() if k in
           ('pos', 'dir', 'pol', 'energy', 'prob')}
@@ -733,7 +770,10 @@ def trace(elements, photons, uniforms, n_draws=2):
                 u = uniforms[base + slot, np.nonzero(hit)[0]]
                 tab = selector_table(el['selector'])
                 blz = blaze_angle(el, fr, d_old, ly) if tab.get('type') == 'interp' else None
-                order, ptot, ok = select_order(tab, ph['energy'][hit], u, blz)
+                if selector_loop and tab.get('type') != 'interp':
+                    order, ptot, ok = select_order_loop(tab, ph['energy'][hit], u)
+                else:
+                    order, ptot, ok = select_order(tab, ph['energy'][hit], u, blz)
                 if not ok.all():
                     raise ValueError('no order selected')
                 new, blaze = diffract(el, fr, d_old, ph['energy'][hit], ly, order, ptot)
@@ -743,9 +783,11 @@ def trace(elements, photons, uniforms, n_draws=2):
                 out['order'] = order
                 out['blaze'] = blaze
             elif kind == 'lens_perfect':
-                new, pol = perfect_lens(el, fr, d_old, q, pol_old)
+                new, pol, pf = perfect_lens(el, fr, d_old, q, pol_old, ph['energy'][hit])
                 out['dir'] = new
                 out['polarization'] = pol
+                if el.get('reflectivity') is not None:
+                    out['probability'] = pf
             elif kind == 'lens_thin':
                 new, pol = thin_lens(el, fr, d_old, q, pol_old)
                 out['dir'] = new

@@ -24,23 +24,40 @@ def _ci(inst, **kw):
     return CompiledInstrument(inst, **kw)
 
 
-def _parity(inst, n, seed, energies='uniform', grid=True, n_draws=2):
+def _parity(inst, n, seed, energies='uniform', grid=True, n_draws=2, exempt=True, max_near_frac=1e-3):
     ci = _ci(inst, grid=grid, n_draws=n_draws)
     ph, u = make_inputs(n, seed, ci.n_uniform_slots, energies)
     ref = oracle_trace(inst, ph, u, n_draws=n_draws)
     got, st = gpu_trace(ci, ph, u, precision=64, grid=grid)
-    rep = compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    rep = compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref) if exempt else None,
+                  max_near_frac=max_near_frac)
     return ref, got, st, rep
 
 
 @pytest.mark.parametrize('grid', [True, False])
 def test_flat_grid_fp64_parity(gpu, grid):
-    """121 coplanar FlatGrating facets (shared edges), one 8192^2 detector."""
-    ref, got, st, rep = _parity(configs.flat_grid_instrument(), 20000, 1, grid=grid)
+    """121 coplanar FlatGrating facets (shared edges), one 8192^2 detector.
+    The zero order (half the photons) focuses exactly onto the corner of four
+    pixels, where the pixel index is decided by the last bit of the detector
+    coordinate: those photons' pixel indices may differ (they are counted;
+    0.17 % of the photons were measured, the bound here is 0.5 %).  Every
+    other integer is exact and every float within 1e-9 relative."""
+    ref, got, st, rep = _parity(configs.flat_grid_instrument(), 20000, 1, grid=grid,
+                                max_near_frac=5e-3)
     assert np.isfinite(got['det_x']).sum() > 10000
-    # zeroth order focuses onto a pixel corner: pixel index is a near-boundary
-    # decision there; everything else is exact
     assert rep['facet'][1:] == (0, 0) and rep['order'][1:] == (0, 0)
+    assert rep['pix_x'][1] == 0 and rep['pix_y'][1] == 0
+
+
+@pytest.mark.parametrize('grid', [True, False])
+def test_flat_grid_shifted_detector_exact_pixels(gpu, grid):
+    """The same with the detector moved by a third of a pixel in y and z, so
+    that the zero order lands inside a pixel: every pixel index must equal
+    the oracle's with no exemption (detector.py:59-75 pixel convention)."""
+    ref, got, st, rep = _parity(configs.flat_grid_instrument(det_shift=0.024 / 3), 20000, 1,
+                                grid=grid, exempt=False)
+    assert (got['pix_x'] >= 0).sum() > 10000
+    assert rep['pix_x'][1:] == (0, 0) and rep['pix_y'][1:] == (0, 0)
 
 
 @pytest.mark.parametrize('grid', [True, False])
@@ -242,14 +259,17 @@ def test_cat_order_convention_table(gpu):
 
 @pytest.mark.parametrize('name', ['flat_grid_1000', 'cat_rowland_200_2000'])
 def test_golden_vectors(gpu, name):
-    """The FP64 HIP path against the committed oracle vectors."""
+    """The FP64 HIP path against the committed oracle vectors (the flat
+    grid's zero order on a pixel corner: pixel indices there may differ,
+    counted, at most 0.5 % of the photons)."""
     from helpers import load_golden
     from marxs_amd import configs
     meta, els, ph, u, out = load_golden(name)
     inst = getattr(configs, meta['builder'])(**meta['kwargs'])
     inst = inst[0] if isinstance(inst, tuple) else inst
     got, st = gpu_trace(inst, ph, u, precision=64)
-    compare(out, got, rtol=1e-9, near=near_pixel_boundary(out))
+    compare(out, got, rtol=1e-9, near=near_pixel_boundary(out),
+            max_near_frac=5e-3 if name.startswith('flat_grid') else 1e-3)
 
 
 @pytest.mark.parametrize('grid', ['even', 'uneven', 'two'])
@@ -357,3 +377,62 @@ def test_cat_rowland_blaze_dependent_table_fp32(gpu):
     print(f'FP32 interp: order agreement {agree:.6f}, rms {np.sqrt(np.mean(err ** 2)):.4f} um')
     assert same.sum() > 5000 and agree >= 0.999
     assert both.sum() > 1000 and np.sqrt(np.mean(err ** 2)) <= 0.5
+
+
+# ------------------------------------------------------- lens reflectivity
+def _refl_spline(f, x, y):
+    from scipy.interpolate import RectBivariateSpline
+    xg, yg = np.meshgrid(x, y, indexing='ij')
+    return RectBivariateSpline(x, y, f(xg, yg), kx=1, ky=1)
+
+
+def test_perfect_lens_reflectivity_known_answer(gpu):
+    """marxs/optics/tests/test_mirror.py:62-80 through the GPU: probability
+    0.367205 for the reference's test photon and spline."""
+    import marxs_amd as mx
+    interp = _refl_spline(lambda x, y: np.exp(-np.sqrt((x / 2) ** 2 + y ** 2)),
+                          np.linspace(-3, 3, 100), np.linspace(-3, 3, 100))
+    lens = mx.PerfectLens(focallength=123.0, reflectivity_interpolator=interp)
+    ph = {'pos': np.array([[1., 0, 0]]), 'dir': np.array([[-1., 0, 0]]),
+          'pol': np.array([[0., 1, 0]]), 'energy': np.ones(1), 'prob': np.ones(1)}
+    for prec in (64, 32):
+        got, st = gpu_trace(mx.Sequence(elements=[lens]), ph, np.zeros((0, 1)), precision=prec)
+        assert np.allclose(got['energy'], 1)
+        assert np.allclose(got['pol'], [0, 1, 0])
+        assert np.allclose(got['prob'], 0.367205)
+
+
+def test_perfect_lens_reflectivity_parity(gpu):
+    """A tabulated reflectivity (energy 0.2-2.5 keV x angle 0-0.02 rad) on the
+    200-facet workload's lens: FP64 equal to the oracle, which calls scipy's
+    RectBivariateSpline itself (indices exact, floats 1e-9 relative); FP32
+    probabilities within 1e-5 relative."""
+    from marxs_amd.tracer import CompiledInstrument
+    rng = np.random.default_rng(12)
+    e, a = np.round(np.arange(0.2, 2.5001, 0.1), 10), np.linspace(0., 0.02, 21)
+    vals = (0.95 - 0.1 * e[:, None] - 8. * a[None, :]) * (1 - 0.02 * rng.random((len(e), len(a))))
+    from scipy.interpolate import RectBivariateSpline
+    interp = RectBivariateSpline(e, a, vals, kx=1, ky=1)
+    inst, parts = configs.cat_rowland_instrument(200)
+    parts['lens'].reflectivity_interpolator = interp
+    ci = _ci(inst)
+    ph, u = make_inputs(30000, 13, ci.n_uniform_slots)
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(ci, ph, u, precision=64)
+    compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    assert ref['prob'].max() < 0.95 and (ref['prob'] > 0).sum() > 1000
+    g32, _ = gpu_trace(ci, ph, u, precision=32)
+    ok = (ref['facet'] == g32['facet']) & (ref['CCD'] == g32['CCD'])
+    rel = np.abs(g32['prob'] - ref['prob'])[ok] / np.maximum(ref['prob'][ok], 1e-300)
+    assert float(rel.max()) < 1e-5
+
+
+def test_perfect_lens_reflectivity_rejected_forms(gpu):
+    import marxs_amd as mx
+    from marxs_amd.compiler import CompileError
+    from scipy.interpolate import RectBivariateSpline
+    x = np.linspace(0, 1, 5)
+    for bad in (lambda e, a, grid=False: np.ones_like(e),
+                RectBivariateSpline(x, x, np.ones((5, 5)), kx=2, ky=2)):
+        with pytest.raises(CompileError):
+            _ci(mx.Sequence(elements=[mx.PerfectLens(focallength=10., reflectivity_interpolator=bad)]))

@@ -352,6 +352,54 @@ def test_perfect_lens_offset():
     assert not np.allclose(r1['dir'], r3['dir'])
 
 
+def _refl_spline(f, x=None, y=None):
+    from scipy.interpolate import RectBivariateSpline
+    x = np.linspace(-3, 3, 100) if x is None else x
+    y = np.linspace(-3, 3, 100) if y is None else y
+    xg, yg = np.meshgrid(x, y, indexing='ij')
+    return RectBivariateSpline(x, y, f(xg, yg), kx=1, ky=1)
+
+
+def test_perfect_lens_reflectivity_known_answer():
+    """test_mirror.py:62-80: exp(-sqrt((E/2)^2 + a^2)) as a degree-1 spline
+    on [-3, 3]^2, one test photon (1 keV, along -x through the centre, so the
+    angle is 0): probability 0.367205."""
+    from marxs_amd.base import parse_position_keywords
+    interp = _refl_spline(lambda x, y: np.exp(-np.sqrt((x / 2) ** 2 + y ** 2)))
+    ph = {'pos': np.array([[1., 0, 0]]), 'dir': np.array([[-1., 0, 0]]),
+          'pol': np.array([[0., 1, 0]]), 'energy': np.ones(1), 'prob': np.ones(1)}
+    el = {'kind': 'lens_perfect', 'pos4d': parse_position_keywords({}), 'focallength': 123.,
+          'reflectivity': interp}
+    res = OT.trace([el], ph, np.zeros((0, 1)))
+    assert np.allclose(res['energy'], 1)
+    assert np.allclose(res['pol'], [0, 1, 0])
+    assert np.allclose(res['prob'], 0.367205)
+
+
+def test_perfect_lens_reflectivity_angle_quarter_squared():
+    """Closed form: rays parallel to the axis at height r turn by
+    atan(r / f) towards the focus; a reflectivity linear in the angle (which
+    a degree-1 spline represents exactly) gives probability
+    (1 - 2 atan(r / f) / 4)^2 times the energy factor."""
+    from marxs_amd.base import parse_position_keywords
+    f = 1000.
+    r = np.array([0., 10., 50., 100., 200.])
+    n = len(r)
+    interp = _refl_spline(lambda e, a: (1 - 0.1 * e) * (1 - 2 * a),
+                          x=np.linspace(0., 5., 11), y=np.linspace(0., 0.5, 6))
+    e = np.array([0.5, 1., 1.5, 2., 4.])
+    ph = {'pos': np.column_stack([np.full(n, 10.), r, np.zeros(n)]),
+          'dir': np.tile([-1., 0, 0], (n, 1)), 'pol': np.tile([0., 0, 1], (n, 1)),
+          'energy': e, 'prob': np.ones(n)}
+    el = {'kind': 'lens_perfect', 'pos4d': parse_position_keywords({'zoom': 300}), 'focallength': f,
+          'reflectivity': interp}
+    res = OT.trace([el], ph, np.zeros((0, n)))
+    ang = np.arctan(r / f)
+    assert np.allclose(res['prob'], ((1 - 0.1 * e) * (1 - 2 * ang / 4)) ** 2, rtol=1e-12)
+    # a planted mistake: the full angle instead of a quarter of it
+    assert not np.allclose(res['prob'][1:], ((1 - 0.1 * e) * (1 - 2 * ang))[1:] ** 2, rtol=1e-3)
+
+
 def test_thin_lens_paraxial_focus():
     """Thin-lens rotation by rho/f toward the axis: a ray at height rho
     crosses the axis at rho / tan(rho / f) behind the lens."""

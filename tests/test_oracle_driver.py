@@ -158,3 +158,28 @@ def test_planted_mistake_is_caught(monkeypatch, name, attr, make, checks):
     for fn in fns:
         with pytest.raises(AssertionError):
             fn()
+
+
+def test_per_photon_selector_loop_equals_vectorised():
+    """The reference's one-photon-at-a-time EfficiencyFile loop
+    (grating.py:90-96), restated for the CPU baseline, selects the same
+    orders and totals as the vectorised selector on the 200-facet workload,
+    including draws on the cumulative probabilities themselves."""
+    from marxs_amd import configs
+    from helpers import make_inputs
+    inst, parts = configs.cat_rowland_instrument(200)
+    from marxs_amd.compiler import compile_tree
+    ns = compile_tree(inst).n_uniform_slots
+    ph, u = make_inputs(3000, 21, ns)
+    els = describe(inst)
+    a = OT.trace(els, ph, u)
+    b = OT.trace(els, ph, u, selector_loop=True)
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=np.asarray(a[k]).dtype.kind == 'f'), k
+    tab = OT.selector_table(els[2]['selector'])
+    e = np.repeat(tab['energy'][:4], 15)
+    uu = np.tile(tab['cdf'][0], 4)
+    va = OT.select_order(tab, e, uu)
+    vb = OT.select_order_loop(tab, e, uu)
+    for x, y in zip(va, vb):
+        assert np.array_equal(x, y)

@@ -286,3 +286,63 @@ def test_selector_tables_without_orders_rejected():
         g = mx.FlatGrating(d=1e-3, order_selector=sel, zoom=[1, 20, 20])
         with pytest.raises(C.CompileError, match='no orders'):
             C.compile_tree(mx.Sequence(elements=[g]))
+
+
+def test_packaged_l1_table_is_the_default():
+    """The L1 transmission table ships with the package (the reference reads
+    its copy at import, catgrating.py:41) and is the default: it equals the
+    reference's data file kept as a fixture, and the default L1 gives the
+    reference's 4 um answer at 1 keV (test_cat.py:38-48: 0.22458)."""
+    e_pkg, t_pkg = CG.read_transmission_table(CG.l1transtab_path)
+    e_fix, t_fix = CG.read_transmission_table(SI)
+    assert np.array_equal(e_pkg, e_fix) and np.array_equal(t_pkg, t_fix)
+    l1 = CG.L1(order_selector=mx.OrderSelector([-5]))
+    assert np.array_equal(l1.trans_energy, e_fix)
+    assert np.isclose(l1.transfunc(1.0), 0.22458, rtol=1e-4)
+
+
+def test_lens_reflectivity_table_compiles():
+    """A degree-1 RectBivariateSpline compiles to its grids and values
+    (FITPACK's coefficients, which equal the values at the grid points);
+    other callables and higher degrees are rejected."""
+    from scipy.interpolate import RectBivariateSpline
+    x, y = np.linspace(0.2, 2.5, 7), np.linspace(0., 0.02, 5)
+    vals = np.outer(1 - 0.1 * x, 1 - 4 * y)
+    sp = RectBivariateSpline(x, y, vals, kx=1, ky=1)
+    gx, gy, c = C.reflectivity_table(sp)
+    assert np.array_equal(gx, x) and np.array_equal(gy, y) and np.allclose(c, vals, rtol=1e-15)
+    lens = mx.PerfectLens(focallength=100., reflectivity_interpolator=sp)
+    t = C.compile_tree([lens])
+    o = t.elem_i[0, 6]
+    assert o >= 0 and t.sel_d[o] == len(x) and t.sel_d[o + 1] == len(y)
+    assert np.array_equal(t.sel_d[o + 2:o + 2 + len(x)], x)
+    t0 = C.compile_tree([mx.PerfectLens(focallength=100.)])
+    assert t0.elem_i[0, 6] == -1
+    for bad in (lambda e, a, grid=False: np.ones_like(e),
+                RectBivariateSpline(x, y, vals, kx=2, ky=2)):
+        with pytest.raises(C.CompileError):
+            C.compile_tree([mx.PerfectLens(focallength=100., reflectivity_interpolator=bad)])
+
+
+def test_interpolated_table_fp64_lds_copy():
+    """The FP64 LDS image holds an interpolated table as float64, exactly
+    the values of the global tables: grids, values [order][wave][angle],
+    then the values summed over the orders, then the orders."""
+    eff = configs.blaze_interp_efficiency()
+    inst, parts = configs.cat_rowland_instrument(600, efficiency=eff)
+    t = C.compile_tree(inst)
+    k = 0
+    row = t.sel_i[k]
+    assert row[8] & 8 and row[7] >= 0 and row[11] >= 0
+    nr, no, nt = int(row[0]), int(row[1]), int(row[9])
+    img = t.lds_image64
+    n64 = nr + nt + no * nr * nt + nr * nt
+    blk = img[row[7]:row[7] + 8 * n64].view(np.float64)
+    assert np.array_equal(blk[:nr], t.sel_d[row[2]:row[2] + nr])
+    assert np.array_equal(blk[nr:nr + nt], t.sel_d[row[4]:row[4] + nt])
+    assert np.array_equal(blk[nr + nt:nr + nt + no * nr * nt], t.sel_d[row[3]:row[3] + no * nr * nt])
+    assert np.array_equal(blk[nr + nt + no * nr * nt:], t.sel_d[row[11]:row[11] + nr * nt])
+    o0 = row[7] + ((8 * n64 + 15) & ~15)
+    assert np.array_equal(img[o0:o0 + 4 * no].view(np.int32), t.sel_o[row[5]:row[5] + no])
+    # the image header row of the selector is the table's row
+    assert np.array_equal(img[:4 * C.SELI].view(np.int32), t.sel_i[0])
