@@ -28,8 +28,8 @@ from . import _lib
 from .mathutils import decompose44
 from .compiler import compile_tree, STAGE_GRATING, STAGE_DETECTOR
 
-__all__ = ['facet_geometry', 'FacetLoss', 'AdjointProblem', 'euler_sxyz_torch',
-           'central_differences']
+__all__ = ['facet_geometry', 'FacetLoss', 'AdjointProblem', 'ChunkedAdjoint', 'entrance_chunk',
+           'euler_sxyz_torch', 'central_differences']
 
 
 def _rot(axis, a):
@@ -247,7 +247,11 @@ class AdjointProblem:
     def restrict_to_traced(self, y_traced, rtol=1e-10):
         """Zero the weight of photons whose traced detector coordinate the
         one-facet replay does not reproduce (photons that met more than one
-        facet or detector in the trace).  Returns the number excluded."""
+        facet or detector in the trace).  Returns the number excluded.
+
+        The work buffers of :meth:`loss_and_grad` are rebuilt afterwards; a
+        graph captured before keeps its own buffers alive (they travel with
+        the graph) and replays the old photon set: capture again."""
         self._pb = None
         y = self.coordinates(self.geometry(self.theta0, self.period0))
         used = self.weight != 0
@@ -316,6 +320,9 @@ class AdjointProblem:
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=s):
             self.launch_loss_and_grad(precision, s.cuda_stream)
+        # the graph holds raw pointers into these buffers: keep them alive for
+        # as long as the graph, whatever later happens to self._pb
+        g._marxs_buffers = self._pb
         return g
 
 
@@ -355,16 +362,41 @@ class FacetLoss(torch.autograd.Function):
 
 def central_differences(prob, theta, period, h_t=1e-4, h_r=1e-6, h_d=1e-10, facets=None,
                         precision=64):
-    """Central finite differences of L for each facet parameter (columns:
-    dx, dy, dz, rx, ry, rz, period), with the photon paths frozen."""
+    """Central finite differences of L = mu + sigma for each facet parameter
+    (columns: dx, dy, dz, rx, ry, rz, period), with the photon paths frozen.
+
+    Each difference is formed from per-photon coordinate differences, not as
+    the difference of two loss values: moving facet f changes only its own
+    photons' coordinates, by Delta_i, and then exactly
+        mu(+-) - mu0 = d = sum_f w Delta / W,
+        sigma(+-)^2 - sigma0^2 = -d^2 + sum_f w Delta (2 (y0 - mu0) + Delta) / W,
+    so the result keeps the digits of Delta (a loss of ~500 mm differenced
+    over a step of 1e-4 mm would lose nine of them).  Only facet f's photons
+    are replayed for its steps."""
     F = theta.shape[0]
     facets = range(F) if facets is None else facets
     out = torch.full((F, 7), float('nan'), dtype=torch.float64, device=theta.device)
     steps = [h_t] * 3 + [h_r] * 3 + [h_d]
     with torch.no_grad():
+        g0 = prob.geometry(theta, period).contiguous()
+        y0 = prob.coordinates(g0, precision)
+        ok = ~torch.isnan(y0) & (prob.weight != 0)
+        w = torch.where(ok, prob.weight, torch.zeros_like(prob.weight))
+        W = w.sum()
+        mu0 = (w * torch.where(ok, y0, torch.zeros_like(y0))).sum() / W
+        var0 = (w * torch.where(ok, y0 - mu0, torch.zeros_like(y0)) ** 2).sum() / W
         for f in facets:
+            idx = torch.nonzero(ok & (prob.facet == f)).flatten()
+            if idx.numel() == 0:
+                out[f] = 0.
+                continue
+            src = {'n': int(idx.numel()), 'pos': prob.pos[:, idx].contiguous(),
+                   'dir': prob.dir[:, idx].contiguous(), 'energy': prob.energy[idx].contiguous(),
+                   'weight': w[idx].contiguous(), 'facet': prob.facet[idx].contiguous(),
+                   'order': prob.order[idx].contiguous(), 'ccd': prob.ccd[idx].contiguous()}
+            wf, yf = w[idx], y0[idx]
             for k in range(7):
-                vals = []
+                dmu, dvar = [], []
                 for sgn in (1., -1.):
                     th = theta.clone()
                     pe = period.clone()
@@ -372,6 +404,115 @@ def central_differences(prob, theta, period, h_t=1e-4, h_r=1e-6, h_d=1e-10, face
                         th[f, k] += sgn * steps[k]
                     else:
                         pe[f] += sgn * steps[k]
-                    vals.append(FacetLoss.apply(prob.geometry(th, pe), prob, precision))
-                out[f, k] = (vals[0] - vals[1]) / (2 * steps[k])
+                    g = prob.geometry(th, pe).contiguous()
+                    y = torch.empty(src['n'], dtype=torch.float64, device=prob.device)
+                    _launch(prob._args(g, precision, y_out=y, src=src),
+                            torch.cuda.current_stream(prob.device).cuda_stream)
+                    delta = y - yf
+                    d = (wf * delta).sum() / W
+                    dmu.append(d)
+                    dvar.append(-d * d + (wf * delta * (2 * (yf - mu0) + delta)).sum() / W)
+                sp = torch.sqrt(var0 + dvar[0])
+                sm = torch.sqrt(var0 + dvar[1])
+                dL = (dmu[0] - dmu[1]) + (dvar[0] - dvar[1]) / (sp + sm)
+                out[f, k] = dL / (2 * steps[k])
     return out
+
+
+def entrance_chunk(ci, front, src, uniforms, weight_fn, precision=64):
+    """Adjoint inputs for one chunk of photons, traced on demand: ``ci``
+    traces the whole instrument (frozen facet, order and CCD choices, the
+    traced coordinate), ``front`` the elements before the grating array
+    (the photons as they reach it), both from the same ``uniforms``.
+    ``weight_fn(traced DevicePhotons)`` gives each photon's weight in the
+    loss (0 excludes it).  Returns the dict :class:`ChunkedAdjoint` takes
+    (``y_traced``: all three traced position components)."""
+    from .tracer import DevicePhotons
+    full = DevicePhotons(src.dir, src.pol, src.energy, src.prob, device=src.device)
+    ci.trace_device(full, uniforms=uniforms, precision=precision)
+    ent = DevicePhotons(src.dir, src.pol, src.energy, src.prob, device=src.device)
+    front.trace_device(ent, uniforms=uniforms[:front.n_uniform_slots].contiguous(), precision=precision)
+    return {'pos': ent.pos, 'dir': ent.dir, 'energy': ent.energy, 'weight': weight_fn(full),
+            'facet': full.icols['facet'], 'order': full.icols['order'], 'ccd': full.icols['CCD'],
+            'y_traced': full.pos}
+
+
+class ChunkedAdjoint:
+    """L = centroid + RMS width and its gradient over more photons than are
+    held at once: the photons come in chunks that ``chunk(k)`` recomputes on
+    demand (e.g. :func:`entrance_chunk` tracing chunk k's photons) and that
+    are replayed one after another into the same per-facet sums A, B and
+    moments; the pose chain rule runs once at the end.  Only one chunk's
+    entrance state (76 bytes per photon) lives at a time.  The sums are
+    those of the one-pass :meth:`AdjointProblem.launch_loss_and_grad`, so
+    the result equals it over the union of the chunks up to the order of
+    the additions.
+
+    ``chunk(k)`` returns a dict of device tensors pos, dir (3, n), energy,
+    weight (n), facet, order, ccd (int32, n) and optionally ``y_traced``
+    ((3, n) traced positions: photons whose one-facet replay does not
+    reproduce them are excluded, as :meth:`AdjointProblem.restrict_to_traced`
+    does).  The moments are shifted by the weighted mean coordinate of chunk
+    0 at the unperturbed pose.
+    """
+
+    def __init__(self, gratings, detectors, chunk, n_chunks, coord=1, device=None):
+        self.chunk, self.n_chunks = chunk, int(n_chunks)
+        c0 = chunk(0)
+        self.prob = AdjointProblem(gratings, detectors, c0['pos'], c0['dir'], c0['energy'],
+                                   c0['weight'], c0['facet'], c0['order'], c0['ccd'], coord=coord,
+                                   device=device)
+        if 'y_traced' in c0:
+            self.prob.restrict_to_traced(c0['y_traced'][coord])
+        self.prob._pose_buffers()
+        self.geom0 = self.prob.geometry(self.prob.theta0, self.prob.period0).contiguous()
+        self.n_used = 0
+
+    def _compact(self, c):
+        p = self.prob
+        f64 = dict(dtype=torch.float64, device=p.device)
+        pos = c['pos'].to(**f64).contiguous()
+        dir_ = c['dir'].to(**f64).contiguous()
+        energy = c['energy'].to(**f64).contiguous()
+        weight = c['weight'].to(**f64).contiguous()
+        i32 = dict(dtype=torch.int32, device=p.device)
+        facet, order, ccd = (c[k].to(**i32).contiguous() for k in ('facet', 'order', 'ccd'))
+        use = (weight != 0) & (facet >= 0) & (ccd >= 0)
+        if 'y_traced' in c:
+            # the unperturbed one-facet replay must reproduce the trace
+            y = torch.empty(pos.shape[1], dtype=torch.float64, device=p.device)
+            whole = {'n': pos.shape[1], 'pos': pos, 'dir': dir_, 'energy': energy, 'weight': weight,
+                     'facet': facet, 'order': order, 'ccd': ccd}
+            _launch(p._args(self.geom0, 64, y_out=y, src=whole),
+                    torch.cuda.current_stream(p.device).cuda_stream)
+            yt = c['y_traced'][p.coord].to(**f64)
+            use = use & (torch.abs(y - yt) <= 1e-10 * torch.abs(yt).clamp(min=1.0))
+        idx = torch.nonzero(use).flatten()
+        return {'n': int(idx.numel()), 'pos': pos[:, idx].contiguous(), 'dir': dir_[:, idx].contiguous(),
+                'energy': energy[idx].contiguous(), 'weight': weight[idx].contiguous(),
+                'facet': facet[idx].contiguous(), 'order': order[idx].contiguous(),
+                'ccd': ccd[idx].contiguous()}
+
+    def loss_and_grad(self, theta, period, precision=32):
+        """(L, dL/dtheta (F, 6), dL/dperiod (F,)) over all chunks."""
+        p = self.prob
+        pb = p._pose_buffers()
+        stream = torch.cuda.current_stream(p.device).cuda_stream
+        pb['theta'].copy_(theta)
+        pb['period'].copy_(period)
+        pb['accum'].zero_()
+        _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(p._pose_args(0)), ctypes.c_void_p(stream)))
+        self.n_used = 0
+        for k in range(self.n_chunks):
+            src = self._compact(self.chunk(k))
+            self.n_used += src['n']
+            if src['n'] == 0:
+                continue
+            one = p._args(pb['geom'], precision, grad=pb['accum'], moments=pb['moments'], src=src)
+            one.flags = _lib.MX_AJ_SUMS
+            one.lds_accumulate = int(p.F <= 315)
+            one.y_shift = p.y_shift
+            _launch(one, stream)
+            del src
+        _lib.check(_lib.lib.mx_facet_pose(ctypes.byref(p._pose_args(2)), ctypes.c_void_p(stream)))
+        return pb['loss'][0].clone(), pb['grad7'][:, :6].clone(), pb['grad7'][:, 6].clone()

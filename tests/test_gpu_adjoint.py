@@ -1,11 +1,15 @@
 """Adjoint of the detector line centroid + width w.r.t. facet poses and
-periods (needs an MI355X).
+periods (needs an MI355X), on the differentiation workload: 200-facet
+Rowland array with N(0, 0.1 mm) / N(0, 30 arcsec) facet perturbations,
+1e6 photons at 0.65 keV, loss = centroid + RMS width of order -6.
 
 Checks: the replay coordinate equals the parity-tested trace; the FP64
-adjoint agrees with central finite differences (frozen paths) to <= 1e-4
-relative; the FP32 adjoint agrees with the FP64 adjoint to within the
-relative error its FP32 detector coordinates put into the seeds
-(rms(y32 - y64) / sigma, at least 1e-4)."""
+adjoint agrees entrywise with central finite differences (frozen paths) to
+<= 1e-4 relative on every facet parameter whose gradient exceeds 1e-3 of
+its column's largest; the FP32 adjoint agrees with the FP64 adjoint to
+within the relative error its FP32 detector coordinates put into the seeds
+(rms(y32 - y64) / sigma, at least 1e-4); the chunked replay (photons
+traced chunk by chunk, never held together) equals the one-shot gradient."""
 import numpy as np
 import pytest
 
@@ -22,7 +26,7 @@ def problem(gpu):
     inst, parts = configs.cat_rowland_instrument(200)
     tolerancing.wiggle(parts['gratings'], dx=0.1, dy=0.1, dz=0.1, rx=1.4544e-4, ry=1.4544e-4,
                        rz=1.4544e-4, rng=np.random.default_rng(5))
-    n = 200_000
+    n = 1_000_000
     rng = np.random.default_rng(6)
     ci = CompiledInstrument(inst)
     u = rng.random((ci.n_uniform_slots + 1, n))
@@ -44,6 +48,7 @@ def problem(gpu):
     prob.n_selected = int(sel.sum())
     prob.n_dropped = prob.restrict_to_traced(full.pos[1])
     sel = sel & (prob.weight != 0)
+    prob.inputs = (inst, parts, ph, uni)
     return prob, full, sel
 
 
@@ -64,6 +69,8 @@ def test_replay_matches_trace(problem):
 
 
 def test_adjoint_vs_finite_differences(problem):
+    """Every facet parameter whose gradient is at least 1e-3 of its column's
+    largest: |adjoint - central difference| <= 1e-4 |central difference|."""
     import torch
     prob, full, sel = problem
     th = prob.theta0.clone().requires_grad_(True)
@@ -71,18 +78,17 @@ def test_adjoint_vs_finite_differences(problem):
     L = prob.loss(th, pe, precision=64)
     L.backward()
     g = torch.cat([th.grad, pe.grad[:, None]], dim=1)
-    # the facets carrying most of the selected photons
-    counts = torch.bincount(prob.facet[sel].long(), minlength=prob.F)
-    top = torch.argsort(counts, descending=True)[:6].tolist()
     from marxs_amd.adjoint import central_differences
-    fd = central_differences(prob, prob.theta0, prob.period0, facets=top)
+    fd = central_differences(prob, prob.theta0, prob.period0)
+    n_checked = 0
     for k, name in enumerate(['dx', 'dy', 'dz', 'rx', 'ry', 'rz', 'period']):
-        a = g[top, k]
-        b = fd[top, k]
-        scale = torch.abs(g[:, k]).max()
-        err = float((torch.abs(a - b) / scale).max())
-        print(name, float(scale), err)
-        assert err <= 1e-4, (name, err)
+        big = torch.abs(g[:, k]) >= 1e-3 * torch.abs(g[:, k]).max()
+        a, b = g[big, k], fd[big, k]
+        err = torch.abs(a - b) / torch.abs(b)
+        n_checked += int(big.sum())
+        print(name, 'checked', int(big.sum()), 'max rel', float(err.max()))
+        assert float(err.max()) <= 1e-4, (name, float(err.max()))
+    assert n_checked >= 7 * 150
 
 
 def test_fp32_adjoint_matches_fp64(problem):
@@ -177,3 +183,43 @@ def test_fused_graph_replays_follow_the_parameters(problem):
         if rep:
             assert not torch.equal(got[1], prev)
         prev = got[1]
+
+
+def test_chunked_replay_equals_one_shot(problem):
+    """The same 1e6 photons traced and replayed in four chunks of 2.5e5
+    (ChunkedAdjoint over entrance_chunk: each chunk's entrance state and
+    frozen choices recomputed by a trace, then accumulated) give the one-shot
+    loss and gradient (the sums differ only in the order of the additions)."""
+    import torch
+    import marxs_amd as mx
+    from marxs_amd.adjoint import ChunkedAdjoint, entrance_chunk
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    prob, full, sel = problem
+    inst, parts, ph, uni = prob.inputs
+    ci = CompiledInstrument(inst)
+    front = CompiledInstrument(mx.Sequence(elements=[parts['aperture'], parts['lens']]))
+    n, m = len(ph['energy']), 250_000
+
+    def chunk(k):
+        s = slice(k * m, (k + 1) * m)
+        src = DevicePhotons(ph['dir'][s].T, ph['pol'][s].T, ph['energy'][s], ph['prob'][s],
+                            device=prob.device)
+
+        def weight(t):
+            return torch.where((t.icols['order'] == -6) & (t.icols['CCD'] >= 0), t.prob,
+                               torch.zeros_like(t.prob))
+        return entrance_chunk(ci, front, src, uni[:, s].contiguous(), weight)
+
+    ca = ChunkedAdjoint(parts['gratings'], parts['ccds'], chunk, n // m, coord=1)
+    rng = torch.Generator(device=prob.device).manual_seed(4)
+    th = prob.theta0 + 1e-4 * torch.randn(prob.theta0.shape, generator=rng, dtype=torch.float64,
+                                          device=prob.device)
+    for prec in (64, 32):
+        Lc, gtc, gpc = ca.loss_and_grad(th, prob.period0, precision=prec)
+        Lo, gto, gpo = prob.loss_and_grad(th, prob.period0, precision=prec)
+        assert ca.n_used == int((prob.weight != 0).sum())
+        assert abs(float(Lc - Lo)) <= 1e-12 * abs(float(Lo))
+        for a, b in ((gto, gtc), (gpo[:, None], gpc[:, None])):
+            for k in range(a.shape[1]):
+                scale = float(torch.abs(a[:, k]).max())
+                assert float(torch.abs(a[:, k] - b[:, k]).max()) <= 1e-10 * scale, (prec, k)

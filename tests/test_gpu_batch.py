@@ -220,3 +220,93 @@ def test_run_tolerances_batched_with_device_figures_of_merit(gpu):
         assert np.array_equal(a['Aeff'], b['Aeff'])
         assert np.array_equal(a['R'], b['R'], equal_nan=True)
     assert np.isfinite(dev[0]['R']).sum() >= 3
+
+
+# --------------------------------------------------------- 64-variant study
+C5_PHOTONS = 20_000
+
+
+@pytest.fixture(scope='module')
+def c5(gpu):
+    """The batching workload's shape: 64 design variants (Rowland R
+    5000-7000 mm, blaze 1-2 deg, 50-800 facets, each with its own 16-CCD
+    row), 2e4 photons each (the workload itself uses 2e5; fewer keep the
+    test short, the tables are the workload's)."""
+    import torch
+    from marxs_amd import configs
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    variants = configs.design_variants(64, np.random.default_rng(5), facet_range=(50, 800))
+    cis, inputs, unis, hosts = [], [], [], []
+    for k, (tree, parts, pars) in enumerate(variants):
+        ci = CompiledInstrument(tree)
+        ph, u = make_inputs(C5_PHOTONS, 500 + k, ci.n_uniform_slots)
+        cis.append(ci)
+        inputs.append(DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'], device=gpu))
+        unis.append(torch.as_tensor(u, device=gpu))
+        hosts.append((tree, ph, u, pars))
+    return cis, inputs, unis, hosts
+
+
+def test_c5_batched_fp64_equals_each_single_trace(c5):
+    """One batched FP64 launch over all 64 variants (26 126 facets); every
+    variant's columns equal its own single-variant trace bit for bit, and
+    its status counters equal the single trace's
+    (run_tolerances semantics, marxs/design/tolerancing.py:275-281)."""
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = c5
+    nf = [h[3]['n_facets'] for h in hosts]
+    assert min(nf) < 100 and max(nf) > 750 and len(cis) == 64
+    bt = BatchedTrace(cis, inputs, unis, precision=64)
+    st = bt.run()
+    for v, ci in enumerate(cis):
+        dp, sst = _single(ci, inputs[v], unis[v], 64)
+        _assert_identical(_columns(dp), _columns(bt.outputs[v]))
+        assert st[v]['multi_hit'] == int(sst.cpu()[4]), v
+        assert st[v]['grid_fallback'] == int(sst.cpu()[3]), v
+
+
+def test_c5_variants_against_oracle(c5):
+    """The smallest and the largest variant, and two large variants whose
+    loose culling lists do not fit in LDS and are read from global memory,
+    against the oracle's trace of the same photons (indices exact, floats
+    1e-9 relative)."""
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = c5
+    nf = np.array([h[3]['n_facets'] for h in hosts])
+    order = np.argsort(nf)
+    spill = [int(v) for v in order[::-1] if cis[v].tables.stage_i[2, 39] < 0]
+    assert len(spill) >= 3
+    pick = sorted({int(order[0]), int(order[-1]), spill[1], spill[2]})
+    bt = BatchedTrace(cis, inputs, unis, precision=64)
+    bt.run()
+    for v in pick:
+        tree, ph, u, pars = hosts[v]
+        ref = oracle_trace(tree, ph, u)
+        got = _columns(bt.outputs[v])
+        got = {k: (x.T if x.ndim == 2 else x) for k, x in got.items()}
+        got = {k: (x.astype(np.int64) if x.dtype.kind == 'i' else x) for k, x in got.items()}
+        print('variant', v, 'facets', nf[v])
+        rep = compare(ref, got, rtol=1e-9, near=near_pixel_boundary(ref))
+        assert rep['facet'][1:] == (0, 0) and rep['order'][1:] == (0, 0) and rep['CCD'][1:] == (0, 0)
+        assert (got['facet'] >= 0).sum() > 500 and (got['CCD'] >= 0).sum() > 100
+
+
+def test_c5_batched_fp32_close_to_fp64(c5):
+    """The batched FP32 launch over the 64 variants: detector coordinates of
+    the photons both precisions put on the same facet and CCD within
+    0.5 um RMS of the FP64 launch."""
+    from marxs_amd.batch import BatchedTrace
+    cis, inputs, unis, hosts = c5
+    b64 = BatchedTrace(cis, inputs, unis, precision=64)
+    b64.run()
+    b32 = BatchedTrace(cis, inputs, unis, precision=32)
+    b32.run()
+    errs, n_same = [], 0
+    for v in range(len(cis)):
+        a, b = _columns(b64.outputs[v]), _columns(b32.outputs[v])
+        ok = (a['CCD'] >= 0) & (a['CCD'] == b['CCD']) & (a['facet'] == b['facet'])
+        n_same += int(ok.sum())
+        errs.append(np.hypot(a['det_x'] - b['det_x'], a['det_y'] - b['det_y'])[ok])
+    e = np.concatenate(errs) * 1e3
+    print('C5 FP32 vs FP64: rms', float(np.sqrt(np.mean(e ** 2))), 'um over', n_same)
+    assert n_same > 10_000 and float(np.sqrt(np.mean(e ** 2))) <= 0.5
