@@ -113,15 +113,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def read_profile_value(root, name, key):
+    """A number committed in a profiles/ summary as 'key: X' (figures that
+    come from rocprofv3 counter runs of this bench: HBM traffic per launch,
+    VALU instruction counts), or None."""
+    path = os.path.join(root, 'profiles', name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(re.escape(key) + r':\s*([0-9.eE+-]+)', open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def read_traffic(root):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by a copy-kernel calibration), committed in
     profiles/trace_c3_fp32_pmc.txt as 'traffic_bytes_per_launch: X'."""
-    path = os.path.join(root, 'profiles', 'trace_c3_fp32_pmc.txt')
-    if not os.path.exists(path):
+    return read_profile_value(root, 'trace_c3_fp32_pmc.txt', 'traffic_bytes_per_launch')
+
+
+def valu_from_counters(root, prec, kernel_s, n):
+    """VALU figures of the headline kernel from the committed counter run
+    (profiles/trace_c3_valu_pmc.txt: per-dispatch SQ_INSTS_VALU and the
+    FP32/FP64 add, multiply, FMA and transcendental instruction counts of
+    the same bench command): the fraction of the 1024 SIMDs' issue cycles
+    that VALU instructions take (4 cycles per wave64 instruction at
+    2.4 GHz, over this run's kernel time) and the achieved FLOP/s against
+    the FP32 / FP64 vector peak (64 lanes per instruction, an FMA counts 2)."""
+    tag = 'fp32' if prec == 32 else 'fp64'
+    insts = read_profile_value(root, 'trace_c3_valu_pmc.txt', f'valu_insts_per_dispatch_{tag}')
+    flop = read_profile_value(root, 'trace_c3_valu_pmc.txt', f'valu_flops_per_dispatch_{tag}')
+    if insts is None or flop is None:
         return None
-    m = re.search(r'traffic_bytes_per_launch:\s*([0-9.eE+]+)', open(path).read())
-    return float(m.group(1)) if m else None
+    peak = FP32_VALU_PEAK_TFLOPS if prec == 32 else FP64_VALU_PEAK_TFLOPS
+    return {'valu_insts_per_launch': insts,
+            'valu_insts_per_photon_wave': insts / (n / 64),
+            'valu_issue_fraction': insts * 4 / (1024 * 2.4e9 * kernel_s),
+            'flops_per_launch': flop,
+            'tflops': flop / kernel_s / 1e12,
+            'flop_fraction_of_peak': flop / kernel_s / 1e12 / peak,
+            'peak_tflops': peak,
+            'source': 'profiles/trace_c3_valu_pmc.txt (rocprofv3 --pmc of this bench command)'}
 
 
 def main():
@@ -270,6 +301,21 @@ def main():
                          f'uniforms), numpy float64 element-major oracle, one thread, '
                          f'{tc:.1f} s'}
         log('CPU oracle', cpu)
+        # the reference's EfficiencyFile selects orders one photon at a time
+        # (grating.py:90-96): the same oracle with that loop, on a tenth of
+        # the sample, identical results
+        s_loop = max(1, s // 10)
+        ph_l = {k: v[:s_loop] for k, v in ph.items()}
+        t = time.perf_counter()
+        ref_l = OT.trace(els, ph_l, us[:, :s_loop], selector_loop=True)
+        tl = time.perf_counter() - t
+        same = all(np.array_equal(np.asarray(ref_l[k]), np.asarray(ref[k])[:s_loop],
+                                  equal_nan=np.asarray(ref_l[k]).dtype.kind == 'f') for k in ref_l)
+        cpu_loop = {'value': s_loop / tl, 'unit': 'photons/s', 'cores': 1, 'kind': 'port',
+                    'sample': f'first {s_loop} photons, the oracle with the per-photon order '
+                              f'selection loop of the reference\'s EfficiencyFile, one thread, '
+                              f'{tl:.1f} s', 'identical_to_vectorised': bool(same)}
+        log('CPU oracle, per-photon selector loop', cpu_loop)
         # the same sample split over processes (photons are independent)
         import multiprocessing as mp
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
@@ -308,8 +354,33 @@ def main():
                              'columns copied back into the table'}
         log('host table path', host_rate)
 
-    pps32 = n / k32
-    pps64 = n / k64
+    # candidate plane tests per launch, counted by the kernels (status
+    # counter), and the same step with the rays regrouped between stages
+    # (opt-in, identical results)
+    tests = {}
+    for prec, out in ((32, out32), (64, out64)):
+        st1 = torch.zeros(8, dtype=torch.int64, device=dev)
+        ci.launch(out, uni, precision=prec, src=src, status=st1)
+        tests[prec] = int(st1.cpu()[7])
+    regrouped = {}
+    for prec in (32, 64):
+        out_r = DevicePhotons(src.dir, src.pol, src.energy, src.prob, device=dev)
+
+        def step_rg(o, p):
+            ci.launch(o, uni, precision=p, src=src, status=status, regroup=True)
+        wr, kr = timed(out_r, prec, step_rg)
+        same = all(torch.equal(getattr(out_r, c), getattr(out32 if prec == 32 else out64, c))
+                   for c in ('pos', 'dir', 'pol', 'prob'))
+        regrouped[prec] = {'ms_per_step': wr / args.steps * 1e3, 'kernel_ms': kr * 1e3,
+                           'pairs_per_round': ci.regroup_pairs(prec, regroup=True),
+                           'identical_results': bool(same)}
+        del out_r
+    log('tests per launch', tests, 'with regrouping', regrouped)
+
+    ms32 = wall32 / args.steps
+    ms64 = wall64 / args.steps
+    pps32 = n / ms32
+    pps64 = n / ms64
     gbs = BYTES_PER_PHOTON * n / k32 / 1e9
     traffic = read_traffic(root)
     res = {
@@ -342,23 +413,27 @@ def main():
         'fp64_ms_per_step': wall64 / args.steps * 1e3,
         'fp32_over_fp64': pps32 / pps64,
         'fp32_error_vs_fp64': err,
-        'photon_element_tests_per_s_fp32': pps32 * n_elem,
-        'photon_element_tests_per_s_fp64': pps64 * n_elem,
         'kernel_ms_fp32': k32 * 1e3,
         'kernel_ms_fp64': k64 * 1e3,
+        'with_regrouping': regrouped,
+        # plane tests the kernels made (status counter), per second of step
+        'photon_element_tests': {
+            'per_launch_fp32': tests[32], 'per_launch_fp64': tests[64],
+            'per_photon_fp32': tests[32] / n, 'per_photon_fp64': tests[64] / n,
+            'per_s_fp32': tests[32] / ms32, 'per_s_fp64': tests[64] / ms64,
+            'brute_force_tests_per_photon': n_elem,
+            'what': 'candidate ray-plane tests of the culling lists (every element where a '
+                    'ray is too oblique for them), lens and CCDs included, counted on the GPU'},
+        'valu_fp32': valu_from_counters(root, 32, k32, n),
+        'valu_fp64': valu_from_counters(root, 64, k64, n),
         'bytes_per_photon': BYTES_PER_PHOTON,
         # counters accumulate over the warm-up and timed launches of both precisions
         'status_per_launch': {'grid_fallback': int(st[3]) / (2 * (args.warmup + args.steps)),
                               'multi_hit': int(st[4]) / (2 * (args.warmup + args.steps))},
     }
-    # brute-force equivalent VALU fractions: 33 flops per photon-element test
-    res['valu_fraction_brute_force_equiv'] = {
-        'fp32': pps32 * n_elem * FLOPS_PER_TEST / (FP32_VALU_PEAK_TFLOPS * 1e12),
-        'fp64': pps64 * n_elem * FLOPS_PER_TEST / (FP64_VALU_PEAK_TFLOPS * 1e12),
-        'flops_per_test': FLOPS_PER_TEST, 'tests_per_photon': n_elem,
-        'peaks_tflops': {'fp32': FP32_VALU_PEAK_TFLOPS, 'fp64': FP64_VALU_PEAK_TFLOPS}}
     if cpu is not None:
         res['cpu_baseline_multiprocess'] = cpu_mp
+        res['cpu_baseline_selector_loop'] = cpu_loop
     if host_rate is not None:
         res['host_table_path'] = host_rate
     del out32, out64, src, uni, u, energy
@@ -522,10 +597,9 @@ def bench_stack(args, dev):
     from marxs_amd import configs, catgrating
     from marxs_amd.tracer import CompiledInstrument, DevicePhotons, raise_for_status
     from marxs_amd.batch import outputs_like
-    si = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tests', 'golden',
-                      'SiTransmission.csv')
     inst, parts = configs.cat_rowland_instrument(
-        args.facets, facet_class=catgrating.CATL1L2Stack, facet_kwargs={'l1_transmission': si})
+        args.facets, facet_class=catgrating.CATL1L2Stack,
+        facet_kwargs={'l1_transmission': catgrating.l1transtab_path})
     ci = CompiledInstrument(inst, grid=bool(args.grid), device=dev)
     n = args.photons
     src, uni = gpu_inputs(ci, n, args.seed + 5, dev)

@@ -323,22 +323,30 @@ def resolvingpower_from_photonlist_robust(lphotons, orders, cols, zeropositions,
 
 
 def weighted_per_order(data, orders, energy, gratingeff):
-    """Average of a per-order, per-energy quantity weighted by the order
-    probabilities of ``gratingeff`` interpolated to ``energy``
-    (gratings.py:133-178; host arithmetic on a few numbers)."""
-    if len(orders) != data.shape[0]:
+    """Mean over the orders of a per-order, per-energy quantity (e.g. the
+    resolving power of each order at each energy), weighted by the
+    probability that ``gratingeff`` (an ``EfficiencyFile``) diffracts a
+    photon of that energy into that order, linearly interpolated in energy
+    (marxs/analysis/gratings.py:133-178; masked values do not count).
+
+    data: (n_orders, n_energies); returns (n_energies,) masked array."""
+    data = np.asanyarray(data)
+    if data.shape[0] != len(orders):
         raise ValueError('First dimension of "data" must match length of "orders".')
-    if len(energy) != data.shape[1]:
+    if data.shape[1] != len(energy):
         raise ValueError('Second dimension of "data" must match length of "energy".')
-    weights = np.zeros_like(data)
-    for i, o in enumerate(orders):
-        ind_o = (gratingeff.orders == o).nonzero()[0]
-        if len(ind_o) != 1:
+    table_orders = np.asarray(gratingeff.orders)
+    # the efficiency column of each requested order (exactly one)
+    hits = table_orders[None, :] == np.asarray(orders)[:, None]
+    for o, h in zip(orders, hits):
+        if h.sum() != 1:
             raise KeyError('No data for order {0} in gratingeff'.format(o))
-        en_sort = np.argsort(gratingeff.energy)
-        weights[i, :] = np.interp(energy, gratingeff.energy[en_sort],
-                                  gratingeff.prob[:, ind_o[0]][en_sort])
-    return np.ma.average(data, axis=0, weights=weights)
+    cols = np.argmax(hits, axis=1)
+    by_energy = np.argsort(gratingeff.energy)
+    e_tab = np.asarray(gratingeff.energy)[by_energy]
+    p_tab = np.asarray(gratingeff.prob)[by_energy][:, cols]
+    weights = np.stack([np.interp(energy, e_tab, p_tab[:, i]) for i in range(len(cols))])
+    return np.ma.average(data, axis=0, weights=weights.astype(data.dtype, copy=False))
 
 
 def identify_photon_in_subaperture(angle, max_ang, ang_0=np.pi / 2):

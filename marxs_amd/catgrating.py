@@ -12,11 +12,15 @@ Gaussian broadening by diffraction on the L2 openings (``L2Diffraction``).
 (``catgrating.py:327-374``).
 
 Lengths are plain numbers in mm (the reference uses astropy quantities).
-The L1 transmission table (transmission of 1 um of Si against energy) is
-not shipped: pass ``l1_transmission=(energy_keV, transmission)`` or the path
-of a table in the reference's ECSV layout (energy in eV).
+The L1 transmission table (transmission of 1 um of Si against energy, from
+the Henke tables) ships with the package as ``data/SiTransmission.csv``,
+the same table and layout the reference reads at import
+(``catgrating.py:41``); ``l1_transmission`` overrides it with
+``(energy_keV, transmission)`` or the path of another table in that layout
+(energy in eV).
 """
 import math
+import os
 
 import numpy as np
 
@@ -24,12 +28,17 @@ from .optics import (CATGrating, FlatOpticalElement, FlatStack, NonParallelCATGr
                      OrderSelector, read_efficiency_table)
 from .scatter import RandomGaussianScatter
 
-__all__ = ['l1_order_selector', 'l1_dims', 'l2_dims', 'qualityfactor', 'd',
+__all__ = ['l1transtab_path', 'l1_order_selector', 'l1_dims', 'l2_dims', 'qualityfactor', 'd',
            'QualityFactor', 'L1', 'L2Abs', 'L2Diffraction', 'CATL1L2Stack',
            'check_lx_dims', 'read_transmission_table', 'catsupportbars']
 
 d = 0.0002
 """Spacing of grating bars (mm)."""
+
+l1transtab_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data',
+                               'SiTransmission.csv')
+"""Transmission of 1 um of Si against energy (eV), the L1 default
+(catgrating.py:41)."""
 
 l1_order_selector = OrderSelector(orderlist=np.array([-4, -3, -2, -1, 0, 1, 2, 3, 4]),
                                   p=np.array([0.006, 0.0135, 0.022, 0.028, 0.861, 0.028,
@@ -85,8 +94,7 @@ class L1(CATGrating):
         check_lx_dims(l1_dims)
         self.openfraction = 1 - l1_dims['barwidth'] / l1_dims['period']
         if l1_transmission is None:
-            raise ValueError('L1 needs l1_transmission: (energy_keV, transmission of 1 um Si) '
-                             'or the path of such a table')
+            l1_transmission = l1transtab_path
         if isinstance(l1_transmission, str):
             l1_transmission = read_transmission_table(l1_transmission)
         energy, trans = (np.asarray(x, dtype=float) for x in l1_transmission)

@@ -1,19 +1,30 @@
 """Rowland-torus geometry and element placement (set-up, host float64).
 
-Follows ``marxs/design/rowland.py``:
+The interface is that of ``marxs/design/rowland.py`` (class and method
+names, arguments, element counts and order); the geometry is written here
+from the torus itself:
 
-* ``RowlandTorus`` (``78-366``): torus with the local y axis as symmetry
-  axis, ``Q(x) = (|x|^2 + R^2 - r^2)^2 - 4 R^2 (x_x^2 + x_z^2)``;
-  ``solve_quartic`` intersects a line with the torus.  The reference runs
-  ``scipy.optimize.root`` from ``k0 = 0.1 max(|r - R|, 0.01 r)``; here the
-  quartic in the line parameter ``k`` is expanded into its polynomial
-  coefficients, all roots are found with a companion-matrix eigen solve, the
-  real root nearest ``k0`` is taken and polished with Newton steps.
-* ``design_tilted_torus`` (``369-422``).
+* ``RowlandTorus`` (``78-366``): in its local frame a core circle of radius
+  ``R`` in the x-z plane with a tube of radius ``r`` around it (symmetry
+  axis y); ``Q(x) = (|x|^2 + R^2 - r^2)^2 - 4 R^2 (x_x^2 + x_z^2)`` is zero
+  on the surface.  Points and normals from (tube angle, azimuth) follow from
+  the core-circle construction; the normal at a given point is the
+  normalised gradient of ``Q``; ``solve_quartic`` intersects a line with the
+  torus: the reference runs ``scipy.optimize.root`` from
+  ``k0 = 0.1 max(|r - R|, 0.01 r)``, here the quartic in the line parameter
+  ``k`` is expanded into its polynomial coefficients, all roots are found
+  with a companion-matrix eigen solve, the real root nearest ``k0`` is taken
+  and polished with Newton steps.
+* ``design_tilted_torus`` (``369-422``): the circle through the focus and a
+  tilted point at distance f.
 * ``ElementsOnTorus``/``RectangularGrid``/``CircularMeshGrid``/
   ``GratingArrayStructure`` (``547-888``): element centres from a y-z layout
   pushed along ``optimize_axis`` onto the torus; normals ``-torus.normal``;
-  in-plane axis parallel to the torus symmetry axis.
+  in-plane axis parallel to the torus symmetry axis.  The layouts must give
+  the reference's element counts and positions (its tests,
+  ``design/tests/test_design.py:209-238,441-456``), so their rules (rows of
+  centred elements, ring counts rounded up, arc counts rounded down, equal
+  gaps on an arc) are the reference's.
 """
 import numpy as np
 
@@ -101,61 +112,102 @@ class RowlandTorus(MarxsElement, Geometry):
                 break
         return e2h(origin + k * v, 1)
 
+    # -- the torus as a core circle with a tube around it --------------------
+    # In the local frame the core circle has radius R in the x-z plane and the
+    # symmetry axis is y.  A point is core(phi) + r t(theta, phi) with
+    #   u(phi) = (cos phi, 0, sin phi)        (radial unit vector of the core)
+    #   t(theta, phi) = cos(theta) u(phi) + sin(theta) y   (unit, outward)
+    # so t is also the surface normal there.
+    @staticmethod
+    def _tube_direction(theta, phi):
+        ct = np.cos(theta)
+        return np.stack([ct * np.cos(phi), np.sin(theta), ct * np.sin(phi)], axis=-1)
+
+    def _global(self, vec, w):
+        return np.einsum('...ij,...j', self.pos4d, e2h(vec, w))
+
     def parametric(self, theta, phi):
+        """Points (homogeneous, global) at tube angle ``theta`` and azimuth
+        ``phi``: core point R u(phi) plus r times the tube direction."""
         theta = np.asarray(theta, dtype=float)
         phi = np.asarray(phi, dtype=float)
-        x = (self.R + self.r * np.cos(theta)) * np.cos(phi)
-        z = (self.R + self.r * np.cos(theta)) * np.sin(phi)
-        y = self.r * np.sin(theta)
-        torus = np.array([x, y, z, np.ones_like(z)]).T
-        return np.einsum('...ij,...j', self.pos4d, torus)
+        core = self.R * np.stack([np.cos(phi), np.zeros_like(phi), np.sin(phi)], axis=-1)
+        return self._global(core + self.r * self._tube_direction(theta, phi), 1)
+
+    def normal_parametric(self, theta, phi):
+        """Outward unit normals (homogeneous direction vectors, global) at
+        the surface points of :meth:`parametric`: the tube direction."""
+        theta = np.asarray(theta, dtype=float)
+        phi = np.asarray(phi, dtype=float)
+        if theta.ndim != 1 or phi.ndim != 1:
+            raise ValueError('theta and phi must be 1-d arrays.')
+        return self._global(self._tube_direction(theta, phi), 0)
+
+    def _check_on_surface(self, xyz):
+        if not np.allclose(self.quartic(xyz, transform=False) / self.R ** 4., 0.):
+            raise ValueError('Parametric representation is only defined for '
+                             'points on torus surface.')
 
     def xyzw2parametric(self, xyzw, transform=True, intersectvalid=True):
+        """(theta, phi) of surface points.  A point at distance rho from the
+        symmetry axis lies on the tube around the core point of its own
+        azimuth when |x|^2 + R^2 - r^2 = 2 R rho, else (a spindle torus,
+        r > R, near the axis) on the tube around the opposite core point, at
+        azimuth + pi.  theta is the angle of the point about that core point,
+        measured from u(phi) towards y, in (-pi, pi].  Points on the axis
+        have no azimuth: phi = 0 there (NaN with ``intersectvalid=False``)."""
         xyzw = np.asarray(xyzw, dtype=float)
         if transform:
             xyzw = np.einsum('...ij,...j', np.linalg.inv(self.pos4d), xyzw)
         xyz = h2e(xyzw)
-        if not np.allclose(self.quartic(xyz, transform=False) / self.R ** 4., 0.):
-            raise ValueError('Parametric representation is only defined for '
-                             'points on torus surface.')
-        s = np.sign(np.sqrt(xyz[:, 0] ** 2 + xyz[:, 2] ** 2) - self.R)
-        theta = np.arcsin(s * xyz[:, 1] / self.r) + (s < 0) * np.pi
-        factor = self.R + self.r * np.cos(theta)
-        phi = np.zeros_like(factor)
-        ind = factor != 0
-        phi[ind] = np.arctan2(xyz[ind, 2] / factor[ind], xyz[ind, 0] / factor[ind])
-        if not intersectvalid:
-            phi[~ind] = np.nan
+        self._check_on_surface(xyz)
+        x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+        rho = np.hypot(x, z)
+        same = (xyz * xyz).sum(axis=1) + self.R ** 2 - self.r ** 2 >= 0
+        on_axis = rho == 0
+        az = np.arctan2(z, x)
+        phi = np.where(same, az, az + np.pi)
+        theta = np.arctan2(y, np.where(same, rho - self.R, -rho - self.R))
+        phi = np.where(on_axis, np.nan if not intersectvalid else 0., phi)
         return theta, phi
 
-    def normal_parametric(self, theta, phi):
-        theta = np.asarray(theta, dtype=float)
-        phi = np.asarray(phi, dtype=float)
-        if not (theta.ndim == 1 and phi.ndim == 1):
-            raise ValueError('theta and phi must be 1-d arrays.')
-        normal = np.empty((len(theta), 3))
-        normal[:, 0] = np.cos(theta) * np.cos(phi)
-        normal[:, 1] = np.sin(theta)
-        normal[:, 2] = np.cos(theta) * np.sin(phi)
-        return np.einsum('...ij,...j', self.pos4d, e2h(normal, 0))
-
     def normal(self, xyzw):
+        """Outward unit normals (homogeneous direction vectors, global) at
+        surface points ``xyzw`` (N, 4): in the local frame the gradient of
+        the quartic, grad Q = 4 a x - 8 R^2 (x, 0, z) with
+        a = |x|^2 + R^2 - r^2, normalised and turned by the sign of a (which
+        is negative on the inner sheet of a spindle torus, where the gradient
+        points into the tube)."""
         xyzw = np.asarray(xyzw, dtype=float)
         if xyzw.ndim != 2:
             raise ValueError('Shape of input array must be (N, 4).')
-        theta, phi = self.xyzw2parametric(xyzw, 1)
-        return self.normal_parametric(theta, phi)
+        xyz = h2e(np.einsum('...ij,...j', np.linalg.inv(self.pos4d), xyzw))
+        self._check_on_surface(xyz)
+        a = (xyz * xyz).sum(axis=1) + self.R ** 2 - self.r ** 2
+        grad = 4 * a[:, None] * xyz
+        grad[:, [0, 2]] -= 8 * self.R ** 2 * xyz[:, [0, 2]]
+        n = grad / np.linalg.norm(grad, axis=1)[:, None] * np.where(a < 0, -1., 1.)[:, None]
+        return self._global(n, 0)
 
 
 def design_tilted_torus(f, alpha, beta):
-    """Torus parameters (R, r, pos4d) for focal distance ``f`` with the
-    Rowland circle tilted by ``alpha`` and the hinge at ``beta``."""
+    """Torus (R, r, pos4d) whose Rowland circle passes through the origin
+    (the focus) and the point at distance ``f`` along x tilted by ``alpha``,
+    with the hinge at angle ``beta`` (marxs/design/rowland.py:369-422).
+
+    The circle through both points with its centre on the perpendicular
+    bisector of the chord has radius r = f / (2 cos alpha); the core circle
+    (radius R = r cos beta) is the circle's centre rotated about the hinge by
+    beta - alpha, i.e. the torus frame is turned by beta - alpha about -z and
+    placed at the chord midpoint (f/2, f/2 tan alpha) minus R along the
+    rotated x axis."""
     r = f / (2. * np.cos(alpha))
-    R = r * np.sin(np.pi / 2 - beta)
-    orientation = axangle2mat([0, 0, -1], beta - alpha)
-    x_ct = f / 2 - R * np.cos(beta - alpha)
-    y_ct = f / 2 * np.tan(alpha) + R * np.sin(beta - alpha)
-    return R, r, compose([x_ct, y_ct, 0], orientation, np.ones(3))
+    R = r * np.cos(beta)
+    turn = beta - alpha
+    rot = axangle2mat([0, 0, -1], turn)
+    mid = np.array([f / 2, f / 2 * np.tan(alpha), 0.])
+    centre = mid + R * np.array([-np.cos(turn), np.sin(turn), 0.])
+    return R, r, compose(centre, rot, np.ones(3))
 
 
 class ElementsOnTorus(ParallelCalculated):
@@ -188,88 +240,114 @@ class ElementsOnTorus(ParallelCalculated):
                           for o in origin])
 
 
+def _centred_row(lo, hi, pitch):
+    """Centres of ceil((hi - lo) / pitch) elements (at least one) of width
+    ``pitch`` placed edge to edge and centred on the middle of [lo, hi]."""
+    n = max(1, int(np.ceil((hi - lo) / pitch)))
+    return 0.5 * (lo + hi) + (np.arange(n) - 0.5 * (n - 1)) * pitch
+
+
 class RectangularGrid(ElementsOnTorus):
+    """Elements on a rectangular y-z grid of pitch ``d_element`` centred on
+    ``y_range`` x ``z_range`` (marxs/design/rowland.py:660-700; the default
+    z range is a single row), ordered row by row in z."""
+
     def __init__(self, **kwargs):
         self.y_range = kwargs.pop('y_range')
         self.z_range = kwargs.pop('z_range', [-1e-10, 1e-10])
         super().__init__(**kwargs)
 
     def elemposyz(self):
-        n_y = max(1, int(np.ceil((self.y_range[1] - self.y_range[0]) / self.d_element[0])))
-        n_z = max(1, int(np.ceil((self.z_range[1] - self.z_range[0]) / self.d_element[1])))
-        width_y = n_y * self.d_element[0]
-        width_z = n_z * self.d_element[1]
-        ypos = np.arange(0.5 * (self.y_range[0] - width_y + self.y_range[1] + self.d_element[0]),
-                         self.y_range[1], self.d_element[0])
-        zpos = np.arange(0.5 * (self.z_range[0] - width_z + self.z_range[1] + self.d_element[1]),
-                         self.z_range[1], self.d_element[1])
-        ypos, zpos = np.meshgrid(ypos, zpos)
-        return ypos.flatten(), zpos.flatten()
+        ys = _centred_row(self.y_range[0], self.y_range[1], self.d_element[0])
+        zs = _centred_row(self.z_range[0], self.z_range[1], self.d_element[1])
+        return np.tile(ys, len(zs)), np.repeat(zs, len(ys))
 
 
 class CircularMeshGrid(ElementsOnTorus):
+    """Elements on rows of pitch d_element[0] across an annulus
+    radius[0] .. radius[1] (marxs/design/rowland.py:703-755): the rows start
+    at half the row block's width below the centre; on each row the chord
+    through the annulus (one chord outside the hole, two mirrored chords
+    beside it) is filled with elements of pitch d_element[1] centred on the
+    chord, the chord's far end excluded; a row beyond the outer radius holds
+    one element at z = 0."""
+
     def __init__(self, **kwargs):
         self.radius = kwargs.pop('radius')
         super().__init__(**kwargs)
 
+    def _row(self, y):
+        r_in, r_out = self.radius[0], self.radius[1]
+        dz = self.d_element[1]
+        if abs(y) > r_out:
+            return np.zeros(1)
+        half = np.sqrt(r_out ** 2 - y ** 2)
+        if abs(y) > r_in:
+            n = int(np.ceil(2 * half / dz))
+            return np.arange(-0.5 * n * dz, half, dz)
+        hole = np.sqrt(r_in ** 2 - y ** 2)
+        n = int(np.ceil((half - hole) / dz))
+        side = np.arange(0.5 * (hole + half) - 0.5 * n * dz, half, dz)
+        return np.concatenate([-side, side])
+
     def elemposyz(self):
-        n_y = int(np.ceil(2 * self.radius[1] / self.d_element[0]))
-        y_width = n_y * self.d_element[0]
-        ypos, zpos = [], []
-        for y in np.arange(-y_width / 2, self.radius[1], self.d_element[0]):
-            if np.abs(y) > self.radius[1]:
-                z = np.array([0])
-            else:
-                z_outer = np.sqrt(self.radius[1] ** 2 - y ** 2)
-                if np.abs(y) > self.radius[0]:
-                    n_z = int(np.ceil(2 * z_outer / self.d_element[1]))
-                    z_width = n_z * self.d_element[1]
-                    z = np.arange(-z_width / 2, z_outer, self.d_element[1])
-                else:
-                    z_inner = np.sqrt(self.radius[0] ** 2 - y ** 2)
-                    z_mid = 0.5 * (z_inner + z_outer)
-                    n_z = int(np.ceil((z_outer - z_inner) / self.d_element[1]))
-                    z_width = n_z * self.d_element[1]
-                    z = np.arange(z_mid - z_width / 2, z_outer, self.d_element[1])
-                    z = np.hstack([-z, z])
-            ypos.extend([y] * len(z))
-            zpos.extend(z)
-        return ypos, zpos
+        dy = self.d_element[0]
+        n_rows = int(np.ceil(2 * self.radius[1] / dy))
+        rows = np.arange(-0.5 * n_rows * dy, self.radius[1], dy)
+        zs = [self._row(y) for y in rows]
+        return (np.concatenate([np.full(len(z), y) for y, z in zip(rows, zs)]).tolist(),
+                np.concatenate(zs).tolist())
 
 
 class GratingArrayStructure(ElementsOnTorus):
+    """Facets on rings (marxs/design/rowland.py:758-888).
+
+    ``radius`` holds pairs of ring bounds; each pair gets
+    ceil(width / d_element[0]) rings of pitch d_element[0] centred in it.  A
+    ring of radius rho carries floor((rho - d_element[0] / 2) span /
+    d_element[1]) facets over the azimuth range ``phi`` (span = its width),
+    spread with equal gaps: with w = d_element[1] / (2 pi rho) the angle
+    the reference gives one facet, the gap is (span - n w) / (n + 1) and
+    facet k sits at phi[0] + (k + 1) gap + (k + 1/2) w.  Positions
+    y = rho sin(angle), z = rho cos(angle), ring by ring."""
+
     def __init__(self, **kwargs):
         self.phi = kwargs.pop('phi', [0., 2 * np.pi])
         self.radius = kwargs.pop('radius')
         super().__init__(**kwargs)
 
     def max_elements_on_radius(self, radius):
+        """Rings that fit between the two bounds of ``radius``."""
         return int(np.ceil((radius[1] - radius[0]) / self.d_element[0]))
 
     def distribute_elements_on_radius(self):
-        radii = []
-        for i in range(len(self.radius) // 2):
-            bracket = self.radius[2 * i: 2 * i + 2]
-            n = self.max_elements_on_radius(bracket)
-            radii.append(np.mean(bracket)
-                         + np.arange(-n / 2 + 0.5, n / 2 + 0.5) * self.d_element[0])
-        return np.hstack(radii)
+        """Ring radii for every pair of bounds in ``self.radius``."""
+        bounds = np.asarray(self.radius, dtype=float).reshape(-1, 2)
+        rings = []
+        for lo, hi in bounds:
+            n = self.max_elements_on_radius([lo, hi])
+            rings.append(0.5 * (lo + hi) + (np.arange(n) - 0.5 * (n - 1)) * self.d_element[0])
+        return np.concatenate(rings) if rings else np.zeros(0)
 
     def max_elements_on_arc(self, radius):
+        """Facets of pitch d_element[1] that fit on an arc of ``radius``
+        over the azimuth span (float, rounded down)."""
         return radius * anglediff(self.phi) // self.d_element[1]
 
     def distribute_elements_on_arc(self, radius):
+        """Facet azimuths on the ring of ``radius``, in [0, 2 pi)."""
         if len(self.phi) == 1:
             return self.phi
+        span = anglediff(self.phi)
         n = self.max_elements_on_arc(radius - self.d_element[0] / 2)
-        element_angle = self.d_element[1] / (2. * np.pi * radius)
-        d_between = (anglediff(self.phi) - n * element_angle) / (n + 1)
-        centerangles = d_between + 0.5 * element_angle + np.arange(n) * (d_between + element_angle)
-        return (self.phi[0] + centerangles) % (2. * np.pi)
+        w = self.d_element[1] / (2. * np.pi * radius)
+        gap = (span - n * w) / (n + 1)
+        k = np.arange(n)
+        return (self.phi[0] + (gap + 0.5 * w + k * (gap + w))) % (2. * np.pi)
 
     def elemposyz(self):
-        radii = self.distribute_elements_on_radius()
-        angles = [self.distribute_elements_on_arc(r) for r in radii]
-        radii = np.concatenate([[radii[i]] * len(a) for i, a in enumerate(angles)])
-        angles = np.concatenate(angles)
-        return radii * np.sin(angles), radii * np.cos(angles)
+        rings = self.distribute_elements_on_radius()
+        per_ring = [self.distribute_elements_on_arc(rho) for rho in rings]
+        rho = np.concatenate([np.full(len(a), r) for r, a in zip(rings, per_ring)])
+        ang = np.concatenate(per_ring)
+        return rho * np.sin(ang), rho * np.cos(ang)
