@@ -344,6 +344,19 @@ typedef struct MxOrderStats {
 int64_t mx_order_stats_workspace(int64_t n, int32_t n_orders, int32_t with_values);
 int mx_order_stats(const MxOrderStats *args, void *stream);
 
+/* mx_order_stats without a caller workspace: the scratch memory is a static
+ * device array of MX_OS_STATIC_BYTES (64 MiB) that every GPU the library runs
+ * on holds until the process ends; args->workspace is ignored.  The call
+ * allocates nothing (no hipMalloc, no allocating library routine).  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  Limit: at most
+ * MX_OS_STATIC_BYTES of workspace as mx_order_stats_workspace counts it
+ * (about 2.4e6 photons with a value column, 15 orders); larger calls return
+ * 1 with an error.  Calls on one GPU share the array: calls that overlap on
+ * one GPU (different streams) race. */
+#define MX_OS_STATIC_BYTES (64LL << 20)
+int mx_order_stats_static(const MxOrderStats *args, void *stream);
+
 /* Fill n int32 values with v (a kernel, so that it can be captured in a
  * hipGraph without a memset node).  No reference counterpart: device
  * utility for the callers of this library. */

@@ -81,6 +81,7 @@ MX_AJ_NGRAD = 13
 MX_AJ_SUMS = 1
 MX_MAX_ORDERS = 64
 MX_OS_FILTER = 1
+MX_OS_STATIC_BYTES = 64 << 20
 
 
 def _load():
@@ -108,6 +109,8 @@ def _load():
     lib.mx_order_stats_workspace.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]
     lib.mx_order_stats.restype = ctypes.c_int
     lib.mx_order_stats.argtypes = [ctypes.POINTER(MxOrderStats), P]
+    lib.mx_order_stats_static.restype = ctypes.c_int
+    lib.mx_order_stats_static.argtypes = [ctypes.POINTER(MxOrderStats), P]
     v = lib.mx_abi_version()
     if v != MX_ABI_VERSION:
         raise ImportError(f'libmarxs_hip.so ABI {v}, expected {MX_ABI_VERSION}: rebuild it')

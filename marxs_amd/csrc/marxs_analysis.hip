@@ -30,6 +30,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 #include <stdint.h>
 #include <stdio.h>
 #include <math.h>
@@ -637,4 +638,71 @@ extern "C" int mx_order_stats(const MxOrderStats *a, void *stream) {
         return 2;
     }
     return 0;
+}
+
+// ------------------------------------------------ allocation-free variant
+// mx_order_stats_static: mx_order_stats with its workspace in `scratch`, a
+// file-scope device array of MX_OS_STATIC_BYTES, for callers that cannot
+// allocate (code under graph capture, tight serving loops).  Every GPU the
+// library runs on holds its own copy until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (in host code it is a
+// host-side stand-in): the device address of this GPU's copy comes from
+// hipGetSymbolAddress on the entry point's first call on that GPU and is
+// kept per device.  Calls on one GPU share the array, so calls that overlap
+// on one GPU race; it keeps the last call's contents, and every word a call
+// reads it has written first (counts, offsets, segments, values, clipping
+// state), so nothing needs clearing.
+static __device__ __align__(256) unsigned char scratch[MX_OS_STATIC_BYTES];
+
+#define MX_OS_MAX_DEVICES 64
+static std::mutex g_scratch_mutex;
+static void *g_scratch_dev[MX_OS_MAX_DEVICES];
+
+static int scratch_address(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MX_OS_MAX_DEVICES) {
+        mx_set_error("order stats (static workspace): no current device or device index too large");
+        return 2;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            char buf[256];
+            snprintf(buf, sizeof(buf), "order stats (static workspace): hipGetSymbolAddress failed: %s",
+                     hipGetErrorString(e));
+            mx_set_error(buf);
+            return 2;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+extern "C" int mx_order_stats_static(const MxOrderStats *a, void *stream) {
+    if (!a) { mx_set_error("order stats: arguments NULL"); return 1; }
+    if (a->n < 0 || a->n > 0x7fffffffLL) { mx_set_error("order stats: photon count outside [0, 2^31)"); return 1; }
+    if (a->n_orders < 1 || a->n_orders > MX_MAX_ORDERS) { mx_set_error("order stats: n_orders outside [1, 64]"); return 1; }
+    OsLayout L;
+    if (os_layout(a->n, a->n_orders, a->value != nullptr, L)) {
+        mx_set_error("order stats: sort size query failed");
+        return 2;
+    }
+    if ((int64_t)L.total > (int64_t)MX_OS_STATIC_BYTES) {
+        char buf[256];
+        snprintf(buf, sizeof(buf),
+                 "order stats (static workspace): %lld photons need %lld bytes of workspace, the static "
+                 "workspace holds %lld (use mx_order_stats with a workspace of your own)",
+                 (long long)a->n, (long long)L.total, (long long)MX_OS_STATIC_BYTES);
+        mx_set_error(buf);
+        return 1;
+    }
+    void *ws = nullptr;
+    if (scratch_address(&ws)) return 2;
+    MxOrderStats b = *a;
+    b.workspace = ws;
+    b.workspace_bytes = (int64_t)MX_OS_STATIC_BYTES;
+    return mx_order_stats(&b, stream);
 }
