@@ -659,6 +659,16 @@ __device__ __noinline__ double atan2_call(double y, double x) { return atan2(y, 
 // uniform draw).
 __device__ __noinline__ double inv_normal_cdf(double u) { return normcdfinv(u); }
 
+// The same in float for the FP32 path, on the tail nearer to u: the
+// distance to 0 or 1 is formed in double, so that it keeps its relative
+// precision in float (a draw just below 1 would otherwise round to 1.0f and
+// give an infinite deviate); Phi^-1(u) = -Phi^-1(1 - u).
+__device__ __forceinline__ float inv_normal_cdf32(double u) {
+    const bool upper = u > 0.5;
+    const float z = normcdfinvf((float)(upper ? 1.0 - u : u));
+    return upper ? -z : z;
+}
+
 template <bool CALL> __device__ __forceinline__ double acos64(double x) {
     return CALL ? acos_call(x) : acos(x);
 }
@@ -1068,7 +1078,7 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             }
             const double u3 = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
             const double u4 = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
-            const float ang = (float)inv_normal_cdf(u3) * sigma;
+            const float ang = inv_normal_cdf32(u3) * sigma;
             const f3 pd = unitf(R.df);
             const float cc = 0.99999f * 0.99999f;
             const bool xaxis = (1.0f - cc) * (pd.x * pd.x) < cc * (pd.y * pd.y + pd.z * pd.z);
@@ -1094,11 +1104,11 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             f3 nd = R.df;
             float a_in = 0.0f, a_perp = 0.0f;
             if (sr[13] != 0.0) {
-                a_in = (float)(inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 3, i, cnt)) * sr[13]);
+                a_in = inv_normal_cdf32(draw(T, P, R, si[4], si[5], k0 + 3, i, cnt)) * (float)sr[13];
                 nd = axangle_t32(perpplane, a_in, R.df);
             }
             if (sr[16] != 0.0) {
-                a_perp = (float)(inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 4, i, cnt)) * sr[16]);
+                a_perp = inv_normal_cdf32(draw(T, P, R, si[4], si[5], k0 + 4, i, cnt)) * (float)sr[16];
                 nd = axangle_t32(radial, a_perp, nd);
             }
             R.polf = ptrans32(R.df, nd, R.polf);
