@@ -360,7 +360,7 @@ def main():
     tests = {}
     for prec, out in ((32, out32), (64, out64)):
         st1 = torch.zeros(8, dtype=torch.int64, device=dev)
-        ci.launch(out, uni, precision=prec, src=src, status=st1)
+        ci.launch(out, uni, precision=prec, src=src, status=st1, count_tests=True)
         tests[prec] = int(st1.cpu()[7])
     regrouped = {}
     for prec in (32, 64):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 18
+#define MX_ABI_VERSION 19
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -96,6 +96,17 @@ extern "C" {
 #define MX_OP_SCAT_CONST 6
 #define MX_OP_RADIAL     7
 #define MX_STACK_DRAWS   5   /* uniforms per stack interaction */
+/* stack record of a stack element (22 float64 in sel_d at elem_i[6]): L1
+ * groove direction l (0-2) and d (3-5), period (6), blaze centre and slope
+ * (7, 8), quality factor (9), L1 open fraction (10), L2 open fraction and
+ * shadowing coefficient (11, 12), scatter parameter (13), sel_d offset and
+ * length of the L1 transmission table (14, 15: energies in keV,
+ * transmissions, then the bucket count, 1 / bucket width and the nb + 1
+ * bucket starts), radial scatter sigma_perp and centre (16, 17-19), byte
+ * offsets of the table's copy in the FP32 / FP64 LDS image (20, 21; -1:
+ * none: energies, transmissions, bucket count and 1 / width as float64,
+ * then the bucket starts as uint16).  elem_i: 4 L1 sign mode, 5 L1
+ * transmission flag, 7 L1 selector row. */
 
 /* precision of mx_trace */
 #define MX_PREC_FP64 64   /* reference op order, no FMA contraction      */
@@ -103,6 +114,8 @@ extern "C" {
 
 /* mx_trace flags */
 #define MX_FLAG_NO_GRID 1  /* test every element of every array stage */
+#define MX_FLAG_COUNT_TESTS 4  /* count the ray-plane tests in status[MX_ST_TESTS] (an LDS
+                                  atomic per scan step of a wave: a few per cent of time) */
 #define MX_FLAG_REGROUP 2  /* regroup the rays of a workgroup before every culled array stage
                               (results are identical either way; measured slower on the
                               600-facet workload, see mx_trace_regroup_pairs) */
@@ -122,7 +135,8 @@ extern "C" {
 #define MX_ST_BAD_ARGS      5
 #define MX_ST_TRANS_RANGE   6  /* energy outside the L1 transmission table     */
 #define MX_ST_TESTS         7  /* photon-element plane tests made (candidates of
-                                  the culling lists, or every element) */
+                                  the culling lists, or every element); counted
+                                  only with MX_FLAG_COUNT_TESTS */
 
 typedef struct MxTables {
     int32_t n_stages;

@@ -49,7 +49,7 @@ CR = 40                 # float64 per tube record
 # number of operations, operation k in bits 4 + 3k)
 OP_GRATING, OP_QF, OP_L1, OP_L2ABS, OP_SCAT_L2, OP_SCAT_CONST, OP_RADIAL = 1, 2, 3, 4, 5, 6, 7
 STACK_DRAWS = 5     # draws per stack interaction: order, L1 order, L1 open, normal, angle
-SR = 20             # float64 per stack record
+SR = 22             # float64 per stack record
 
 SI, SD, ET, EP, EI, SELI = 64, 32, 16, 16, 8, 12
 ORDER_NONE = -2 ** 31
@@ -686,8 +686,10 @@ def _build_lds(tables, budget32=None, budget64=None):
     elements, while it fits: element test records (16 x f32 / f64), for FP32
     also the interaction parameter records, then the culling grid as uint16
     cell offsets and uint16 element indices (tight tier) and the successor
-    blocks of both tiers; last, the loose tier's cell offsets and items of
-    each stage while the image leaves ``LDS_RESERVE`` bytes of LDS free.
+    blocks of both tiers; then the L1 transmission tables of stack stages
+    (offsets in the stack records, words 20 / 21); last, the loose tier's
+    cell offsets and items of each stage while the image leaves
+    ``LDS_RESERVE`` bytes of LDS free.
     Offsets (bytes, -1 = global memory): stage_i[20]/[21] records FP32/FP64,
     [22] FP32 parameters, [24]/[25]/[30] FP32 tight cell offsets/items/
     successors, [26]/[27]/[31] the same FP64, [36]/[37]/[38] FP32 loose cell
@@ -815,6 +817,33 @@ def _build_lds(tables, budget32=None, budget64=None):
                     blk[o0:o0 + len(sit)] = sit
                     si[s, c2] = add(blk)
                 staged.append((s, g))
+        # L1 transmission tables of stack stages (energies and transmissions
+        # as float64, the bucket count and 1 / width as float64, the bucket
+        # starts as uint16): their byte offsets go to the stack records
+        rcol = 20 if prec == 32 else 21
+        placed = {}
+        for st in tables.stages:
+            if st.kind != STAGE_STACK:
+                continue
+            for e in range(st.elem0, st.elem0 + len(st.elements)):
+                r = int(tables.elem_i[e, 6])
+                tables.sel_d[r + rcol] = -1
+                to, tn = int(tables.sel_d[r + 14]), int(tables.sel_d[r + 15])
+                if tn < 2:
+                    continue
+                if to not in placed:
+                    x = tables.sel_d[to:to + 2 * tn]
+                    hdr = tables.sel_d[to + 2 * tn:to + 2 * tn + 2]
+                    nb = int(hdr[0])
+                    first = tables.sel_d[to + 2 * tn + 2:to + 2 * tn + 3 + nb] if nb else np.zeros(0)
+                    size = _align(16 * tn + 16 + 2 * len(first))
+                    if tn >= 65536 or off + size > budget:
+                        placed[to] = -1
+                    else:
+                        placed[to] = add(np.concatenate([
+                            np.concatenate([x, hdr]).view(np.uint8),
+                            first.astype(np.uint16).view(np.uint8)]))
+                tables.sel_d[r + rcol] = placed[to]
         # the loose tier's cell lists, while LDS_RESERVE stays free
         for s, g in staged:
             size = _align(2 * len(g['cell_start_l'])) + _align(2 * len(g['items_l']))
@@ -998,9 +1027,12 @@ def _stack_params(x, ep, ei, sel_index, sel_rows, sel_d, sel_o):
     the L2 inner free width in mm), offset and length of the L1 transmission
     table in sel_d (14, 15: energies in keV, then transmissions, then its
     bucket index, ``_bucket_index``), radial
-    scatter: perpendicular sigma (16) and centre (17-19).  L1 selector row,
-    sign mode and transmission flag in ei[7], ei[4], ei[5]."""
+    scatter: perpendicular sigma (16) and centre (17-19), byte offsets of the
+    transmission table's copy in the FP32 and FP64 LDS images (20, 21; -1:
+    none, set by ``_build_lds``).  L1 selector row, sign mode and transmission
+    flag in ei[7], ei[4], ei[5]."""
     rec = np.zeros(SR)
+    rec[20] = rec[21] = -1
     ei[4] = ei[5] = ei[7] = -1
     for op, ly in stack_program(x):
         if op == OP_GRATING:

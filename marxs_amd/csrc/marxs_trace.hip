@@ -234,6 +234,7 @@ template <typename T> __device__ __forceinline__ cptr<T> cp(const T *p) { return
 // lanes, so the counters take no vector registers.
 struct Counters {
     unsigned *c;
+    bool tests;      // count the plane tests (MX_FLAG_COUNT_TESTS; wave-uniform)
 };
 #define MX_NCOUNT 8
 __device__ __forceinline__ void count(const Counters &cnt, int k) {
@@ -772,17 +773,19 @@ __device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, i
 // bucket index: nb, 1 / bucket width, then for k = 0 .. nb the first index
 // whose energy is >= x[0] + k * width (the last = n); the search starts from
 // the bucket of e widened by one bucket on each side, so that rounding in
-// the bucket number cannot leave the answer outside it.
-__device__ __forceinline__ double trans_interp(const double *x, const double *y, int n, double e, bool &out) {
+// the bucket number cannot leave the answer outside it.  first: the bucket
+// starts (float64 in sel_d, uint16 in an LDS copy).
+template <typename DP, typename FP>
+__device__ __forceinline__ double trans_interp(DP x, DP y, FP first, int n, double e, bool &out) {
     out = (e < x[0]) || (e > x[n - 1]);
-    const double *bk = y + n;
+    const DP bk = y + n;
     const int nb = (int)bk[0];
     int lo = 0, hi = n;
     if (nb > 2) {
         const double t = (e - x[0]) * bk[1];
         const int b = t > 0.0 ? (t < (double)nb ? (int)t : nb - 1) : 0;
-        lo = (int)bk[2 + (b > 0 ? b - 1 : 0)];
-        hi = (int)bk[2 + (b + 2 < nb ? b + 2 : nb)];
+        lo = (int)first[b > 0 ? b - 1 : 0];
+        hi = (int)first[b + 2 < nb ? b + 2 : nb];
     }
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -793,6 +796,21 @@ __device__ __forceinline__ double trans_interp(const double *x, const double *y,
     const int l = h - 1;
     const double slope = (y[h] - y[l]) / (x[h] - x[l]);
     return slope * (e - x[l]) + y[l];
+}
+
+// The L1 transmission at energy e from the table at sel_d offset `off` (nt
+// entries), or from its copy at byte `lds_off` of the LDS image (-1: none):
+// energies, transmissions, the bucket count and 1 / width as float64, then
+// the bucket starts as uint16.
+__device__ __forceinline__ double l1_transmission(const MxTables &T, const unsigned char *lds, double off,
+                                                  double lds_off, int nt, double energy, bool &out) {
+    if (lds_off >= 0.0) {
+        const double *x = (const double *)(lds + (int)lds_off);
+        const unsigned short *first = (const unsigned short *)(x + 2 * nt + 2);
+        return trans_interp(x, x + nt, first, nt, energy, out);
+    }
+    const cptr<double> x = cp(T.sel_d) + (int64_t)off;
+    return trans_interp(x, x + nt, x + 2 * nt + 2, nt, energy, out);
 }
 
 // The reference's scatter applies transforms3d's axangle2mat(axis, angle)
@@ -811,6 +829,40 @@ __device__ __forceinline__ d3 axangle_t(d3 ax, double ang, d3 v) {
     return mk(m00 * v.x + m10 * v.y + m20 * v.z,
               m01 * v.x + m11 * v.y + m21 * v.z,
               m02 * v.x + m12 * v.y + m22 * v.z);
+}
+
+// The draws of one stack interaction (slots 5 k + j, j = 0 membrane order,
+// 1 L1 order, 2 L1 open, 3 and 4 scatter), loaded together before the first
+// layer: one memory latency instead of one per layer.  Only the draws the
+// layers make are loaded, so that the slot-reuse counter counts as the
+// layers do; a stack has at most one layer of each type, and a radial layer
+// in a stack that also scatters draws its slots again when it runs.
+struct StackDraws {
+    double u[MX_STACK_DRAWS];
+    bool scat;   // the stack has a scatter layer
+};
+template <typename SRP>
+__device__ __forceinline__ StackDraws stack_draws(const MxTables &T, const MxPhotons &P, const Ray &R,
+                                                  const cptr<int> si, int prog, SRP sr, int k0, int64_t i,
+                                                  const Counters &cnt) {
+    unsigned ops = 0;
+    for (int k = 0; k < (prog & 15); ++k) ops |= 1u << ((prog >> (4 + 3 * k)) & 7);
+    StackDraws D;
+    for (int j = 0; j < MX_STACK_DRAWS; ++j) D.u[j] = 0.0;
+    D.scat = (ops & ((1u << MX_OP_SCAT_L2) | (1u << MX_OP_SCAT_CONST))) != 0;
+    if (ops & (1u << MX_OP_GRATING)) D.u[0] = draw(T, P, R, si[4], si[5], k0, i, cnt);
+    if (ops & (1u << MX_OP_L1)) {
+        D.u[1] = draw(T, P, R, si[4], si[5], k0 + 1, i, cnt);
+        D.u[2] = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
+    }
+    if (D.scat) {
+        D.u[3] = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
+        D.u[4] = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
+    } else if (ops & (1u << MX_OP_RADIAL)) {
+        if (sr[13] != 0.0) D.u[3] = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
+        if (sr[16] != 0.0) D.u[4] = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
+    }
+    return D;
 }
 
 // FlatStack.process_photons (marxs/optics/base.py:258-281) for one photon
@@ -837,7 +889,8 @@ __device__ __forceinline__ d3 axangle_t(d3 ax, double ang, d3 v) {
 // blaze centre and slope (7, 8), quality factor (9), L1 open fraction (10),
 // L2 open fraction and coefficient (11, 12), scatter parameter (13),
 // transmission table offset and length (14, 15), radial scatter sigma_perp
-// (16) and centre (17-19).
+// (16) and centre (17-19), byte offsets of the transmission table's copies in
+// the FP32 and FP64 LDS images (20, 21; -1: none).
 template <int PREC>
 __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons &P, Ray &R, const cptr<int> si,
                                              const double *et, const double *ep, const int *ei, double ly,
@@ -852,13 +905,14 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
     // the quality factor's order: the membrane's, or (no grating layer before
     // it) the photon's order column stage_i[49]
     int m_last = si[49] >= 0 ? P.icol[si[49]][i] : MX_ORDER_NONE;
+    const StackDraws D = stack_draws(T, P, R, si, prog, sr, k0, i, cnt);
     const int nops = prog & 15;
     for (int k = 0; k < nops; ++k) {
         const int op = (prog >> (4 + 3 * k)) & 7;
         if (op == MX_OP_GRATING || op == MX_OP_L1) {
             const bool l1 = op == MX_OP_L1;
             const double *gp = l1 ? sr : ep;
-            const double u = draw(T, P, R, si[4], si[5], k0 + (l1 ? 1 : 0), i, cnt);
+            const double u = l1 ? D.u[1] : D.u[0];
             int m; double ptot;
             const d3 d_old = d, pol_old = pol;
             if (!select_order<PREC>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds,
@@ -867,15 +921,14 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             const double blaze = diffract64<true>(et, gp, l1 ? ei[4] : ei[2], (l1 ? ei[5] : ei[3]) ? 1.0 : -1.0,
                                             energy, ly, m, d, pol);
             if (l1) {
-                const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
+                const double u2 = D.u[2];
                 if (u2 > sr[10]) {
                     d = d_old;
                     pol = pol_old;
                     m = 0;
-                    const int64_t off = (int64_t)sr[14];
                     const int nt = (int)sr[15];
                     bool out;
-                    ptot = trans_interp(T.sel_d + off, T.sel_d + off + nt, nt, energy, out);
+                    ptot = l1_transmission(T, lds, sr[14], sr[PREC == 32 ? 20 : 21], nt, energy, out);
                     if (out) count(cnt, MX_ST_TRANS_RANGE);
                 }
                 put_i(P, si[44], i, m);
@@ -906,8 +959,8 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             } else {
                 sigma = sr[13];
             }
-            const double u3 = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
-            const double u4 = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
+            const double u3 = D.u[3];
+            const double u4 = D.u[4];
             const double ang = inv_normal_cdf(u3) * sigma;
             const d3 pd = unit3(d);
             const d3 guess = fabs(pd.x) < 0.99999 ? mk(1.0, 0.0, 0.0) : mk(0.0, 1.0, 0.0);
@@ -922,11 +975,11 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             d3 nd = d;
             double a_in = 0.0, a_perp = 0.0;
             if (sr[13] != 0.0) {
-                a_in = inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 3, i, cnt)) * sr[13];
+                a_in = inv_normal_cdf(D.scat ? draw(T, P, R, si[4], si[5], k0 + 3, i, cnt) : D.u[3]) * sr[13];
                 nd = axangle_t(perpplane, a_in, d);
             }
             if (sr[16] != 0.0) {
-                a_perp = inv_normal_cdf(draw(T, P, R, si[4], si[5], k0 + 4, i, cnt)) * sr[16];
+                a_perp = inv_normal_cdf(D.scat ? draw(T, P, R, si[4], si[5], k0 + 4, i, cnt) : D.u[4]) * sr[16];
                 nd = axangle_t(radial, a_perp, nd);
             }
             pol = ptrans64(d, nd, pol);
@@ -984,16 +1037,21 @@ __device__ __forceinline__ double ipow(double b, int e) {
 //    made as (1 - c) p_x^2 < c (p_y^2 + p_z^2), c = 0.99999^2, which float
 //    resolves near the threshold (|p_x| itself rounds to 6e-8 there).
 __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhotons &P, Ray &R, const cptr<int> si,
-                                               const float *er, const float *pl, size_t eg, const int *ei,
+                                               const float *er, const float *pl, size_t eg, const int *ei_,
                                                float ly, int nhit, int64_t i, const Counters &cnt,
                                                const unsigned char *lds) {
 #pragma clang fp contract(fast)
     const int prog = si[47];
-    const double *sr = T.sel_d + ei[6];
+    // the element row and its stack record are read-only tables: read them
+    // through the constant address space, so that their loads may be issued
+    // ahead of the output-column stores of the layers before them
+    const cptr<int> ei = cp(ei_);
+    const cptr<double> sr = cp(T.sel_d) + ei[6];
     const f3 n = ldf3(er + 3);
     const double energy = R.energy;
     const int k0 = MX_STACK_DRAWS * nhit;
     int m_last = si[49] >= 0 ? P.icol[si[49]][i] : MX_ORDER_NONE;
+    const StackDraws D = stack_draws(T, P, R, si, prog, sr, k0, i, cnt);
     const int nops = prog & 15;
     for (int k = 0; k < nops; ++k) {
         const int op = (prog >> (4 + 3 * k)) & 7;
@@ -1009,7 +1067,7 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
                 prm[12] = __int_as_float(ei[3]);
                 pm = prm;
             }
-            const double u = draw(T, P, R, si[4], si[5], k0, i, cnt);
+            const double u = D.u[0];
             int m; double ptot;
             if (!select_order<32>(T, ei[1], energy, u, m, ptot, lds,
                                   [&]() { return (double)blaze32(n, pm, ly, R.df); }))
@@ -1027,8 +1085,8 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             prm[9] = prm[10] = 0.0f;
             prm[11] = __int_as_float(ei[4]);
             prm[12] = __int_as_float(ei[5]);
-            const double u = draw(T, P, R, si[4], si[5], k0 + 1, i, cnt);
-            const double u2 = draw(T, P, R, si[4], si[5], k0 + 2, i, cnt);
+            const double u = D.u[1];
+            const double u2 = D.u[2];
             int m; double ptot;
             if (!select_order<32>(T, ei[7], energy, u, m, ptot, lds,
                                   [&]() { return (double)blaze32(n, prm, ly, R.df); }))
@@ -1038,10 +1096,9 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
                 // through a bar: direction kept, blaze of the incoming ray
                 blaze = (double)blaze32(n, prm, ly, R.df);
                 m = 0;
-                const int64_t off = (int64_t)sr[14];
                 const int nt = (int)sr[15];
                 bool out;
-                ptot = trans_interp(T.sel_d + off, T.sel_d + off + nt, nt, energy, out);
+                ptot = l1_transmission(T, lds, sr[14], sr[20], nt, energy, out);
                 if (out) count(cnt, MX_ST_TRANS_RANGE);
             } else {
                 const float wave = fdiv_fast(MX_E2W_F, (float)energy);
@@ -1076,8 +1133,8 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             } else {
                 sigma = (float)sr[13];
             }
-            const double u3 = draw(T, P, R, si[4], si[5], k0 + 3, i, cnt);
-            const double u4 = draw(T, P, R, si[4], si[5], k0 + 4, i, cnt);
+            const double u3 = D.u[3];
+            const double u4 = D.u[4];
             const float ang = inv_normal_cdf32(u3) * sigma;
             const f3 pd = unitf(R.df);
             const float cc = 0.99999f * 0.99999f;
@@ -1104,11 +1161,13 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             f3 nd = R.df;
             float a_in = 0.0f, a_perp = 0.0f;
             if (sr[13] != 0.0) {
-                a_in = inv_normal_cdf32(draw(T, P, R, si[4], si[5], k0 + 3, i, cnt)) * (float)sr[13];
+                a_in = inv_normal_cdf32(D.scat ? draw(T, P, R, si[4], si[5], k0 + 3, i, cnt) : D.u[3]) *
+                       (float)sr[13];
                 nd = axangle_t32(perpplane, a_in, R.df);
             }
             if (sr[16] != 0.0) {
-                a_perp = inv_normal_cdf32(draw(T, P, R, si[4], si[5], k0 + 4, i, cnt)) * (float)sr[16];
+                a_perp = inv_normal_cdf32(D.scat ? draw(T, P, R, si[4], si[5], k0 + 4, i, cnt) : D.u[4]) *
+                         (float)sr[16];
                 nd = axangle_t32(radial, a_perp, nd);
             }
             R.polf = ptrans32(R.df, nd, R.polf);
@@ -1233,7 +1292,7 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
         // (FP64: 3.5 % faster; FP32 unchanged, so it keeps the plain loop)
         int jn = tb < te ? (int)items[tb] : -1;
         for (int t = tb; t < te; ++t) {
-            count(cnt, MX_ST_TESTS);
+            if (cnt.tests) count(cnt, MX_ST_TESTS);
             const int j0 = jn;
             jn = t + 1 < te ? (int)items[t + 1] : -1;
             const int j = j0 > last ? j0 : 0;
@@ -1257,7 +1316,7 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
 #endif
     for (int t = tb; t < te; t += G) {
         // candidate tests made (photon-element tests of the path)
-        if (G == 1) count(cnt, MX_ST_TESTS);
+        if (G == 1 && cnt.tests) count(cnt, MX_ST_TESTS);
         int jj[G];
         bool hh[G];
 #pragma unroll
@@ -1501,18 +1560,21 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             d3 popt = ld3(ep);
             double f = ep[6];
             double ang;   // angle between the old and the new direction
+            // offset of the reflectivity record (-1: none); a one-element
+            // stage reads it with a scalar load
+            const int refl = n_el == 1 ? cp(T.elem_i)[(size_t)e0 * MX_EI + 6] : ei[6];
             if (PREC == 64) {
                 d3 dn = unit3(R.d);
                 d3 fp = mk(popt.x + f * dn.x, popt.y + f * dn.y, popt.z + f * dn.z);
                 d3 nd = unit3(sub3(fp, q));
-                if (ei[6] >= 0) ang = acos64<EXT>(fabs(dot3(nd, dn)));
+                if (refl >= 0) ang = acos64<EXT>(fabs(dot3(nd, dn)));
                 R.pol = ptrans64(R.d, nd, R.pol);
                 R.d = nd;
             } else {
                 f3 dn = unitf(R.df);
                 d3 fp = mk(popt.x + f * (double)dn.x, popt.y + f * (double)dn.y, popt.z + f * (double)dn.z);
                 f3 nd = unitf(tof(sub3(fp, q)));
-                if (ei[6] >= 0) {
+                if (refl >= 0) {
                     // atan2(|a x b|, |a . b|) keeps the digits of small angles
                     const f3 cr = crossf(nd, dn);
                     ang = (double)atan2f(sqrtf(dotf(cr, cr)), fabsf(dotf(nd, dn)));
@@ -1520,10 +1582,10 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
                 R.polf = ptrans32(R.df, nd, R.polf);
                 R.df = nd;
             }
-            if (ei[6] >= 0) {
+            if (refl >= 0) {
                 // probability *= reflectivity(energy, angle / 4) ** 2
                 // (mirror.py:72-82), a degree-1 tensor spline
-                const double r = lens_reflectivity(T.sel_d + ei[6], R.energy, ang / 4);
+                const double r = lens_reflectivity(T.sel_d + refl, R.energy, ang / 4);
                 const double pf = r * r;
                 if (pf < 0.0 || pf > 1.0) count(cnt, MX_ST_PROB_RANGE);
                 R.prob = R.prob * pf;
@@ -1892,7 +1954,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons 
         for (int v = threadIdx.x; v < nvec; v += BS) dst[v] = src[v];
     }
     __syncthreads();
-    const Counters cnt = {s_count};
+    const Counters cnt = {s_count, (flags & MX_FLAG_COUNT_TESTS) != 0};
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
@@ -1932,7 +1994,7 @@ __global__ __launch_bounds__(BS) void mx_trace_regroup_kernel(MxTables T_arg, Mx
         for (int v = threadIdx.x; v < nvec; v += BS) dst[v] = src[v];
     }
     __syncthreads();
-    const Counters cnt = {s_count};
+    const Counters cnt = {s_count, (flags & MX_FLAG_COUNT_TESTS) != 0};
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t base = (int64_t)blockIdx.x * BS; base < P.n; base += stride)
@@ -1958,7 +2020,7 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
     __shared__ int s_var;
     __shared__ unsigned s_count[MX_NCOUNT];
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
-    const Counters cnt = {s_count};
+    const Counters cnt = {s_count, (flags & MX_FLAG_COUNT_TESTS) != 0};
     if (threadIdx.x < MX_NCOUNT) s_count[threadIdx.x] = 0;
     int cur = -1;
     for (;;) {

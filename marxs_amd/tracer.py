@@ -204,12 +204,14 @@ class CompiledInstrument:
 
     def launch(self, dp, uniforms=None, precision=64, status=None, stream=None,
                existing=(), grid=None, src=None, wg_per_cu=None, regroup=None,
-               regroup_max_frac=None):
+               regroup_max_frac=None, count_tests=False):
         """Launch the trace kernel on device photons (asynchronous).
 
         ``src``: optional DevicePhotons to read pos/dir/pol/prob from (left
         unchanged); results go to ``dp``.  Without it the trace is in place.
         ``regroup``: override of the instrument's ``regroup`` setting.
+        ``count_tests``: count the ray-plane tests into the status counter
+        ``tests`` (costs a few per cent).
         """
         prec = PREC[precision]
         if status is None:
@@ -224,6 +226,8 @@ class CompiledInstrument:
         flags = 0 if use_grid else 1
         if self.regroup if regroup is None else regroup:
             flags |= 2
+        if count_tests:
+            flags |= 4
         if regroup_max_frac is not None:
             # regroup only when at most this fraction of a workgroup's rays
             # can interact with the stage

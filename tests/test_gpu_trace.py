@@ -436,3 +436,30 @@ def test_perfect_lens_reflectivity_rejected_forms(gpu):
                 RectBivariateSpline(x, x, np.ones((5, 5)), kx=2, ky=2)):
         with pytest.raises(CompileError):
             _ci(mx.Sequence(elements=[mx.PerfectLens(focallength=10., reflectivity_interpolator=bad)]))
+
+
+def test_plane_test_counter(gpu):
+    """MX_FLAG_COUNT_TESTS counts the ray-plane tests: without the culling
+    grid every photon tests every element of every array stage once (the
+    scan after a hit goes on with the later elements), 1 + 200 + 16 per
+    photon on the 200-facet workload; with the grid far fewer; counting
+    changes no result."""
+    import torch
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    inst, parts = configs.cat_rowland_instrument(200)
+    n = 30011
+    for grid in (False, True):
+        ci = CompiledInstrument(inst, grid=grid)
+        ph, u = make_inputs(n, 17, ci.n_uniform_slots)
+        outs = []
+        for ct in (True, False):
+            dp = DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'], device=gpu)
+            st = ci.launch(dp, torch.as_tensor(u, device=gpu), 64, count_tests=ct).cpu().numpy()
+            outs.append((dp, st))
+        (a, sa), (b, sb) = outs
+        assert torch.equal(a.pos, b.pos) and torch.equal(a.icols['CCD'], b.icols['CCD'])
+        assert sb[7] == 0
+        if not grid:
+            assert sa[7] == n * (1 + 200 + 16)
+        else:
+            assert n < sa[7] < 0.05 * n * (1 + 200 + 16)

@@ -207,6 +207,35 @@ def test_transmission_bucket_index_brackets_the_search():
     assert np.array_equal(C._bucket_index(np.arange(7.)), [0., 0.])
 
 
+
+def test_transmission_table_lds_copies():
+    """Each image holds one copy of the L1 transmission table (energies,
+    transmissions, bucket count and 1 / width as float64, bucket starts as
+    uint16) at the byte offset every stack record names (words 20 / 21); an
+    image without room for it leaves -1 (the kernel reads sel_d)."""
+    inst, parts = configs.cat_rowland_instrument(
+        50, facet_class=CG.CATL1L2Stack, facet_kwargs={'l1_transmission': SI})
+    t = C.compile_tree(inst, n_draws=3)
+    st = t.stages[2]
+    recs = [t.sel_d[t.elem_i[e, 6]:t.elem_i[e, 6] + C.SR] for e in range(st.elem0, st.elem0 + 50)]
+    off, n = int(recs[0][14]), int(recs[0][15])
+    nb = int(t.sel_d[off + 2 * n])
+    assert nb == 4 * n
+    for col, img in ((20, t.lds_image32), (21, t.lds_image64)):
+        o = {int(r[col]) for r in recs}
+        assert len(o) == 1
+        o = o.pop()
+        assert o >= 0 and o % 16 == 0
+        f = img[o:o + 8 * (2 * n + 2)].view(np.float64)
+        assert np.array_equal(f, t.sel_d[off:off + 2 * n + 2])
+        first = img[o + 8 * (2 * n + 2):o + 8 * (2 * n + 2) + 2 * (nb + 1)].view(np.uint16)
+        assert np.array_equal(first, t.sel_d[off + 2 * n + 2:off + 2 * n + 3 + nb].astype(np.uint16))
+        assert first[-1] == n
+    # no room: the records name no copy
+    C._build_lds(t, budget32=20 * 1024, budget64=20 * 1024)
+    for r in (t.sel_d[t.elem_i[e, 6]:t.elem_i[e, 6] + C.SR] for e in range(st.elem0, st.elem0 + 50)):
+        assert r[20] == -1 and r[21] == -1
+
 def _interp_block(tables, k):
     """Grids, values and orders of interpolated selector k in the FP32 LDS
     image."""
