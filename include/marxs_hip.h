@@ -161,7 +161,10 @@ typedef struct MxTables {
     const unsigned char *lds_image32;  /* bytes copied to LDS by each workgroup, FP32 */
     const unsigned char *lds_image64;  /* the same for FP64 */
     int32_t first_draw_slot;   /* uniform slot of the first grating draw (-1: none), prefetched */
-    int32_t ext_stages;        /* 1: some stage is MX_STAGE_STACK or MX_STAGE_CYLINDER (selects the kernels that run them) */
+    int32_t ext_stages;        /* 1: some stage is MX_STAGE_STACK or MX_STAGE_CYLINDER, or a
+                                  MX_STAGE_LENS_PERFECT element has a reflectivity record
+                                  (elem_i[6] >= 0): selects the kernels that run them; the
+                                  other kernels count such a lens in MX_ST_BAD_ARGS */
     int32_t regroup_stages;    /* number of grating, stack or detector stages with a culling grid
                                   (stage_i[3] bit 0): with MX_FLAG_REGROUP, mx_trace regroups the
                                   rays of a workgroup before each so that waves are all

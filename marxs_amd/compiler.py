@@ -306,6 +306,18 @@ class CompiledTables:
     def n_elems(self):
         return self.elem_t.shape[0]
 
+    @property
+    def ext_stages(self):
+        """1 when the tables need the extended kernel: a stack or tube
+        stage, or a PerfectLens with a reflectivity table."""
+        for st in self.stages:
+            if st.kind in (STAGE_STACK, STAGE_CYLINDER):
+                return 1
+            if st.kind == STAGE_LENS_PERFECT and np.any(
+                    self.elem_i[st.elem0:st.elem0 + len(st.elements), 6] >= 0):
+                return 1
+        return 0
+
     def stage_i_for(self, existing):
         """stage records with output columns already present in the photon
         table switched to 'write on interaction only'."""

@@ -1561,8 +1561,15 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             double f = ep[6];
             double ang;   // angle between the old and the new direction
             // offset of the reflectivity record (-1: none); a one-element
-            // stage reads it with a scalar load
-            const int refl = n_el == 1 ? cp(T.elem_i)[(size_t)e0 * MX_EI + 6] : ei[6];
+            // stage reads it with a scalar load.  Reflectivity is applied by
+            // the extended kernel only (tables with such a lens set
+            // ext_stages); the plain kernel, which keeps the code out of its
+            // register allocation, reports a record as a bad argument.
+            int refl = n_el == 1 ? cp(T.elem_i)[(size_t)e0 * MX_EI + 6] : ei[6];
+            if (!EXT && refl >= 0) {
+                count(cnt, MX_ST_BAD_ARGS);
+                refl = -1;
+            }
             if (PREC == 64) {
                 d3 dn = unit3(R.d);
                 d3 fp = mk(popt.x + f * dn.x, popt.y + f * dn.y, popt.z + f * dn.z);

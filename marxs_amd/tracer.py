@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from .compiler import (compile_tree, flatten, CompileError, ORDER_NONE, STAGE_STACK,
-                       STAGE_CYLINDER, STAGE_GRATING, STAGE_DETECTOR)
+                       STAGE_GRATING, STAGE_DETECTOR)
 from .simulator import BaseContainer, Sequence
 from . import random as mxrandom
 
@@ -74,7 +74,7 @@ class DeviceTables:
                      'succ_items', 'lds_image32', 'lds_image64'):
             setattr(s, name, k[name].data_ptr())
         s.first_draw_slot = self.host.first_draw_slot
-        s.ext_stages = int(any(st.kind in (STAGE_STACK, STAGE_CYLINDER) for st in self.host.stages))
+        s.ext_stages = self.host.ext_stages
         s.regroup_stages = int(sum(1 for k in range(len(self.host.stages))
                                    if self.host.stage_i[k, 0] in (STAGE_GRATING, STAGE_DETECTOR, STAGE_STACK)
                                    and self.host.stage_i[k, 3] & 1))

@@ -427,6 +427,27 @@ def test_perfect_lens_reflectivity_parity(gpu):
     assert float(rel.max()) < 1e-5
 
 
+def test_perfect_lens_reflectivity_needs_extended_kernel(gpu):
+    """Tables with a reflectivity record select the extended kernel; the
+    plain kernel, given such tables with ext_stages cleared, leaves the
+    probability alone and reports the lens in the bad-argument counter."""
+    import torch
+    import marxs_amd as mx
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    interp = _refl_spline(lambda x, y: np.exp(-np.sqrt((x / 2) ** 2 + y ** 2)),
+                          np.linspace(-3, 3, 100), np.linspace(-3, 3, 100))
+    ci = CompiledInstrument(mx.Sequence(elements=[mx.PerfectLens(focallength=123.0,
+                                                                 reflectivity_interpolator=interp)]))
+    assert ci.dev_tables.struct.ext_stages == 1
+    ci.dev_tables.struct.ext_stages = 0
+    for prec in (64, 32):
+        dp = DevicePhotons(np.array([[-1., 0, 0]] * 3).T, np.array([[0., 1, 0]] * 3).T, np.ones(3),
+                           np.ones(3), device=gpu, pos=np.array([[1., 0, 0]] * 3).T)
+        st = ci.launch(dp, torch.zeros((0, 3), dtype=torch.float64, device=gpu), prec).cpu().numpy()
+        assert st[5] == 3
+        assert np.array_equal(dp.prob.cpu().numpy(), np.ones(3))
+
+
 def test_perfect_lens_reflectivity_rejected_forms(gpu):
     import marxs_amd as mx
     from marxs_amd.compiler import CompileError

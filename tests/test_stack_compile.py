@@ -347,6 +347,8 @@ def test_lens_reflectivity_table_compiles():
     assert np.array_equal(t.sel_d[o + 2:o + 2 + len(x)], x)
     t0 = C.compile_tree([mx.PerfectLens(focallength=100.)])
     assert t0.elem_i[0, 6] == -1
+    # a lens with reflectivity runs on the extended kernel
+    assert t.ext_stages == 1 and t0.ext_stages == 0
     for bad in (lambda e, a, grid=False: np.ones_like(e),
                 RectBivariateSpline(x, y, vals, kx=2, ky=2)):
         with pytest.raises(C.CompileError):
