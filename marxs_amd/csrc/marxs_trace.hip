@@ -331,11 +331,21 @@ __device__ __forceinline__ bool select_order_t(HP hd, DP eg, DP cdf, DP tot, IP 
 // over the orders, [wave][angle] (-1: the table has a negative value).
 #define MX_KEV_NM 1.2398419843320026
 __device__ __forceinline__ void hat1(const double *g, int n, double x, int &i, double &h0, double &h1) {
-    x = x < g[0] ? g[0] : (x > g[n - 1] ? g[n - 1] : x);
-    int lo = 0, hi = n - 1;               // largest i <= n - 2 with g[i] <= x
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (g[mid] <= x) lo = mid; else hi = mid;
+    const double g0 = g[0], gn = g[n - 1];
+    x = x < g0 ? g0 : (x > gn ? gn : x);
+    // largest i <= n - 2 with g[i] <= x: first guessed as for an evenly
+    // spaced grid (float arithmetic) and kept when g[i] <= x < g[i + 1] (or
+    // i = n - 2) holds; otherwise (an uneven grid, or x within rounding of a
+    // grid point) by bisection
+    const float t = (float)(x - g0) * ((float)(n - 1) * __builtin_amdgcn_rcpf((float)(gn - g0)));
+    int lo = t > 0.0f ? (t < (float)(n - 2) ? (int)t : n - 2) : 0;   // a NaN x gives 0
+    if (!(g[lo] <= x && (lo == n - 2 || x < g[lo + 1]))) {
+        lo = 0;
+        int hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (g[mid] <= x) lo = mid; else hi = mid;
+        }
     }
     i = lo;
     const double f = 1.0 / (g[i + 1] - g[i]);
@@ -399,8 +409,10 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
     double total = 0.0;
     for (int m = 0; m < no; ++m) {
         const DP d = dat + (size_t)m * stride;
-        double sp = 0.0;
-        sp = sp + (double)d[o00] * hx0 * hy0;
+        // FITPACK starts the sum at 0; 0 + t differs from t only for t = -0,
+        // and the sum then only enters `total` (from +0), so starting at the
+        // first term changes no result
+        double sp = (double)d[o00] * hx0 * hy0;
         sp = sp + (double)d[o01] * hx0 * hy1;
         sp = sp + (double)d[o10] * hx1 * hy0;
         sp = sp + (double)d[o11] * hx1 * hy1;
@@ -417,8 +429,7 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
     double run = 0.0;
     for (int m = 0; m < no; ++m) {
         const DP d = dat + (size_t)m * stride;
-        double sp = 0.0;
-        sp = sp + (double)d[o00] * hx0 * hy0;
+        double sp = (double)d[o00] * hx0 * hy0;
         sp = sp + (double)d[o01] * hx0 * hy1;
         sp = sp + (double)d[o10] * hx1 * hy0;
         sp = sp + (double)d[o11] * hx1 * hy1;
