@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 19
+#define MX_ABI_VERSION 20
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -200,6 +200,28 @@ typedef struct MxPhotons {
 int mx_trace(const MxTables *tables, const MxPhotons *photons,
              int32_t precision, int32_t flags, void *stream,
              unsigned long long *status);
+
+/* Split trace: mx_trace for tables whose extended stages are one array of
+ * stack elements (FlatStack.process_photons, marxs/optics/base.py:258-281,
+ * e.g. CATL1L2Stack facets, marxs/missions/mitsnl/catgrating.py:327-374),
+ * run as two kernels: the first traces every stage with the plain kernel's
+ * code and stops each photon at its first hit of stage split_stage, writing
+ * the ray there to a queue in `workspace`; the second runs the deferred layer
+ * programs, the rest of that stage and the later stages for the queued
+ * photons only.  FP64: the results (columns and status counters) equal
+ * mx_trace's bit for bit; FP32: indices and counters equal, float columns
+ * within float rounding of mx_trace's (the two kernels contract some float
+ * multiply-adds differently).  split_stage must be an MX_STAGE_STACK stage whose layer
+ * program has no MX_OP_RADIAL layer, and the other stages must need no
+ * extended kernel (a violation is counted in MX_ST_BAD_ARGS).  workspace:
+ * device memory of at least mx_trace_split_workspace(photons->n, precision)
+ * bytes, 256-byte aligned, owned by the caller; calls that share a workspace
+ * must not overlap.  No regrouping (MX_FLAG_REGROUP is ignored).  No
+ * reference counterpart: a scheduling variant of mx_trace. */
+int64_t mx_trace_split_workspace(int64_t n, int32_t precision);
+int mx_trace_split(const MxTables *tables, const MxPhotons *photons, int32_t precision,
+                   int32_t flags, int32_t split_stage, void *workspace, int64_t workspace_bytes,
+                   void *stream, unsigned long long *status);
 
 /* Ray pairs per exchange round of the regrouping kernel that mx_trace would
  * launch for these tables, precision and flags (MX_FLAG_REGROUP set; the

@@ -16,7 +16,7 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseAr
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 19
+MX_ABI_VERSION = 20
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -97,6 +97,12 @@ def _load():
                              ctypes.c_int32, ctypes.c_int32, P, P]
     lib.mx_trace_regroup_pairs.restype = ctypes.c_int
     lib.mx_trace_regroup_pairs.argtypes = [ctypes.POINTER(MxTables), ctypes.c_int32, ctypes.c_int32]
+    lib.mx_trace_split_workspace.restype = ctypes.c_int64
+    lib.mx_trace_split_workspace.argtypes = [ctypes.c_int64, ctypes.c_int32]
+    lib.mx_trace_split.restype = ctypes.c_int
+    lib.mx_trace_split.argtypes = [ctypes.POINTER(MxTables), ctypes.POINTER(MxPhotons), ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_void_p]
     lib.mx_fill_i32.restype = ctypes.c_int
     lib.mx_fill_i32.argtypes = [P, ctypes.c_int64, ctypes.c_int32, P]
     lib.mx_trace_batched.restype = ctypes.c_int

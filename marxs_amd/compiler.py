@@ -307,6 +307,31 @@ class CompiledTables:
         return self.elem_t.shape[0]
 
     @property
+    def split_stage(self):
+        """Stage at which a trace may be split into two kernels (the plain
+        kernel up to each photon's first hit there, the layer programs and the
+        later stages over the queued photons): the one stack stage, an array
+        of two or more elements whose layer program has no radial scatter
+        layer, when no other stage needs the extended kernel.  None
+        otherwise."""
+        stacks = [k for k, st in enumerate(self.stages) if st.kind == STAGE_STACK]
+        if len(stacks) != 1:
+            return None
+        k = stacks[0]
+        st = self.stages[k]
+        prog = int(self.stage_i[k, 47])
+        ops = [(prog >> (4 + 3 * j)) & 7 for j in range(prog & 15)]
+        if len(st.elements) < 2 or OP_RADIAL in ops:
+            return None
+        for other in self.stages:
+            if other.kind == STAGE_CYLINDER:
+                return None
+            if other.kind == STAGE_LENS_PERFECT and np.any(
+                    self.elem_i[other.elem0:other.elem0 + len(other.elements), 6] >= 0):
+                return None
+        return k
+
+    @property
     def ext_stages(self):
         """1 when the tables need the extended kernel: a stack or tube
         stage, or a PerfectLens with a reflectivity table."""

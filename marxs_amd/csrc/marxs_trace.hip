@@ -1363,12 +1363,119 @@ __device__ __forceinline__ int scan_next(ETP et, bool list, ITP items, int tb, i
     return -1;
 }
 
+// ------------------------------------------------------------- split trace
+// A trace can be split at one stack stage S: the first kernel runs every
+// stage with the plain kernel's code and, at S, stops each photon at its
+// first hit, appending the ray there to a queue; the second kernel runs the
+// deferred interactions (the layer program), the rest of stage S and the
+// later stages for the queued photons only, so that its waves are full.
+// The queue is a structure of arrays in the caller's workspace, split into
+// MX_SPLIT_REGIONS regions by the photon's block of the first kernel (region
+// (i / block) % regions), each with its own counter, so that the appends of
+// the waves go to 256 counters instead of one: a header of the counters (one
+// 128-byte line each), then, each array 256-byte aligned with `cap` =
+// regions x (entries per region) entries, photon index, hit point (3 x
+// float64), direction and polarization at the hit (3 x float32 each on the
+// FP32 path, 3 x float64 on the FP64 path), probability, the two local
+// coordinates (float64) and the element index.
+#define MX_SPLIT_REGIONS 256
+struct SplitQ {
+    unsigned long long *count;   // counter of region r at count[16 r]
+    int64_t per_region;          // entries per region
+    int64_t *idx;
+    double *q[3];
+    unsigned char *d[3], *pol[3];
+    double *prob, *ly, *lz;
+    int32_t *hit;
+};
+__host__ __device__ __forceinline__ int64_t split_align(int64_t b) { return (b + 255) & ~(int64_t)255; }
+// Entries per region for n photons: the blocks of the first kernel that fall
+// in one region, each of `block` photons.
+__host__ __device__ __forceinline__ int64_t split_per_region(int64_t n, int prec) {
+    const int64_t block = prec == 64 ? MX_BLOCK64 : MX_BLOCK32;
+    const int64_t nb = (n + block - 1) / block;
+    return (nb + MX_SPLIT_REGIONS - 1) / MX_SPLIT_REGIONS * block;
+}
+// Byte offsets of the counters and the queue arrays (16 of them, in the
+// order of SplitQ's arrays) and the total size, for n photons.
+__host__ __device__ __forceinline__ int64_t split_layout(int64_t n, int prec, int64_t *off) {
+    const int64_t cap = split_per_region(n, prec) * MX_SPLIT_REGIONS;
+    const int64_t ev = prec == 64 ? 8 : 4;
+    const int64_t sz[16] = {128 * MX_SPLIT_REGIONS, 8 * cap, 8 * cap, 8 * cap, 8 * cap, ev * cap, ev * cap,
+                            ev * cap, ev * cap, ev * cap, ev * cap, 8 * cap, 8 * cap, 8 * cap, 4 * cap, 0};
+    int64_t o = 0;
+    for (int k = 0; k < 16; ++k) {
+        if (off) off[k] = o;
+        o += split_align(sz[k]);
+    }
+    return o;
+}
+template <int PREC>
+__device__ __forceinline__ SplitQ split_queue(unsigned char *ws, int64_t n) {
+    int64_t o[16];
+    split_layout(n, PREC, o);
+    SplitQ Q;
+    Q.count = (unsigned long long *)(ws + o[0]);
+    Q.per_region = split_per_region(n, PREC);
+    Q.idx = (int64_t *)(ws + o[1]);
+    for (int k = 0; k < 3; ++k) {
+        Q.q[k] = (double *)(ws + o[2 + k]);
+        Q.d[k] = ws + o[5 + k];
+        Q.pol[k] = ws + o[8 + k];
+    }
+    Q.prob = (double *)(ws + o[11]);
+    Q.ly = (double *)(ws + o[12]);
+    Q.lz = (double *)(ws + o[13]);
+    Q.hit = (int32_t *)(ws + o[14]);
+    return Q;
+}
+// What array_stage needs in a split trace: the queue (first kernel), or the
+// deferred first hit of the photon (second kernel).
+struct SplitIO {
+    SplitQ Q;
+    int hit;
+    d3 q;
+    double ly, lz;
+};
+// Append the ray at its first hit of the split stage to its region of the
+// queue: one atomic per wave (the first active lane claims the slots of all
+// active lanes; a wave's photons lie in one block, hence in one region).
+template <int PREC>
+__device__ __forceinline__ void split_push(const SplitQ &Q, int64_t i, int hit, d3 q, double ly, double lz,
+                                           const Ray &R) {
+    const unsigned long long m = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int64_t region = (i / (PREC == 64 ? MX_BLOCK64 : MX_BLOCK32)) % MX_SPLIT_REGIONS;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(Q.count + 16 * region, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    const int64_t k = region * Q.per_region +
+                      (int64_t)(base + (unsigned long long)__popcll(m & ((1ULL << lane) - 1ULL)));
+    Q.idx[k] = i;
+    Q.q[0][k] = q.x; Q.q[1][k] = q.y; Q.q[2][k] = q.z;
+    if (PREC == 32) {
+        ((float *)Q.d[0])[k] = R.df.x; ((float *)Q.d[1])[k] = R.df.y; ((float *)Q.d[2])[k] = R.df.z;
+        ((float *)Q.pol[0])[k] = R.polf.x; ((float *)Q.pol[1])[k] = R.polf.y; ((float *)Q.pol[2])[k] = R.polf.z;
+    } else {
+        ((double *)Q.d[0])[k] = R.d.x; ((double *)Q.d[1])[k] = R.d.y; ((double *)Q.d[2])[k] = R.d.z;
+        ((double *)Q.pol[0])[k] = R.pol.x; ((double *)Q.pol[1])[k] = R.pol.y; ((double *)Q.pol[2])[k] = R.pol.z;
+    }
+    Q.prob[k] = R.prob;
+    Q.ly[k] = ly;
+    Q.lz[k] = lz;
+    Q.hit[k] = hit;
+}
+
 // One array stage (lens, grating facets, detectors) for one photon.  L: the
 // stage's records, parameters and culling lists are in LDS.
-template <int PREC, bool L, bool EXT>
-__device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &P, const int s,
+// SPLIT 1: the first kernel of a split trace at its split stage (returns
+// true when the photon's first hit was queued); 2: the second kernel, which
+// starts with the queued hit io->hit at io->q.
+template <int PREC, bool L, bool EXT, int SPLIT = 0>
+__device__ __forceinline__ bool array_stage(const MxTables &T, const MxPhotons &P, const int s,
                                             const int64_t i, const bool use_grid, const Counters &cnt,
-                                            const unsigned char *lds, Ray &R) {
+                                            const unsigned char *lds, Ray &R, const SplitIO *io = nullptr) {
     const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
     const cptr<double> sd = cp(T.stage_d) + (size_t)s * MX_SD;
     const cptr<float> sf = cp(T.stage_f) + (size_t)s * MX_SD;
@@ -1396,6 +1503,15 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
     const int *sul_g = T.succ_items + si[35];
     int last = -1, nhit = 0;
     for (;;) {
+        d3 q;
+        double ly, lz;
+        int hit;
+        if (SPLIT == 2 && nhit == 0) {
+            hit = io->hit;
+            q = io->q;
+            ly = io->ly;
+            lz = io->lz;
+        } else {
         // mode 0: nothing can be hit; 1: tight cell list; 2: every element
         // after `last`; 3: tight successors of the element hit last; 4: loose
         // cell list; 5: loose successors
@@ -1457,13 +1573,13 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         }
         if (mode == 2 && grid) count(cnt, MX_ST_GRID_FALLBACK);
         // ---- next hit in ascending element order
-        d3 q = mk(0, 0, 0);
-        double ly = 0, lz = 0;
+        q = mk(0, 0, 0);
+        ly = 0;
+        lz = 0;
         float kf = 0.0f;
         const bool list = mode != 2;
         const int tb = list ? cb : last + 1;
         const int te = list ? ce : n_el;
-        int hit;
         // global candidate lists: tight successors, loose successors, or
         // cell_items (tight and loose cell lists hold absolute indices)
         const int *items_g = mode == 3 ? su_g : (mode == 5 ? sul_g : T.cell_items);
@@ -1487,6 +1603,11 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
             const float kk = kadv + kf;
             q = mk(R.p.x + (double)kk * (double)R.df.x, R.p.y + (double)kk * (double)R.df.y,
                    R.p.z + (double)kk * (double)R.df.z);
+        }
+        if (SPLIT == 1) {
+            split_push<PREC>(io->Q, i, hit, q, ly, lz, R);
+            return true;
+        }
         }
 
         // ---- interaction with element `hit`
@@ -1629,6 +1750,7 @@ __device__ __forceinline__ void array_stage(const MxTables &T, const MxPhotons &
         if (last >= n_el - 1) break;
     }
     if (nhit == 0) put_defaults(P, si, i);
+    return false;
 }
 
 // ======================================================= the trace kernel
@@ -1982,6 +2104,182 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons 
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
 }
 
+// ------------------------------------------------------------- split trace
+// The split stage s in the first kernel: culling and the element scan, the
+// first hit queued; a photon with no hit gets the stage's default columns and
+// goes on.  The stage must be a stack stage whose layer program has no radial
+// scatter layer (that layer reads the ray's position before the hit, which
+// the queue does not keep); otherwise the photon is counted as a bad argument.
+template <int PREC>
+__device__ __forceinline__ bool split_stage_a(const MxTables &T, const MxPhotons &P, const int s,
+                                              const int64_t i, const bool use_grid, const Counters &cnt,
+                                              const unsigned char *lds, Ray &R, const SplitIO &io) {
+    const cptr<int> si = cp(T.stage_i) + (size_t)s * MX_SI;
+    const int prog = si[47];
+    bool ok = si[0] == MX_STAGE_STACK;
+    for (int k = 0; k < (prog & 15); ++k)
+        if (((prog >> (4 + 3 * k)) & 7) == MX_OP_RADIAL) ok = false;
+    if (!ok) {
+        count(cnt, MX_ST_BAD_ARGS);
+        return false;
+    }
+    if (si[PREC == 64 ? 21 : 20] >= 0)
+        return array_stage<PREC, true, false, 1>(T, P, s, i, use_grid, cnt, lds, R, &io);
+    return array_stage<PREC, false, false, 1>(T, P, s, i, use_grid, cnt, lds, R, &io);
+}
+
+// One photon in the first kernel: all stages, photons queued at the split
+// stage leave the kernel there (the second kernel finishes them).
+template <int PREC>
+__device__ __forceinline__ void trace_one_split_a(const MxTables &T, const MxPhotons &P, const int64_t i,
+                                                  const bool use_grid, const Counters &cnt,
+                                                  const unsigned char *lds, const int split, const SplitIO &io) {
+    Ray R;
+    init_ray(T, P, i, R);
+    for (int s_ = 0; s_ < T.n_stages; ++s_) {
+        const int s = __builtin_amdgcn_readfirstlane(s_);
+        if (s == split) {
+            if (split_stage_a<PREC>(T, P, s, i, use_grid, cnt, lds, R, io)) return;
+        } else {
+            run_stage<PREC, false>(T, P, s, use_grid, cnt, lds, R);
+        }
+    }
+    finish_ray<PREC>(P, R);
+}
+
+// One queued photon in the second kernel: the deferred interaction and the
+// rest of the split stage, then the later stages.
+template <int PREC>
+__device__ __forceinline__ void trace_one_split_b(const MxTables &T, const MxPhotons &P, const SplitQ &Q,
+                                                  const int64_t k, const bool use_grid, const Counters &cnt,
+                                                  const unsigned char *lds, const int split) {
+    const int64_t i = Q.idx[k];
+    Ray R;
+    R.i = i;
+    SplitIO io;
+    io.hit = Q.hit[k];
+    io.q = mk(Q.q[0][k], Q.q[1][k], Q.q[2][k]);
+    io.ly = Q.ly[k];
+    io.lz = Q.lz[k];
+    // the interaction reads no position before the hit (no radial layer)
+    R.p = io.q;
+    if (PREC == 32) {
+        R.df = mkf(((const float *)Q.d[0])[k], ((const float *)Q.d[1])[k], ((const float *)Q.d[2])[k]);
+        R.polf = mkf(((const float *)Q.pol[0])[k], ((const float *)Q.pol[1])[k], ((const float *)Q.pol[2])[k]);
+        R.d = tod(R.df);
+        R.pol = tod(R.polf);
+    } else {
+        R.d = mk(((const double *)Q.d[0])[k], ((const double *)Q.d[1])[k], ((const double *)Q.d[2])[k]);
+        R.pol = mk(((const double *)Q.pol[0])[k], ((const double *)Q.pol[1])[k], ((const double *)Q.pol[2])[k]);
+        R.df = tof(R.d);
+        R.polf = tof(R.pol);
+    }
+    R.prob = Q.prob[k];
+    R.energy = P.energy ? P.energy[i] : 1.0;
+    R.u_first = (T.first_draw_slot >= 0 && T.first_draw_slot < P.n_uniform_slots)
+                    ? P.uniforms[(int64_t)T.first_draw_slot * P.uniform_stride + i] : 0.0;
+    const cptr<int> si = cp(T.stage_i) + (size_t)split * MX_SI;
+    if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, true, 2>(T, P, split, i, use_grid, cnt, lds, R, &io);
+    else array_stage<PREC, false, true, 2>(T, P, split, i, use_grid, cnt, lds, R, &io);
+    for (int s_ = split + 1; s_ < T.n_stages; ++s_) {
+        const int s = __builtin_amdgcn_readfirstlane(s_);
+        run_stage<PREC, true>(T, P, s, use_grid, cnt, lds, R);
+    }
+    finish_ray<PREC>(P, R);
+}
+
+// The two kernels of a split trace (same prologue as mx_trace_kernel: the
+// argument structs read through the kernel-argument segment, the LDS image
+// staged by the workgroup).  The first runs with the plain kernels' block and
+// register budget; the second, with the layer code, over the queue.
+#define MX_SPLIT_PROLOGUE(BSZ)                                                                          \
+    static_assert(sizeof(MxTables) % alignof(MxPhotons) == 0, "kernel argument layout");                \
+    const __attribute__((address_space(4))) unsigned char *ka =                                         \
+        (const __attribute__((address_space(4))) unsigned char *)__builtin_amdgcn_kernarg_segment_ptr(); \
+    const MxTables &T = *(const MxTables *)((const __attribute__((address_space(4))) MxTables *)ka);     \
+    const MxPhotons &P =                                                                                \
+        *(const MxPhotons *)((const __attribute__((address_space(4))) MxPhotons *)(ka + sizeof(MxTables))); \
+    extern __shared__ __align__(16) unsigned char lds[];                                                \
+    __shared__ unsigned s_count[MX_NCOUNT];                                                             \
+    if (P.n != P_arg.n || P.prob != P_arg.prob || T.n_stages != T_arg.n_stages ||                       \
+        T.stage_i != T_arg.stage_i) {                                                                   \
+        if (status && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(status + MX_ST_BAD_ARGS, 1ULL);     \
+        return;                                                                                         \
+    }                                                                                                   \
+    const int lds_bytes = PREC == 64 ? T.lds_bytes64 : T.lds_bytes32;                                   \
+    if (threadIdx.x < MX_NCOUNT) s_count[threadIdx.x] = 0;                                              \
+    if (lds_bytes > 0) {                                                                                \
+        const uint4 *src = (const uint4 *)(PREC == 64 ? T.lds_image64 : T.lds_image32);                 \
+        uint4 *dst = (uint4 *)lds;                                                                      \
+        const int nvec = (lds_bytes + 15) >> 4;                                                         \
+        for (int v = threadIdx.x; v < nvec; v += (BSZ)) dst[v] = src[v];                                \
+    }                                                                                                   \
+    __syncthreads();                                                                                    \
+    const Counters cnt = {s_count, (flags & MX_FLAG_COUNT_TESTS) != 0};                                 \
+    const bool use_grid = !(flags & MX_FLAG_NO_GRID)
+
+template <int PREC, int BS>
+__global__ __launch_bounds__(BS) void mx_trace_split_a_kernel(MxTables T_arg, MxPhotons P_arg, int flags,
+                                                              unsigned long long *status, int split,
+                                                              unsigned char *ws) {
+    MX_SPLIT_PROLOGUE(BS);
+    SplitIO io;
+    io.Q = split_queue<PREC>(ws, P.n);
+    const int64_t stride = (int64_t)gridDim.x * BS;
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
+        trace_one_split_a<PREC>(T, P, i, use_grid, cnt, lds, split, io);
+    __syncthreads();
+    if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
+        atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
+}
+
+template <int PREC, int BS>
+__global__ __launch_bounds__(BS) void mx_trace_split_b_kernel(MxTables T_arg, MxPhotons P_arg, int flags,
+                                                              unsigned long long *status, int split,
+                                                              unsigned char *ws) {
+    MX_SPLIT_PROLOGUE(BS);
+    const SplitQ Q = split_queue<PREC>(ws, P.n);
+    // the regions' entries numbered consecutively: s_first[r] is the number
+    // of entries in the regions before r
+    // (a scan within each wave of the first MX_SPLIT_REGIONS threads, then
+    // the totals of the waves before)
+    __shared__ long long s_first[MX_SPLIT_REGIONS + 1];
+    __shared__ long long s_wave[MX_SPLIT_REGIONS / 64];
+    long long c = 0;
+    if (threadIdx.x < MX_SPLIT_REGIONS) {
+        const unsigned long long q = Q.count[16 * threadIdx.x];
+        c = q < (unsigned long long)Q.per_region ? (long long)q : Q.per_region;
+        const int lane = (int)(threadIdx.x & 63);
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long o = __shfl_up(c, d);
+            if (lane >= d) c += o;
+        }
+        if (lane == 63) s_wave[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < MX_SPLIT_REGIONS) {
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) c += s_wave[w];
+        s_first[threadIdx.x + 1] = c;
+        if (threadIdx.x == 0) s_first[0] = 0;
+    }
+    __syncthreads();
+    const int64_t total = s_first[MX_SPLIT_REGIONS];
+    const int64_t stride = (int64_t)gridDim.x * BS;
+    for (int64_t e = (int64_t)blockIdx.x * BS + threadIdx.x; e < total; e += stride) {
+        int lo = 0, hi = MX_SPLIT_REGIONS - 1;   // the last region r with s_first[r] <= e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_first[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        trace_one_split_b<PREC>(T, P, Q, (int64_t)lo * Q.per_region + (e - s_first[lo]), use_grid, cnt, lds,
+                                split);
+    }
+    __syncthreads();
+    if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
+        atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
+}
+#undef MX_SPLIT_PROLOGUE
+
 // The same trace with the rays regrouped before every regrouping stage:
 // each workgroup takes chunks of BS consecutive photons (one per thread,
 // grid stride) and exchanges rays through `half` slot pairs of LDS placed
@@ -2234,6 +2532,75 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     else
         hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32, false>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "kernel launch failed: %s", hipGetErrorString(err));
+        set_error(buf);
+        return 2;
+    }
+    return 0;
+}
+
+extern "C" int64_t mx_trace_split_workspace(int64_t n, int32_t precision) {
+    if (n < 0) { set_error("negative photon count"); return -1; }
+    if (precision != MX_PREC_FP64 && precision != MX_PREC_FP32) {
+        set_error("precision must be 64 or 32");
+        return -1;
+    }
+    return split_layout(n, precision, nullptr);
+}
+
+extern "C" int mx_trace_split(const MxTables *tables, const MxPhotons *photons, int32_t precision,
+                              int32_t flags, int32_t split_stage, void *workspace, int64_t workspace_bytes,
+                              void *stream, unsigned long long *status) {
+    if (check_tables(tables) || check_photons(photons)) return 1;
+    if (precision != MX_PREC_FP64 && precision != MX_PREC_FP32) {
+        set_error("precision must be 64 or 32");
+        return 1;
+    }
+    if (photons->n == 0 || tables->n_stages == 0) return 0;
+    if (split_stage < 0 || split_stage >= tables->n_stages) { set_error("split stage out of range"); return 1; }
+    if (!workspace || workspace_bytes < split_layout(photons->n, precision, nullptr)) {
+        set_error("split workspace NULL or smaller than mx_trace_split_workspace(n, precision)");
+        return 1;
+    }
+    if (((uintptr_t)workspace & 255) != 0) { set_error("split workspace not 256-byte aligned"); return 1; }
+    const int lds = precision == MX_PREC_FP64 ? tables->lds_bytes64 : tables->lds_bytes32;
+    if (lds < 0 || lds > 160 * 1024) { set_error("LDS request out of range"); return 1; }
+    if (lds > 0 && !(precision == MX_PREC_FP64 ? tables->lds_image64 : tables->lds_image32)) {
+        set_error("LDS image pointer is NULL");
+        return 1;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if (cus <= 0) cus = 256;
+    int per_cu = (flags >> 8) & 0xff;
+    if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
+    hipStream_t st = (hipStream_t)stream;
+    // the queue counters start at zero (a fill kernel, so that the call can
+    // be captured in a graph)
+    if (mx_fill_i32((int32_t *)workspace, 32 * MX_SPLIT_REGIONS, 0, stream) != 0) return 2;
+    unsigned char *ws = (unsigned char *)workspace;
+    const int block_a = precision == MX_PREC_FP64 ? MX_BLOCK64 : MX_BLOCK32;
+    const int block_b = precision == MX_PREC_FP64 ? MX_BLOCK_EXT : MX_BLOCK_EXT32;
+    // the second kernel: one workgroup per CU, each staging the LDS image once
+    int64_t grid_a = (int64_t)cus * per_cu, grid_b = cus;
+    const int64_t nblk_a = (photons->n + block_a - 1) / block_a, nblk_b = (photons->n + block_b - 1) / block_b;
+    if (grid_a > nblk_a) grid_a = nblk_a;
+    if (grid_b > nblk_b) grid_b = nblk_b;
+    if (precision == MX_PREC_FP64) {
+        hipLaunchKernelGGL((mx_trace_split_a_kernel<64, MX_BLOCK64>), dim3((unsigned)grid_a), dim3(block_a), lds, st,
+                           *tables, *photons, (int)flags, status, (int)split_stage, ws);
+        hipLaunchKernelGGL((mx_trace_split_b_kernel<64, MX_BLOCK_EXT>), dim3((unsigned)grid_b), dim3(block_b), lds, st,
+                           *tables, *photons, (int)flags, status, (int)split_stage, ws);
+    } else {
+        hipLaunchKernelGGL((mx_trace_split_a_kernel<32, MX_BLOCK32>), dim3((unsigned)grid_a), dim3(block_a), lds, st,
+                           *tables, *photons, (int)flags, status, (int)split_stage, ws);
+        hipLaunchKernelGGL((mx_trace_split_b_kernel<32, MX_BLOCK_EXT32>), dim3((unsigned)grid_b), dim3(block_b), lds,
+                           st, *tables, *photons, (int)flags, status, (int)split_stage, ws);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         char buf[256];

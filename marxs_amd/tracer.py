@@ -168,15 +168,26 @@ class CompiledInstrument:
         that waves are all interacting or all passing (identical results;
         off by default: slower on the workloads measured, because of the
         workgroup barriers it needs).
+    split : bool
+        Trace instruments with one array of stack elements (e.g.
+        ``CATL1L2Stack`` facets) as a split trace (``mx_trace_split``: the
+        plain kernel up to each photon's first stack hit, then the layer
+        programs over the queued photons only; FP64 results identical, FP32
+        within float rounding) where the tables allow it
+        (``tables.split_stage``).  Off by default: on the 600-facet
+        ``CATL1L2Stack`` workload it is slower in FP32 (2.48 against 2.00 ms
+        per 1e7 photons) and 4 % faster in FP64 (2.76 against 2.88 ms).
     """
 
-    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, **grid_kw):
+    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, split=False, **grid_kw):
         self.tree = tree
         self.tables = compile_tree(tree, n_draws=n_draws, grid=grid, **grid_kw)
         self.device = _dev(device)
         self.dev_tables = DeviceTables(self.tables, self.device)
         self.grid = grid
         self.regroup = regroup
+        self.split = split
+        self._split_ws = {}
         # persistent-grid size per precision (workgroups per CU, 0 = library
         # default); marxs_amd.autotune.tune_trace sets it from measurements
         self.wg_per_cu = {}
@@ -202,9 +213,29 @@ class CompiledInstrument:
         return torch.rand((s, n), dtype=torch.float64, device=self.device,
                           generator=generator or mxrandom.generator(self.device))
 
+    def split_workspace(self, n, precision, stream):
+        """Workspace of a split trace of n photons (device bytes, 256-byte
+        aligned), kept per precision and stream and grown as needed."""
+        need = int(_lib.lib.mx_trace_split_workspace(int(n), PREC[precision]))
+        if need < 0:
+            _lib.check(1)
+        key = (PREC[precision], int(stream or 0))
+        ws = self._split_ws.get(key)
+        if ws is None or ws.numel() < need + 256:
+            ws = torch.empty(need + 256, dtype=torch.uint8, device=self.device)
+            self._split_ws[key] = ws
+        off = (-ws.data_ptr()) % 256
+        return ws.data_ptr() + off, ws.numel() - off
+
+    def uses_split(self, regroup=None, split=None):
+        """Whether a launch with these settings runs the split trace."""
+        if self.regroup if regroup is None else regroup:
+            return False
+        return bool(self.split if split is None else split) and self.tables.split_stage is not None
+
     def launch(self, dp, uniforms=None, precision=64, status=None, stream=None,
                existing=(), grid=None, src=None, wg_per_cu=None, regroup=None,
-               regroup_max_frac=None, count_tests=False):
+               regroup_max_frac=None, count_tests=False, split=None):
         """Launch the trace kernel on device photons (asynchronous).
 
         ``src``: optional DevicePhotons to read pos/dir/pol/prob from (left
@@ -212,6 +243,7 @@ class CompiledInstrument:
         ``regroup``: override of the instrument's ``regroup`` setting.
         ``count_tests``: count the ray-plane tests into the status counter
         ``tests`` (costs a few per cent).
+        ``split``: override of the instrument's ``split`` setting.
         """
         prec = PREC[precision]
         if status is None:
@@ -234,8 +266,14 @@ class CompiledInstrument:
             flags |= (max(1, min(255, int(round(regroup_max_frac * 256)))) & 0xff) << 16
         wpc = self.wg_per_cu.get(prec, 0) if wg_per_cu is None else wg_per_cu
         flags |= (int(wpc) & 0xff) << 8
-        _lib.check(_lib.lib.mx_trace(ctypes.byref(tables), ctypes.byref(P), prec, flags,
-                                     ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
+        if self.uses_split(regroup, split):
+            ws, nb = self.split_workspace(dp.n, precision, stream)
+            _lib.check(_lib.lib.mx_trace_split(ctypes.byref(tables), ctypes.byref(P), prec, flags,
+                                               int(self.tables.split_stage), ctypes.c_void_p(ws), nb,
+                                               ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
+        else:
+            _lib.check(_lib.lib.mx_trace(ctypes.byref(tables), ctypes.byref(P), prec, flags,
+                                         ctypes.c_void_p(stream), ctypes.c_void_p(status.data_ptr())))
         dp.has_pos = True
         return status
 

@@ -80,6 +80,31 @@ def test_argument_errors_need_no_gpu(lib):
     assert rc != 0 and b'precision' in lib.lib.mx_last_error()
 
 
+def test_split_workspace_layout_and_errors_need_no_gpu(lib):
+    """mx_trace_split_workspace: 256 region counters of 128 bytes, then
+    photon index, hit point (3), direction (3) and polarization (3) at the hit
+    (float32 on the FP32 path), probability, two local coordinates and the
+    element index, each array 256-byte aligned, for 256 regions of whole
+    blocks of the first kernel (1024 photons FP32, 768 FP64)."""
+    def a(b):
+        return (b + 255) // 256 * 256
+    for n in (0, 1, 1000, 10_000_019):
+        for prec, ev, block in ((64, 8, 768), (32, 4, 1024)):
+            nb = -(-n // block)
+            cap = -(-nb // 256) * block * 256
+            assert cap >= n
+            want = 128 * 256 + a(8 * cap) * 4 + a(ev * cap) * 6 + a(8 * cap) * 3 + a(4 * cap)
+            assert lib.lib.mx_trace_split_workspace(n, prec) == want
+    assert lib.lib.mx_trace_split_workspace(-1, 64) < 0
+    assert b'negative' in lib.lib.mx_last_error()
+    assert lib.lib.mx_trace_split_workspace(10, 16) < 0
+    t = lib.MxTables()
+    p = lib.MxPhotons()
+    assert lib.lib.mx_trace_split(ctypes.byref(t), ctypes.byref(p), 16, 0, 0, None, 0, None, None) != 0
+    assert b'precision' in lib.lib.mx_last_error()
+    assert lib.lib.mx_trace_split(ctypes.byref(t), ctypes.byref(p), 32, 0, 0, None, 0, None, None) == 0
+
+
 def test_batched_argument_errors_need_no_gpu(lib):
     b = lib.MxBatch()
     assert lib.lib.mx_trace_batched(None, 64, 0, None) != 0

@@ -67,6 +67,27 @@ def test_stack_defaults_mask_and_stage_grouping():
     assert not (t.stage_i[1, 19] & 3)
 
 
+def test_split_stage():
+    """The stage a trace may be split at: the one stack array without a
+    radial layer, with no other stage needing the extended kernel."""
+    from marxs_amd.scatter import RadialMirrorScatter
+    inst, parts = configs.cat_rowland_instrument(50, facet_class=CG.CATL1L2Stack,
+                                                 facet_kwargs={'l1_transmission': SI})
+    assert C.compile_tree(inst).split_stage == 2
+    assert C.compile_tree(configs.cat_rowland_instrument(50)[0]).split_stage is None
+    sel = mx.OrderSelector([-1, 0, 1], [0.3, 0.4, 0.3])
+    a = CG.CATL1L2Stack(order_selector=sel, l1_transmission=SI, position=[60, 0, 0])
+    b = CG.CATL1L2Stack(order_selector=sel, l1_transmission=SI, position=[50, 0, 0])
+    det = mx.FlatDetector(position=[0, 0, 0])
+    assert C.compile_tree(mx.Sequence(elements=[a, b, det])).split_stage == 0
+    # a stack of one element, two stack stages, a radial layer
+    assert C.compile_tree(mx.Sequence(elements=[a, det])).split_stage is None
+    s = RandomGaussianScatter(scatter=1e-4, position=[40, 0, 0])
+    assert C.compile_tree(mx.Sequence(elements=[a, b, s, det])).split_stage is None
+    rad = RadialMirrorScatter(inplanescatter=1e-4, perpplanescatter=1e-5, position=[70, 0, 0])
+    assert C.compile_tree(mx.Sequence(elements=[rad, det])).split_stage is None
+
+
 def test_sigma_zero_scatter_has_no_operation():
     t = C.compile_tree(RandomGaussianScatter(scatter=0.))
     assert t.stages[0].kind == C.STAGE_STACK and t.stage_i[0, 47] == 0
