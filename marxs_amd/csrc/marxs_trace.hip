@@ -51,7 +51,16 @@
 #ifndef MX_BLOCK_EXT32
 #define MX_BLOCK_EXT32 768      /* the FP32 kernels with stack or tube stages */
 #endif
-#define MX_DEFAULT_WG_PER_CU 16 /* persistent grid: workgroups per CU         */
+// Default workgroups per CU of the persistent grid (one resides at a time:
+// the LDS image; more give finer load balance but stage the image again):
+// measured on the 600-facet workloads, 1e7 photons (FP32 / FP64 kernel ms at
+// 1, 2, 4, 16 per CU): plain facets 1.035 1.021 1.019 1.043 / 1.454 1.468
+// 1.470 1.509; interpolated efficiency 1.133 1.098 1.090 1.135 / 1.805
+// 1.802 1.813 1.902; CATL1L2Stack (extended kernel) 2.00 2.02 2.01 2.06 /
+// 2.88 2.89 2.92 3.05.
+#define MX_DEFAULT_WG_PER_CU32 4
+#define MX_DEFAULT_WG_PER_CU64 2
+#define MX_DEFAULT_WG_PER_CU_EXT 1
 #ifndef MX_REGROUP_DEFAULT_FRAC
 #define MX_REGROUP_DEFAULT_FRAC 256  /* exchange when <= this/256 of the rays interact */
 #endif
@@ -2487,7 +2496,9 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
     if (cus <= 0) cus = 256;
     int per_cu = (flags >> 8) & 0xff;   // workgroups per CU; 0 = default
-    if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
+    if (per_cu <= 0)
+        per_cu = ext ? MX_DEFAULT_WG_PER_CU_EXT
+                     : (precision == MX_PREC_FP64 ? MX_DEFAULT_WG_PER_CU64 : MX_DEFAULT_WG_PER_CU32);
     int64_t grid = (int64_t)cus * per_cu;
     if (grid > nblk) grid = nblk;
     // regrouping: the exchange buffer after the LDS image, at least 64 ray
@@ -2577,7 +2588,7 @@ extern "C" int mx_trace_split(const MxTables *tables, const MxPhotons *photons, 
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
     if (cus <= 0) cus = 256;
     int per_cu = (flags >> 8) & 0xff;
-    if (per_cu <= 0) per_cu = MX_DEFAULT_WG_PER_CU;
+    if (per_cu <= 0) per_cu = precision == MX_PREC_FP64 ? MX_DEFAULT_WG_PER_CU64 : MX_DEFAULT_WG_PER_CU32;
     hipStream_t st = (hipStream_t)stream;
     // the queue counters start at zero (a fill kernel, so that the call can
     // be captured in a graph)
