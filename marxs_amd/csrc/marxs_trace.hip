@@ -405,8 +405,8 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
     hat1(tg, nt, blaze, it, hy0, hy1);
     const size_t o00 = (size_t)iw * nt + it, o01 = o00 + 1, o10 = o00 + nt, o11 = o10 + 1;
     const size_t stride = (size_t)nw * nt;
-    double lim = __builtin_inf(), r0 = 0.0, r1 = 0.0;
-    int jc = -1;
+    double lim = __builtin_inf(), r0 = 0.0;
+    int jc = 0;
     if (tsum) {
         double est = 0.0;
         est = est + tsum[o00] * hx0 * hy0;
@@ -425,14 +425,27 @@ __device__ __forceinline__ bool select_interp(GP wg, int nw, GP tg, int nt, DP d
         sp = sp + (double)d[o01] * hx0 * hy1;
         sp = sp + (double)d[o10] * hx1 * hy0;
         sp = sp + (double)d[o11] * hx1 * hy1;
-        const double prev = total;
         total = total + sp;
-        if (jc < 0 && total > lim) { jc = m; r0 = prev; r1 = total; }
+        // with values >= 0 the running sums do not decrease, so the sums not
+        // above lim come first: jc counts them (the candidate's index) and r0
+        // keeps the last (the running sum before the candidate)
+        const bool le = !(total > lim);
+        jc += le ? 1 : 0;
+        r0 = le ? total : r0;
     }
-    if (jc >= 0 && !(r0 / total > u) && r1 / total > u) {
-        order = ords[jc];
-        ptot = total;
-        return true;
+    if (tsum && jc < no) {
+        // the candidate's running sum, r0 plus its value as the loop added it
+        const DP d = dat + (size_t)jc * stride;
+        double sp = (double)d[o00] * hx0 * hy0;
+        sp = sp + (double)d[o01] * hx0 * hy1;
+        sp = sp + (double)d[o10] * hx1 * hy0;
+        sp = sp + (double)d[o11] * hx1 * hy1;
+        const double r1 = r0 + sp;
+        if (!(r0 / total > u) && r1 / total > u) {
+            order = ords[jc];
+            ptot = total;
+            return true;
+        }
     }
     int j = 0;
     double run = 0.0;
