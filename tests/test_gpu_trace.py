@@ -427,6 +427,31 @@ def test_perfect_lens_reflectivity_parity(gpu):
     assert float(rel.max()) < 1e-5
 
 
+def test_perfect_lens_reflectivity_uneven_grid(gpu):
+    """Reflectivity on an uneven (geometric) energy grid and an uneven angle
+    grid, with photon energies on the grid points, one ulp either side and
+    beyond both ends: the kernel's interval search (a guess for an evenly
+    spaced grid, checked, else bisection) against scipy's spline in the
+    oracle, FP64 floats within 1e-9 relative."""
+    from scipy.interpolate import RectBivariateSpline
+    rng = np.random.default_rng(31)
+    e = np.geomspace(0.2, 2.5, 17)
+    a = np.array([0., 0.001, 0.0025, 0.003, 0.006, 0.02])
+    vals = (0.95 - 0.1 * e[:, None] - 8. * a[None, :]) * (1 - 0.02 * rng.random((len(e), len(a))))
+    interp = RectBivariateSpline(e, a, vals, kx=1, ky=1)
+    inst, parts = configs.cat_rowland_instrument(200)
+    parts['lens'].reflectivity_interpolator = interp
+    ci = _ci(inst)
+    n = 30000
+    ph, u = make_inputs(n, 32, ci.n_uniform_slots)
+    en = np.concatenate([e, np.nextafter(e, 0), np.nextafter(e, 10), [0.1, 3.0]])
+    ph['energy'] = en[rng.integers(0, len(en), n)]
+    ref = oracle_trace(inst, ph, u)
+    got, st = gpu_trace(ci, ph, u, precision=64)
+    compare(ref, got, rtol=RTOL, near=near_pixel_boundary(ref))
+    assert (ref['prob'] > 0).sum() > 1000
+
+
 def test_perfect_lens_reflectivity_needs_extended_kernel(gpu):
     """Tables with a reflectivity record select the extended kernel; the
     plain kernel, given such tables with ext_stages cleared, leaves the
