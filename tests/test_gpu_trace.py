@@ -2,7 +2,7 @@
 
 FP64 path: integer outputs (facet, order, CCD, pixel) bit-exact except for
 photons the oracle places within 1e-6 px of a pixel edge (counted and
-reported); every float column within 1e-9 relative (north-star tolerance).
+reported); every float column within 1e-9 relative.
 FP32 path: detector coordinates within 0.5 um RMS of the FP64 oracle.
 """
 import numpy as np
