@@ -617,6 +617,29 @@ def bench_stack(args, dev):
         b.record()
         torch.cuda.synchronize()
         ms[prec] = a.elapsed_time(b) / args.steps
+    # the opt-in split trace (two kernels with a photon queue between them);
+    # FP64 must equal the one-kernel trace bit for bit
+    split_ms, split_eq = {}, None
+    if ci.tables.split_stage is not None:
+        for prec in (32, 64):
+            o = outputs_like(src)
+            for _ in range(args.warmup):
+                ci.launch(o, uni, precision=prec, src=src, status=status, split=True)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(args.steps):
+                ci.launch(o, uni, precision=prec, src=src, status=status, split=True)
+            b.record()
+            torch.cuda.synchronize()
+            split_ms[prec] = a.elapsed_time(b) / args.steps
+            if prec == 64:
+                split_eq = (all(torch.equal(o.icols[c], outs[64].icols[c]) for c in o.icols)
+                            and all(torch.equal(torch.nan_to_num(o.fcols[c], nan=7.25),
+                                                torch.nan_to_num(outs[64].fcols[c], nan=7.25))
+                                    for c in o.fcols)
+                            and torch.equal(o.pos, outs[64].pos) and torch.equal(o.prob, outs[64].prob))
+            del o
     raise_for_status(status.cpu().numpy())
     o32, o64 = outs[32], outs[64]
     same = ((o32.icols['CCD'] == o64.icols['CCD']) & (o64.icols['CCD'] >= 0)
@@ -631,6 +654,11 @@ def bench_stack(args, dev):
            'kernel_ms_fp32': ms[32], 'kernel_ms_fp64': ms[64],
            'fp32_rms_um_vs_fp64': rms, 'photons_on_ccds': int((o64.icols['CCD'] >= 0).sum()),
            'through_l1_bars': int(((o64.icols['order_L1'] == 0) & (o64.icols['facet'] >= 0)).sum())}
+    if split_ms:
+        res['split_trace'] = {'kernel_ms_fp32': split_ms[32], 'kernel_ms_fp64': split_ms[64],
+                              'fp64_equals_one_kernel_bitwise': bool(split_eq),
+                              'what': 'opt-in two-kernel trace (mx_trace_split): the plain kernel up to the '
+                                      'first stack hit, the layer programs over the queued photons'}
     if args.stack_cpu_sample > 0:
         os.environ.setdefault('OMP_NUM_THREADS', '1')
         from oracle import trace as OT
