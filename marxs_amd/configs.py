@@ -32,7 +32,8 @@ from .optics import (CircleAperture, PerfectLens, FlatGrating, CATGrating,
 from .rowland import RowlandTorus, GratingArrayStructure, RectangularGrid
 from .simulator import Sequence, Parallel
 
-__all__ = ['flat_grid_instrument', 'cat_rowland_instrument', 'blaze_efficiency', 'design_variants',
+__all__ = ['flat_grid_instrument', 'cat_rowland_instrument', 'cat_stack_instrument', 'blaze_efficiency',
+           'design_variants',
            'onaxis_photons', 'line_spectrum_energies', 'uniform_energies',
            'CAT_ORDERS', 'LINES']
 
@@ -152,6 +153,15 @@ def cat_rowland_instrument(n_facets=200, R=5900., r=6000., blaze_deg=1.5,
     inst = Sequence(elements=[aper, lens, gas, ccd])
     return inst, {'aperture': aper, 'lens': lens, 'gratings': gas, 'ccds': ccd,
                   'rowland': rowland, 'efficiency': eff}
+
+
+def cat_stack_instrument(n_facets=600, **kw):
+    """``cat_rowland_instrument`` with ``CATL1L2Stack`` facets (membrane,
+    quality factor, L1 cross-support with the packaged Si transmission table,
+    L2 absorption and diffraction).  Returns (Sequence, parts dict)."""
+    from . import catgrating
+    return cat_rowland_instrument(n_facets, facet_class=catgrating.CATL1L2Stack,
+                                  facet_kwargs={'l1_transmission': catgrating.l1transtab_path}, **kw)
 
 
 def design_variants(n_variants=64, rng=None, R_range=(5000., 7000.), blaze_range=(1.0, 2.0),

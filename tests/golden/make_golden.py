@@ -3,7 +3,7 @@ numpy oracle, stored as inputs, pre-drawn uniforms and every output column
 (``<case>.npz``) plus the plain-data instrument description they were traced
 with and how it was built (``<case>.json``).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 import json
 import os
@@ -23,6 +23,7 @@ CASES = {
     # name: (builder, builder args, photons, seed, energies)
     'flat_grid_1000': ('flat_grid_instrument', {}, 1000, 31, 'uniform'),
     'cat_rowland_200_2000': ('cat_rowland_instrument', {'n_facets': 200}, 2000, 32, 'lines'),
+    'cat_stack_200_2000': ('cat_stack_instrument', {'n_facets': 200}, 2000, 33, 'lines'),
 }
 
 
@@ -32,8 +33,10 @@ def build(builder, kwargs):
     return inst[0] if isinstance(inst, tuple) else inst
 
 
-def main():
+def main(names=None):
     for name, (builder, kwargs, n, seed, energies) in CASES.items():
+        if names and name not in names:
+            continue
         inst = build(builder, kwargs)
         els = describe(inst)
         n_slots = OT.uniform_slots(els)[1]
@@ -53,4 +56,4 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    main(sys.argv[1:])

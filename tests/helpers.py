@@ -123,51 +123,72 @@ GOLDEN = __import__('os').path.join(__import__('os').path.dirname(__file__), 'go
 
 def encode_description(els):
     """Instrument description (oracle.describe output) as JSON-able data;
-    identical selector tables are stored once."""
-    sels, out = [], []
+    identical selector tables are stored once (also those of stack layers,
+    which nest as lists of dicts), and so are identical arrays of 32 or more
+    values (e.g. the L1 transmission table of every stack facet)."""
+    sels, arrays, seen = [], [], {}
 
     def arr(a):
         a = np.asarray(a)
-        return {'__array__': a.tolist(), 'dtype': str(a.dtype)}
+        e = {'__array__': a.tolist(), 'dtype': str(a.dtype)}
+        if a.size < 32:
+            return e
+        key = (str(a.dtype), a.shape, a.tobytes())
+        if key not in seen:
+            seen[key] = len(arrays)
+            arrays.append(e)
+        return {'__arrayref__': seen[key]}
 
-    for el in els:
-        d = {}
-        for k, v in el.items():
-            if k == 'selector':
-                enc = {kk: (arr(vv) if isinstance(vv, np.ndarray) else vv) for kk, vv in v.items()}
-                if enc not in sels:
-                    sels.append(enc)
-                d[k] = {'__selector__': sels.index(enc)}
-            elif isinstance(v, np.ndarray):
-                d[k] = arr(v)
-            elif isinstance(v, (np.integer,)):
-                d[k] = int(v)
-            elif isinstance(v, (np.floating,)):
-                d[k] = float(v)
-            elif isinstance(v, (np.bool_,)):
-                d[k] = bool(v)
-            else:
-                d[k] = v
-        out.append(d)
-    return {'selectors': sels, 'elements': out}
+    def enc(v):
+        if isinstance(v, np.ndarray):
+            return arr(v)
+        if isinstance(v, np.bool_):
+            return bool(v)
+        if isinstance(v, np.integer):
+            return int(v)
+        if isinstance(v, np.floating):
+            return float(v)
+        if isinstance(v, dict):
+            out = {}
+            for k, vv in v.items():
+                if k == 'selector' and isinstance(vv, dict):
+                    e = {kk: enc(x) for kk, x in vv.items()}
+                    if e not in sels:
+                        sels.append(e)
+                    out[k] = {'__selector__': sels.index(e)}
+                else:
+                    out[k] = enc(vv)
+            return out
+        if isinstance(v, (list, tuple)):
+            return [enc(x) for x in v]
+        return v
+
+    elements = [enc(el) for el in els]
+    out = {'selectors': sels, 'elements': elements}
+    if arrays:
+        out['arrays'] = arrays
+    return out
 
 
 def decode_description(obj):
+    arrays = obj.get('arrays', [])
+
     def dec(v):
-        if isinstance(v, dict) and '__array__' in v:
-            return np.array(v['__array__'], dtype=v['dtype'])
+        if isinstance(v, dict):
+            if '__array__' in v:
+                return np.array(v['__array__'], dtype=v['dtype'])
+            if '__arrayref__' in v:
+                a = arrays[v['__arrayref__']]
+                return np.array(a['__array__'], dtype=a['dtype'])
+            if '__selector__' in v:
+                return dict(sels[v['__selector__']])
+            return {k: dec(x) for k, x in v.items()}
+        if isinstance(v, list):
+            return [dec(x) for x in v]
         return v
-    sels = [{k: dec(v) for k, v in s.items()} for s in obj['selectors']]
-    els = []
-    for d in obj['elements']:
-        e = {}
-        for k, v in d.items():
-            if isinstance(v, dict) and '__selector__' in v:
-                e[k] = dict(sels[v['__selector__']])
-            else:
-                e[k] = dec(v)
-        els.append(e)
-    return els
+    sels = []
+    sels.extend({k: dec(v) for k, v in s.items()} for s in obj['selectors'])
+    return [dec(d) for d in obj['elements']]
 
 
 def load_golden(name):

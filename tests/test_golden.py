@@ -8,7 +8,7 @@ from helpers import load_golden, encode_description
 from oracle import trace as OT
 from oracle.describe import describe
 
-CASES = ['flat_grid_1000', 'cat_rowland_200_2000']
+CASES = ['flat_grid_1000', 'cat_rowland_200_2000', 'cat_stack_200_2000']
 
 
 @pytest.mark.parametrize('name', CASES)
@@ -28,16 +28,24 @@ def test_builder_still_gives_golden_description(name):
     inst = inst[0] if isinstance(inst, tuple) else inst
     now = describe(inst)
     assert len(now) == len(els)
+
+    def same(a, b, key=''):
+        # poses to 1e-9 mm (placement solves on the torus), all else exact;
+        # stack layers nest as lists of dicts
+        if isinstance(a, dict):
+            assert isinstance(b, dict) and set(a) == set(b), key
+            for k in a:
+                same(a[k], b[k], k)
+        elif isinstance(a, (list, tuple)) and not (a and isinstance(a[0], (int, float, str))):
+            assert len(a) == len(b), key
+            for x, y in zip(a, b):
+                same(x, y, key)
+        elif key == 'pos4d':
+            assert np.allclose(a, b, rtol=0, atol=1e-9), key
+        else:
+            assert np.array_equal(np.asarray(a), np.asarray(b)), key
     for a, b in zip(now, els):
-        assert set(a) == set(b)
-        for k in a:
-            if k == 'pos4d':
-                assert np.allclose(a[k], b[k], rtol=0, atol=1e-9)
-            elif k == 'selector':
-                for kk in a[k]:
-                    assert np.array_equal(np.asarray(a[k][kk]), np.asarray(b[k][kk])), kk
-            else:
-                assert a[k] == b[k] or (np.asarray(a[k]) == np.asarray(b[k])).all(), k
+        same(a, b)
 
 
 def test_golden_cases_exercise_the_path():

@@ -257,7 +257,7 @@ def test_cat_order_convention_table(gpu):
     assert np.all(m5['dir'][1:3, 1] < 0) and np.all(m5['dir'][3:, 1] > 0)
 
 
-@pytest.mark.parametrize('name', ['flat_grid_1000', 'cat_rowland_200_2000'])
+@pytest.mark.parametrize('name', ['flat_grid_1000', 'cat_rowland_200_2000', 'cat_stack_200_2000'])
 def test_golden_vectors(gpu, name):
     """The FP64 HIP path against the committed oracle vectors (the flat
     grid's zero order on a pixel corner: pixel indices there may differ,
