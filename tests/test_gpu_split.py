@@ -131,6 +131,19 @@ def test_split_multi_hit_and_in_place(gpu, prec):
     assert st[4] > 1000      # photons that met both stacks
 
 
+def test_split_without_culling_grid(gpu):
+    """Every element tested (no culling grid): the first kernel's scan and
+    the second kernel's continuation run over all elements; FP64 equal."""
+    from marxs_amd.tracer import CompiledInstrument
+    inst, parts = configs.cat_rowland_instrument(50, facet_class=CG.CATL1L2Stack,
+                                                 facet_kwargs={'l1_transmission': SI})
+    ci = CompiledInstrument(inst, grid=False, split=True)
+    ph, u = make_inputs(20011, 24, ci.n_uniform_slots, 'lines')
+    a, b = _both(ci, ph, u, 64, gpu)
+    _assert_equal(a, b)
+    assert int((b[0].icols['facet'] >= 0).sum()) > 500
+
+
 def test_split_no_hits_and_empty(gpu):
     """Photons that meet no facet (the queue stays empty) and a zero-length
     trace."""
