@@ -28,8 +28,9 @@ Two more sections run after it (``--sections``):
 ``stack``    the trace workload with every facet a CATL1L2Stack (membrane,
              quality factor, L1 cross-support with the Si bar transmission,
              L2 absorption and L2 diffraction), 1e7 photons: FP32 and FP64
-             rates, FP64 indices against the oracle on a sample, the
-             oracle's rate on that sample.
+             rates (FP64 as the split trace, the default there), the other
+             trace form of each precision, FP64 indices against the oracle on
+             a sample, the oracle's rate on that sample.
 ``interp``   the trace workload with a blaze-dependent
              InterpolateEfficiencyTable (wavelength x blaze angle, orders
              -12..2) as every facet's order selector: FP32 and FP64 rates,
@@ -617,22 +618,24 @@ def bench_stack(args, dev):
         b.record()
         torch.cuda.synchronize()
         ms[prec] = a.elapsed_time(b) / args.steps
-    # the opt-in split trace (two kernels with a photon queue between them);
-    # FP64 must equal the one-kernel trace bit for bit
+    # the other form of each precision: the one-kernel trace and the split
+    # trace (two kernels with a photon queue between them; the default for
+    # FP64); FP64 results must be equal bit for bit
     split_ms, split_eq = {}, None
     if ci.tables.split_stage is not None:
         for prec in (32, 64):
             o = outputs_like(src)
+            other = not ci.uses_split(precision=prec)
             for _ in range(args.warmup):
-                ci.launch(o, uni, precision=prec, src=src, status=status, split=True)
+                ci.launch(o, uni, precision=prec, src=src, status=status, split=other)
             torch.cuda.synchronize()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
             for _ in range(args.steps):
-                ci.launch(o, uni, precision=prec, src=src, status=status, split=True)
+                ci.launch(o, uni, precision=prec, src=src, status=status, split=other)
             b.record()
             torch.cuda.synchronize()
-            split_ms[prec] = a.elapsed_time(b) / args.steps
+            split_ms[prec] = (other, a.elapsed_time(b) / args.steps)
             if prec == 64:
                 split_eq = (all(torch.equal(o.icols[c], outs[64].icols[c]) for c in o.icols)
                             and all(torch.equal(torch.nan_to_num(o.fcols[c], nan=7.25),
@@ -655,10 +658,13 @@ def bench_stack(args, dev):
            'fp32_rms_um_vs_fp64': rms, 'photons_on_ccds': int((o64.icols['CCD'] >= 0).sum()),
            'through_l1_bars': int(((o64.icols['order_L1'] == 0) & (o64.icols['facet'] >= 0)).sum())}
     if split_ms:
-        res['split_trace'] = {'kernel_ms_fp32': split_ms[32], 'kernel_ms_fp64': split_ms[64],
-                              'fp64_equals_one_kernel_bitwise': bool(split_eq),
-                              'what': 'opt-in two-kernel trace (mx_trace_split): the plain kernel up to the '
-                                      'first stack hit, the layer programs over the queued photons'}
+        res['trace_form'] = {f'fp{p}': 'split' if not split_ms[p][0] else 'one kernel' for p in (32, 64)}
+        res['other_form'] = {
+            f'fp{p}': {'form': 'split' if split_ms[p][0] else 'one kernel', 'kernel_ms': split_ms[p][1]}
+            for p in (32, 64)}
+        res['other_form']['fp64_split_equals_one_kernel_bitwise'] = bool(split_eq)
+        res['other_form']['what'] = ('split: the plain kernel up to the first stack hit, the layer programs '
+                                     'over the queued photons (mx_trace_split); one kernel: mx_trace')
     if args.stack_cpu_sample > 0:
         os.environ.setdefault('OMP_NUM_THREADS', '1')
         from oracle import trace as OT

@@ -168,18 +168,19 @@ class CompiledInstrument:
         that waves are all interacting or all passing (identical results;
         off by default: slower on the workloads measured, because of the
         workgroup barriers it needs).
-    split : bool
+    split : bool or 'auto'
         Trace instruments with one array of stack elements (e.g.
         ``CATL1L2Stack`` facets) as a split trace (``mx_trace_split``: the
         plain kernel up to each photon's first stack hit, then the layer
         programs over the queued photons only; FP64 results identical, FP32
         within float rounding) where the tables allow it
-        (``tables.split_stage``).  Off by default: on the 600-facet
-        ``CATL1L2Stack`` workload it is slower in FP32 (2.48 against 2.00 ms
-        per 1e7 photons) and 4 % faster in FP64 (2.76 against 2.88 ms).
+        (``tables.split_stage``).  ``'auto'`` (the default) splits FP64
+        traces only: on the 600-facet ``CATL1L2Stack`` workload the split
+        trace is 4-5 % faster in FP64 (2.76 against 2.88 ms per 1e7 photons)
+        and slower in FP32 (2.48 against 2.00 ms).
     """
 
-    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, split=False, **grid_kw):
+    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, split='auto', **grid_kw):
         self.tree = tree
         self.tables = compile_tree(tree, n_draws=n_draws, grid=grid, **grid_kw)
         self.device = _dev(device)
@@ -227,11 +228,14 @@ class CompiledInstrument:
         off = (-ws.data_ptr()) % 256
         return ws.data_ptr() + off, ws.numel() - off
 
-    def uses_split(self, regroup=None, split=None):
+    def uses_split(self, regroup=None, split=None, precision=64):
         """Whether a launch with these settings runs the split trace."""
         if self.regroup if regroup is None else regroup:
             return False
-        return bool(self.split if split is None else split) and self.tables.split_stage is not None
+        sp = self.split if split is None else split
+        if sp == 'auto':
+            sp = PREC[precision] == 64
+        return bool(sp) and self.tables.split_stage is not None
 
     def launch(self, dp, uniforms=None, precision=64, status=None, stream=None,
                existing=(), grid=None, src=None, wg_per_cu=None, regroup=None,
@@ -266,7 +270,7 @@ class CompiledInstrument:
             flags |= (max(1, min(255, int(round(regroup_max_frac * 256)))) & 0xff) << 16
         wpc = self.wg_per_cu.get(prec, 0) if wg_per_cu is None else wg_per_cu
         flags |= (int(wpc) & 0xff) << 8
-        if self.uses_split(regroup, split):
+        if self.uses_split(regroup, split, precision):
             ws, nb = self.split_workspace(dp.n, precision, stream)
             _lib.check(_lib.lib.mx_trace_split(ctypes.byref(tables), ctypes.byref(P), prec, flags,
                                                int(self.tables.split_stage), ctypes.c_void_p(ws), nb,

@@ -1,7 +1,7 @@
-"""The opt-in split trace (mx_trace_split: the plain kernel up to each
-photon's first hit of the stack stage, the layer programs and later stages
-over the queued photons) against the one-kernel trace of the same tables
-(needs an MI355X).
+"""The split trace (mx_trace_split: the plain kernel up to each photon's
+first hit of the stack stage, the layer programs and later stages over the
+queued photons; the default for FP64 traces of stack arrays) against the
+one-kernel trace of the same tables (needs an MI355X).
 
 Photons are independent and the split changes only which kernel runs which
 part of a photon's trace.  FP64 (no contraction, the reference's operation
@@ -172,7 +172,10 @@ def test_split_not_used_where_tables_do_not_allow(gpu):
     inst, parts = configs.cat_rowland_instrument(50, facet_class=CG.CATL1L2Stack,
                                                  facet_kwargs={'l1_transmission': SI})
     assert not CompiledInstrument(inst, regroup=True, split=True).uses_split()
-    assert not CompiledInstrument(inst).uses_split()
+    assert not CompiledInstrument(inst, split=False).uses_split()
+    # the default splits FP64 traces only
+    ci = CompiledInstrument(inst)
+    assert ci.uses_split(precision=64) and not ci.uses_split(precision=32)
 
 
 def test_split_oracle_parity_fp64(gpu):
