@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MX_ABI_VERSION 20
+#define MX_ABI_VERSION 21
 
 #define MX_MAX_FCOLS 16
 #define MX_MAX_ICOLS 8
@@ -170,7 +170,12 @@ typedef struct MxTables {
                                   (stage_i[3] bit 0): with MX_FLAG_REGROUP, mx_trace regroups the
                                   rays of a workgroup before each so that waves are all
                                   interacting or all passing; 0 = never regroup */
-    int32_t reserved;
+    int32_t blaze_stages;      /* number of grating or stack stages with an order selector that
+                                  depends on the blaze angle (interpolated efficiency table;
+                                  stage_i[50] = 1): mx_trace then runs plain kernels that form
+                                  the angle once for the selector and the diffraction; 0 = the
+                                  kernels that form it in the diffraction (FP64 results are
+                                  identical either way) */
 } MxTables;
 
 typedef struct MxPhotons {

@@ -16,7 +16,7 @@ __all__ = ['lib', 'MxTables', 'MxPhotons', 'MxAdjointArgs', 'MxBatch', 'MxPoseAr
 MX_MAX_FCOLS = 16
 MX_MAX_ICOLS = 8
 MX_NSTATUS = 8
-MX_ABI_VERSION = 20
+MX_ABI_VERSION = 21
 
 LIB_PATH = os.environ.get('MARXS_AMD_LIB') or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), 'libmarxs_hip.so')
@@ -32,7 +32,7 @@ class MxTables(ctypes.Structure):
                 ('sel_d', P), ('sel_o', P), ('cell_start', P), ('cell_items', P),
                 ('succ_start', P), ('succ_items', P), ('lds_image32', P), ('lds_image64', P),
                 ('first_draw_slot', ctypes.c_int32), ('ext_stages', ctypes.c_int32),
-                ('regroup_stages', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+                ('regroup_stages', ctypes.c_int32), ('blaze_stages', ctypes.c_int32)]
 
 
 class MxPhotons(ctypes.Structure):

@@ -627,6 +627,16 @@ def compile_tree(tree, n_draws=2, grid=True, grid_min_elements=4, **grid_kw):
                 if name not in fcols:
                     fcols.append(name)
                 rec[slot] = fcols.index(name)
+        # 1: an order selector of the stage (a grating's, or a stack's
+        # membrane or L1 selector) depends on the blaze angle (interpolated
+        # table); the kernel then hands the angle the selector formed on to
+        # the diffraction instead of forming it again
+        rec[50] = 0
+        if st.kind in (STAGE_GRATING, STAGE_STACK):
+            blaze_rows = {r for r, row in enumerate(sel_rows) if row[8] & 8}
+            cols = (1, 7) if st.kind == STAGE_STACK else (1,)
+            rec[50] = int(any(int(elem_i[k, c]) in blaze_rows
+                              for k in range(st.elem0, e) for c in cols))
         if st.kind == STAGE_STACK and qf_reads_order_column(st.program):
             # the quality factor of this stack reads the 'order' column
             if icol_kind.get('order', 'order') != 'order':

@@ -734,20 +734,26 @@ __device__ __forceinline__ float blaze32(f3 n, FP pr, float ly, f3 d) {
 }
 
 // FlatGrating.diffract_photons (marxs/optics/grating.py:233-266), FP64 in
-// the reference's operation order.  Returns the blaze angle.
+// the reference's operation order.  Returns the blaze angle; have_blaze:
+// blaze_in is that angle already (blaze64 of the same ray and facet, which
+// an interpolated order selector computed), so it is not formed again.
 template <bool CALL = false>
 __device__ __forceinline__ double diffract64(const double *et, const double *ep, int sign_mode,
                                              double trans, double energy, double ly, int m,
-                                             d3 &d, d3 &pol) {
+                                             d3 &d, d3 &pol, const bool have_blaze = false,
+                                             const double blaze_in = 0.0) {
     d3 pp = unit3(d);
     d3 l = ld3(ep), dg = ld3(ep + 3), n = ld3(et + 3);
     double wave = MX_E2W / energy;
     double pl = dot3(pp, l);
-    d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
-    double cb = fabs(dot3(pperp, n));
-    cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
-    double blaze = acos64<CALL>(cb);
-    blaze = blaze + (ep[7] + ly * ep[8]);
+    double blaze = blaze_in;
+    if (!have_blaze) {
+        d3 pperp = unit3(mk(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
+        double cb = fabs(dot3(pperp, n));
+        cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
+        blaze = acos64<CALL>(cb);
+        blaze = blaze + (ep[7] + ly * ep[8]);
+    }
     double pd0 = dot3(pp, dg);
     double sgn;
     if (sign_mode == 0) sgn = -1.0;
@@ -768,21 +774,108 @@ __device__ __forceinline__ double diffract64(const double *et, const double *ep,
     return blaze;
 }
 
+// The blaze angle with the unit direction and its component along the
+// grooves (the first steps of diffract64, the same operations), and the rest
+// of the diffraction given them: an order selector that depends on the blaze
+// angle and the diffraction then share one evaluation.
+struct BlazeParts64 {
+    d3 pp;
+    double pl, blaze;
+};
+template <bool CALL = false>
+__device__ __forceinline__ BlazeParts64 blaze_parts64(const double *et, const double *ep, double ly, d3 d) {
+    BlazeParts64 b;
+    b.pp = unit3(d);
+    const d3 l = ld3(ep), n = ld3(et + 3);
+    b.pl = dot3(b.pp, l);
+    const d3 pperp = unit3(mk(b.pp.x - b.pl * l.x, b.pp.y - b.pl * l.y, b.pp.z - b.pl * l.z));
+    double cb = fabs(dot3(pperp, n));
+    cb = cb < 0.0 ? 0.0 : (cb > 1.0 ? 1.0 : cb);
+    b.blaze = acos64<CALL>(cb) + (ep[7] + ly * ep[8]);
+    return b;
+}
+__device__ __forceinline__ void diffract_rest64(const double *et, const double *ep, int sign_mode, double trans,
+                                                double energy, int m, const BlazeParts64 &b, d3 &d, d3 &pol) {
+    const d3 pp = b.pp;
+    const d3 l = ld3(ep), dg = ld3(ep + 3), n = ld3(et + 3);
+    const double wave = MX_E2W / energy;
+    const double pd0 = dot3(pp, dg);
+    double sgn;
+    if (sign_mode == 0) sgn = -1.0;
+    else sgn = pd0 > 0.0 ? 1.0 : (pd0 < 0.0 ? -1.0 : (pd0 == 0.0 ? 1.0 : pd0));
+    const double t = (sgn * (double)m) * wave;
+    const double p_d = pd0 + t / ep[6];
+    const double p_l = b.pl;
+    const double p_n = sqrt(1. - p_d * p_d - p_l * p_l);
+    const double pn0 = dot3(pp, n);
+    double dirn = pn0 > 0.0 ? 1.0 : (pn0 < 0.0 ? -1.0 : (pn0 == 0.0 ? 0.0 : pn0));
+    dirn = dirn * trans;
+    const double w = dirn * p_n;
+    const d3 nd = mk(p_d * dg.x + p_l * l.x + w * n.x,
+                     p_d * dg.y + p_l * l.y + w * n.y,
+                     p_d * dg.z + p_l * l.z + w * n.z);
+    pol = ptrans64(d, nd, pol);
+    d = nd;
+}
+struct BlazeParts32 {
+    f3 pp;
+    float pl, blaze;
+};
+template <typename FP>
+__device__ __forceinline__ BlazeParts32 blaze_parts32(f3 n, FP pr, float ly, f3 d) {
+#pragma clang fp contract(fast)
+    BlazeParts32 b;
+    b.pp = unitf(d);
+    const f3 l = mkf(pr[0], pr[1], pr[2]);
+    b.pl = dotf(b.pp, l);
+    const f3 pperp = unitf(mkf(b.pp.x - b.pl * l.x, b.pp.y - b.pl * l.y, b.pp.z - b.pl * l.z));
+    float cb = fabsf(dotf(pperp, n));
+    cb = cb > 1.0f ? 1.0f : cb;
+    b.blaze = acosf(cb) + (pr[7] + ly * pr[8]);
+    return b;
+}
+template <typename FP>
+__device__ __forceinline__ void diffract_rest32(f3 n, FP pr, float wave, int m, const BlazeParts32 &b, f3 &d,
+                                                f3 &pol) {
+#pragma clang fp contract(fast)
+    const f3 pp = b.pp;
+    const f3 l = mkf(pr[0], pr[1], pr[2]), dg = mkf(pr[3], pr[4], pr[5]);
+    const int sign_mode = __float_as_int(pr[11]);
+    const float trans = __float_as_int(pr[12]) ? 1.0f : -1.0f;
+    const float pl = b.pl;
+    const float pd0 = dotf(pp, dg);
+    const float sgn = sign_mode == 0 ? -1.0f : (pd0 < 0.0f ? -1.0f : 1.0f);
+    const float p_d = pd0 + fdiv_fast((sgn * (float)m) * wave, pr[6]);
+    const float p_n = sqrtf(1.0f - p_d * p_d - pl * pl);
+    const float pn0 = dotf(pp, n);
+    const float w = (pn0 > 0.0f ? 1.0f : (pn0 < 0.0f ? -1.0f : 0.0f)) * trans * p_n;
+    const f3 nd = mkf(p_d * dg.x + pl * l.x + w * n.x,
+                      p_d * dg.y + pl * l.y + w * n.y,
+                      p_d * dg.z + pl * l.z + w * n.z);
+    pol = ptrans32(d, nd, pol);
+    d = nd;
+}
+
 // The same in float.  n: facet normal; pr: parameter record (l, d, period,
 // blaze centre, blaze slope as f32; sign mode and transmission as i32 in
-// words 11, 12).  Returns the blaze angle including the modifier.
+// words 11, 12).  Returns the blaze angle including the modifier (given as
+// blaze_in when have_blaze, from blaze32 of the same ray and facet).
 template <typename FP>
-__device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, int m, f3 &d, f3 &pol) {
+__device__ __forceinline__ float diffract32(f3 n, FP pr, float wave, float ly, int m, f3 &d, f3 &pol,
+                                            const bool have_blaze = false, const float blaze_in = 0.0f) {
 #pragma clang fp contract(fast)
     f3 pp = unitf(d);
     f3 l = mkf(pr[0], pr[1], pr[2]), dg = mkf(pr[3], pr[4], pr[5]);
     const int sign_mode = __float_as_int(pr[11]);
     const float trans = __float_as_int(pr[12]) ? 1.0f : -1.0f;
     float pl = dotf(pp, l);
-    f3 pperp = unitf(mkf(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
-    float cb = fabsf(dotf(pperp, n));
-    cb = cb > 1.0f ? 1.0f : cb;
-    float blaze = acosf(cb) + (pr[7] + ly * pr[8]);
+    float blaze = blaze_in;
+    if (!have_blaze) {
+        f3 pperp = unitf(mkf(pp.x - pl * l.x, pp.y - pl * l.y, pp.z - pl * l.z));
+        float cb = fabsf(dotf(pperp, n));
+        cb = cb > 1.0f ? 1.0f : cb;
+        blaze = acosf(cb) + (pr[7] + ly * pr[8]);
+    }
     float pd0 = dotf(pp, dg);
     float sgn = sign_mode == 0 ? -1.0f : (pd0 < 0.0f ? -1.0f : 1.0f);
     float p_d = pd0 + fdiv_fast((sgn * (float)m) * wave, pr[6]);
@@ -948,11 +1041,18 @@ __device__ __forceinline__ void stack_layers(const MxTables &T, const MxPhotons 
             const double u = l1 ? D.u[1] : D.u[0];
             int m; double ptot;
             const d3 d_old = d, pol_old = pol;
-            if (!select_order<PREC>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds,
-                                    [&]() { return blaze64<true>(et, gp, ly, d_old); }))
+            // an interpolated selector's blaze angle is handed on to the
+            // diffraction (formed there only when the selector did not)
+            double bz = 0.0;
+            bool hb = false;
+            if (!select_order<PREC>(T, l1 ? ei[7] : ei[1], energy, u, m, ptot, lds, [&]() {
+                    bz = blaze64<true>(et, gp, ly, d_old);
+                    hb = true;
+                    return bz;
+                }))
                 count(cnt, MX_ST_ORDER_EMPTY);
             const double blaze = diffract64<true>(et, gp, l1 ? ei[4] : ei[2], (l1 ? ei[5] : ei[3]) ? 1.0 : -1.0,
-                                            energy, ly, m, d, pol);
+                                                  energy, ly, m, d, pol, hb, bz);
             if (l1) {
                 const double u2 = D.u[2];
                 if (u2 > sr[10]) {
@@ -1102,11 +1202,16 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             }
             const double u = D.u[0];
             int m; double ptot;
-            if (!select_order<32>(T, ei[1], energy, u, m, ptot, lds,
-                                  [&]() { return (double)blaze32(n, pm, ly, R.df); }))
+            float bz = 0.0f;
+            bool hb = false;
+            if (!select_order<32>(T, ei[1], energy, u, m, ptot, lds, [&]() {
+                    bz = blaze32(n, pm, ly, R.df);
+                    hb = true;
+                    return (double)bz;
+                }))
                 count(cnt, MX_ST_ORDER_EMPTY);
             const float wave = fdiv_fast(MX_E2W_F, (float)energy);
-            const double blaze = (double)diffract32(n, pm, wave, ly, m, R.df, R.polf);
+            const double blaze = (double)diffract32(n, pm, wave, ly, m, R.df, R.polf, hb, bz);
             m_last = m;
             put_i(P, si[13], i, m);
             put_f(P, si[14], i, blaze);
@@ -1121,13 +1226,18 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
             const double u = D.u[1];
             const double u2 = D.u[2];
             int m; double ptot;
-            if (!select_order<32>(T, ei[7], energy, u, m, ptot, lds,
-                                  [&]() { return (double)blaze32(n, prm, ly, R.df); }))
+            float bz = 0.0f;
+            bool hb = false;
+            if (!select_order<32>(T, ei[7], energy, u, m, ptot, lds, [&]() {
+                    bz = blaze32(n, prm, ly, R.df);
+                    hb = true;
+                    return (double)bz;
+                }))
                 count(cnt, MX_ST_ORDER_EMPTY);
             double blaze;
             if (u2 > sr[10]) {
                 // through a bar: direction kept, blaze of the incoming ray
-                blaze = (double)blaze32(n, prm, ly, R.df);
+                blaze = (double)(hb ? bz : blaze32(n, prm, ly, R.df));
                 m = 0;
                 const int nt = (int)sr[15];
                 bool out;
@@ -1135,7 +1245,7 @@ __device__ __forceinline__ void stack_layers32(const MxTables &T, const MxPhoton
                 if (out) count(cnt, MX_ST_TRANS_RANGE);
             } else {
                 const float wave = fdiv_fast(MX_E2W_F, (float)energy);
-                blaze = (double)diffract32(n, prm, wave, ly, m, R.df, R.polf);
+                blaze = (double)diffract32(n, prm, wave, ly, m, R.df, R.polf, hb, bz);
             }
             put_i(P, si[44], i, m);
             put_f(P, si[45], i, blaze);
@@ -1494,7 +1604,7 @@ __device__ __forceinline__ void split_push(const SplitQ &Q, int64_t i, int hit, 
 // SPLIT 1: the first kernel of a split trace at its split stage (returns
 // true when the photon's first hit was queued); 2: the second kernel, which
 // starts with the queued hit io->hit at io->q.
-template <int PREC, bool L, bool EXT, int SPLIT = 0>
+template <int PREC, bool L, bool EXT, int SPLIT = 0, bool BZ = EXT>
 __device__ __forceinline__ bool array_stage(const MxTables &T, const MxPhotons &P, const int s,
                                             const int64_t i, const bool use_grid, const Counters &cnt,
                                             const unsigned char *lds, Ray &R, const SplitIO *io = nullptr) {
@@ -1646,20 +1756,39 @@ __device__ __forceinline__ bool array_stage(const MxTables &T, const MxPhotons &
             double blaze;
             if (PREC == 64) {
                 id = ei[0];
-                if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds,
-                                        [&]() { return blaze64<EXT>(et, ep, ly, R.d); }))
-                    count(cnt, MX_ST_ORDER_EMPTY);
-                blaze = diffract64<EXT>(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
+                if (BZ) {
+                    // one blaze angle for the selector and the diffraction
+                    const BlazeParts64 bp = blaze_parts64<EXT>(et, ep, ly, R.d);
+                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds, [&]() { return bp.blaze; }))
+                        count(cnt, MX_ST_ORDER_EMPTY);
+                    diffract_rest64(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, m, bp, R.d, R.pol);
+                    blaze = bp.blaze;
+                } else {
+                    if (!select_order<PREC>(T, ei[1], R.energy, u, m, ptot, lds,
+                                            [&]() { return blaze64<EXT>(et, ep, ly, R.d); }))
+                        count(cnt, MX_ST_ORDER_EMPTY);
+                    blaze = diffract64<EXT>(et, ep, ei[2], ei[3] ? 1.0 : -1.0, R.energy, ly, m, R.d, R.pol);
+                }
             } else {
                 const float wave = fdiv_fast(MX_E2W_F, (float)R.energy);
                 if (L && si[22] >= 0) {
                     const float *prm = (const float *)(lds + si[22]) + (size_t)hit * 16;
                     const float *er = (const float *)rec_l + (size_t)hit * MX_ET;
                     id = __float_as_int(prm[9]);
-                    if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds,
-                                            [&]() { return (double)blaze32(ldf3(er + 3), prm, (float)ly, R.df); }))
-                        count(cnt, MX_ST_ORDER_EMPTY);
-                    blaze = (double)diffract32(ldf3(er + 3), prm, wave, (float)ly, m, R.df, R.polf);
+                    if (BZ) {
+                        const BlazeParts32 bp = blaze_parts32(ldf3(er + 3), prm, (float)ly, R.df);
+                        if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds,
+                                                [&]() { return (double)bp.blaze; }))
+                            count(cnt, MX_ST_ORDER_EMPTY);
+                        diffract_rest32(ldf3(er + 3), prm, wave, m, bp, R.df, R.polf);
+                        blaze = (double)bp.blaze;
+                    } else {
+                        if (!select_order<PREC>(T, __float_as_int(prm[10]), R.energy, u, m, ptot, lds, [&]() {
+                                return (double)blaze32(ldf3(er + 3), prm, (float)ly, R.df);
+                            }))
+                            count(cnt, MX_ST_ORDER_EMPTY);
+                        blaze = (double)diffract32(ldf3(er + 3), prm, wave, (float)ly, m, R.df, R.polf);
+                    }
                 } else {
                     id = ei[0];
                     float prm[16];
@@ -1797,7 +1926,7 @@ __device__ __forceinline__ void init_ray(const MxTables &T, const MxPhotons &P, 
 }
 
 // Stage s (wave-uniform) for one photon.
-template <int PREC, bool EXT>
+template <int PREC, bool EXT, bool BZ = EXT>
 __device__ __forceinline__ void run_stage(const MxTables &T, const MxPhotons &P, const int s,
                                           const bool use_grid, const Counters &cnt,
                                           const unsigned char *lds, Ray &R) {
@@ -1858,8 +1987,8 @@ __device__ __forceinline__ void run_stage(const MxTables &T, const MxPhotons &P,
             count(cnt, MX_ST_BAD_ARGS);
             return;
         }
-        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, EXT>(T, P, s, i, use_grid, cnt, lds, R);
-        else array_stage<PREC, false, EXT>(T, P, s, i, use_grid, cnt, lds, R);
+        if (si[PREC == 64 ? 21 : 20] >= 0) array_stage<PREC, true, EXT, 0, BZ>(T, P, s, i, use_grid, cnt, lds, R);
+        else array_stage<PREC, false, EXT, 0, BZ>(T, P, s, i, use_grid, cnt, lds, R);
     }
 }
 
@@ -1875,7 +2004,7 @@ __device__ __forceinline__ void finish_ray(const MxPhotons &P, Ray &R) {
 }
 
 // One photon through all stages.
-template <int PREC, bool EXT>
+template <int PREC, bool EXT, bool BZ = EXT>
 __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P, const int64_t i,
                                           const bool use_grid, const Counters &cnt,
                                           const unsigned char *lds) {
@@ -1885,7 +2014,7 @@ __device__ __forceinline__ void trace_one(const MxTables &T, const MxPhotons &P,
         // the stage index is the same in every lane: keep it (and all stage
         // metadata derived from it) in scalar registers
         const int s = __builtin_amdgcn_readfirstlane(s_);
-        run_stage<PREC, EXT>(T, P, s, use_grid, cnt, lds, R);
+        run_stage<PREC, EXT, BZ>(T, P, s, use_grid, cnt, lds, R);
     }
     finish_ray<PREC>(P, R);
 }
@@ -2084,7 +2213,11 @@ __device__ __forceinline__ void trace_chunk(const MxTables &T, const MxPhotons &
 // stage_i/sel_i) once, then walks photons with a grid stride.
 // EXT: the tables hold stack or tube stages (a separate instantiation, so that the
 // layer code does not weigh on the register allocation of the others).
-template <int PREC, int BS, bool EXT>
+// BZ: an order selector depends on the blaze angle (interpolated tables), so
+// the grating stages form the angle once for the selector and the diffraction
+// (the extended kernels always do; the plain ones without such a selector
+// keep the diffraction's own evaluation, which schedules better there).
+template <int PREC, int BS, bool EXT, bool BZ = EXT>
 __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons P_arg, int flags,
                                                        unsigned long long *status) {
     // Read the two structs from the kernel-argument segment through
@@ -2120,7 +2253,7 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons 
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const int64_t stride = (int64_t)gridDim.x * BS;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
-        trace_one<PREC, EXT>(T, P, i, use_grid, cnt, lds);
+        trace_one<PREC, EXT, BZ>(T, P, i, use_grid, cnt, lds);
     __syncthreads();
     if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
@@ -2547,12 +2680,18 @@ extern "C" int mx_trace(const MxTables *tables, const MxPhotons *photons, int32_
     if (precision == MX_PREC_FP64 && ext)
         hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK_EXT, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
+    else if (precision == MX_PREC_FP64 && tables->blaze_stages > 0)
+        hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64, false, true>), dim3((unsigned)grid), dim3(block), lds,
+                           st, *tables, *photons, (int)flags, status);
     else if (precision == MX_PREC_FP64)
         hipLaunchKernelGGL((mx_trace_kernel<64, MX_BLOCK64, false>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
     else if (ext)
         hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK_EXT32, true>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);
+    else if (tables->blaze_stages > 0)
+        hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32, false, true>), dim3((unsigned)grid), dim3(block), lds,
+                           st, *tables, *photons, (int)flags, status);
     else
         hipLaunchKernelGGL((mx_trace_kernel<32, MX_BLOCK32, false>), dim3((unsigned)grid), dim3(block), lds, st,
                            *tables, *photons, (int)flags, status);

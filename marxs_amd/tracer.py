@@ -78,6 +78,7 @@ class DeviceTables:
         s.regroup_stages = int(sum(1 for k in range(len(self.host.stages))
                                    if self.host.stage_i[k, 0] in (STAGE_GRATING, STAGE_DETECTOR, STAGE_STACK)
                                    and self.host.stage_i[k, 3] & 1))
+        s.blaze_stages = int(np.sum(self.host.stage_i[:, 50] == 1))
         return s
 
     def struct_for(self, existing):

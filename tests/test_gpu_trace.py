@@ -359,6 +359,33 @@ def test_interpolated_total_estimate_only_guides(gpu, scale):
     assert len(set(got['order'][got['facet'] >= 0].tolist())) >= 10
 
 
+def test_blaze_sharing_kernel_equals_plain(gpu):
+    """Tables with a blaze-dependent selector (blaze_stages > 0) run the
+    plain kernels that form the blaze angle once for the selector and the
+    diffraction.  The kernels that form it in the diffraction, run on the
+    same tables with blaze_stages cleared, give FP64 columns identical bit
+    for bit, and FP32 orders equal for all but float-rounding cases."""
+    inst, parts = configs.cat_rowland_instrument(200, efficiency=configs.blaze_interp_efficiency())
+    ci = _ci(inst)
+    assert ci.dev_tables.struct.blaze_stages == 1
+    ph, u = make_inputs(30000, 24, ci.n_uniform_slots, 'lines')
+    out = {}
+    for bz in (1, 0):
+        ci.dev_tables.struct.blaze_stages = bz
+        for prec in (64, 32):
+            out[bz, prec], st = gpu_trace(ci, ph, u, precision=prec)
+    ci.dev_tables.struct.blaze_stages = 1
+    a, b = out[1, 64], out[0, 64]
+    assert set(a) == set(b)
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=a[k].dtype.kind == 'f'), k
+    assert (a['facet'] >= 0).sum() > 3000
+    a, b = out[1, 32], out[0, 32]
+    same = (a['facet'] == b['facet']) & (a['facet'] >= 0)
+    assert same.sum() > 3000
+    assert float((a['order'] == b['order'])[same].mean()) >= 0.999
+
+
 def test_cat_rowland_blaze_dependent_table_fp32(gpu):
     """The FP32 kernel reads the interpolated table from its LDS copy (values
     rounded to float32): against the oracle, orders agree for all but a few

@@ -398,3 +398,29 @@ def test_interpolated_table_fp64_lds_copy():
     assert np.array_equal(img[o0:o0 + 4 * no].view(np.int32), t.sel_o[row[5]:row[5] + no])
     # the image header row of the selector is the table's row
     assert np.array_equal(img[:4 * C.SELI].view(np.int32), t.sel_i[0])
+
+
+def test_blaze_selector_stage_flag():
+    """stage_i[50] = 1 marks a grating or stack stage with an order selector
+    that depends on the blaze angle (an interpolated table, the membrane's or
+    the L1's); the device struct counts such stages in blaze_stages, which
+    selects the kernels that form the angle once."""
+    from marxs_amd.compiler import STAGE_GRATING, STAGE_STACK
+    inst, parts = configs.cat_rowland_instrument(20, efficiency=configs.blaze_interp_efficiency())
+    t = C.compile_tree(inst)
+    kinds = t.stage_i[:, 0]
+    assert t.stage_i[kinds == STAGE_GRATING, 50].tolist() == [1]
+    assert np.all(t.stage_i[kinds != STAGE_GRATING, 50] == 0)
+    inst, parts = configs.cat_rowland_instrument(20)
+    assert np.all(C.compile_tree(inst).stage_i[:, 50] == 0)
+    # stacks: a tabulated membrane selector (the L1 selector is one too)
+    # leaves the flag at 0, an interpolated one sets it
+    wave = np.linspace(0.4, 6.5, 9)
+    theta = np.deg2rad(np.linspace(0, 3, 7))
+    tab = mx.InterpolateEfficiencyTable.from_arrays(wave, theta, np.arange(-1, 2),
+                                                    np.full((3, 9, 7), 0.2))
+    for sel, flag in ((mx.OrderSelector([-1, 0, 1], [0.3, 0.4, 0.3]), 0), (tab, 1)):
+        st = CG.CATL1L2Stack(order_selector=sel, l1_transmission=CG.l1transtab_path,
+                             zoom=[1, 25, 25], position=[50, 0, 0])
+        t = C.compile_tree(mx.Sequence(elements=[st]))
+        assert t.stage_i[t.stage_i[:, 0] == STAGE_STACK, 50].tolist() == [flag]
