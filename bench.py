@@ -618,9 +618,9 @@ def bench_stack(args, dev):
         b.record()
         torch.cuda.synchronize()
         ms[prec] = a.elapsed_time(b) / args.steps
-    # the other form of each precision: the one-kernel trace and the split
-    # trace (two kernels with a photon queue between them; the default for
-    # FP64); FP64 results must be equal bit for bit
+    # the other form of each precision: the split trace (two kernels with a
+    # photon queue between them; opt-in) beside the one-kernel default; FP64
+    # results must be equal bit for bit
     split_ms, split_eq = {}, None
     if ci.tables.split_stage is not None:
         for prec in (32, 64):

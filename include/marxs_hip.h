@@ -119,9 +119,10 @@ extern "C" {
 #define MX_FLAG_REGROUP 2  /* regroup the rays of a workgroup before every culled array stage
                               (results are identical either way; measured slower on the
                               600-facet workload, see mx_trace_regroup_pairs) */
-/* bits 8..15: workgroups launched per CU (persistent grid); 0 = the library
- * default: 4 for the FP32 kernel, 2 for the FP64 kernel, 1 for the extended
- * kernels (stack, tube or lens-reflectivity stages) */
+/* bits 8..15: workgroups launched per CU (persistent grid; each takes a
+ * contiguous share of the photons, which its waves take in chunks of 64);
+ * 0 = the library default: 2 for the FP32 kernel, 1 for the FP64 kernel, 1
+ * for the extended kernels (stack, tube or lens-reflectivity stages) */
 #define MX_FLAG_WG_PER_CU(n) (((n) & 0xff) << 8)
 /* bits 16..23: regroup a workgroup's rays before a stage only when at most
  * n/256 of them can interact with it (1..255; 0 = the library default) */

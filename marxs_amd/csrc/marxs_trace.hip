@@ -52,14 +52,15 @@
 #define MX_BLOCK_EXT32 768      /* the FP32 kernels with stack or tube stages */
 #endif
 // Default workgroups per CU of the persistent grid (one resides at a time:
-// the LDS image; more give finer load balance but stage the image again):
-// measured on the 600-facet workloads, 1e7 photons (FP32 / FP64 kernel ms at
-// 1, 2, 4, 16 per CU): plain facets 1.035 1.021 1.019 1.043 / 1.454 1.468
-// 1.470 1.509; interpolated efficiency 1.133 1.098 1.090 1.135 / 1.805
-// 1.802 1.813 1.902; CATL1L2Stack (extended kernel) 2.00 2.02 2.01 2.06 /
-// 2.88 2.89 2.92 3.05.
-#define MX_DEFAULT_WG_PER_CU32 4
-#define MX_DEFAULT_WG_PER_CU64 2
+// the LDS image; each stages it again), whose waves take chunks of 64
+// photons from their workgroup's share: measured on the 600-facet
+// workloads, 1e7 photons (FP32 / FP64 kernel ms at 1, 2, 4 per CU): plain
+// facets 0.944 0.925 0.943 / 1.301 1.309 1.326; interpolated efficiency
+// 0.984 0.976 0.994 / 1.498 1.507 1.533; CATL1L2Stack (extended kernel)
+// 1.931 1.938 1.956 / (split trace) 2.737 2.719 2.718; mirror scatter,
+// facets and Rowland tube (extended) 1.657 1.649 1.664 / 1.906 1.916 1.945.
+#define MX_DEFAULT_WG_PER_CU32 2
+#define MX_DEFAULT_WG_PER_CU64 1
 #define MX_DEFAULT_WG_PER_CU_EXT 1
 #ifndef MX_REGROUP_DEFAULT_FRAC
 #define MX_REGROUP_DEFAULT_FRAC 256  /* exchange when <= this/256 of the rays interact */
@@ -2208,6 +2209,30 @@ __device__ __forceinline__ void trace_chunk(const MxTables &T, const MxPhotons &
     if (R.i >= 0) finish_ray<PREC>(P, R);
 }
 
+// The photon loop of a persistent workgroup: its contiguous share of the n
+// photons (rounded up to 64), in chunks of 64 consecutive photons, one per
+// lane, that its waves take in turn from an LDS counter (*next, zero before
+// the call and a barrier).  A wave whose photons interact little takes more
+// chunks, so the waves of a workgroup end within one chunk of each other;
+// with a fixed share per wave (a grid-stride loop) the workgroup waited for
+// its slowest wave: 1e7 photons, 600 facets, FP32 1.008 -> 0.924 ms, FP64
+// 1.458 -> 1.296 ms.
+template <typename F>
+__device__ __forceinline__ void photon_chunks(const int64_t n, unsigned *next, F &&f) {
+    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + 63) & ~(int64_t)63;
+    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
+    const int lane = (int)(threadIdx.x & 63);
+    for (;;) {
+        unsigned c = 0;
+        if (lane == 0) c = atomicAdd(next, 64u);
+        c = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl((int)c, 0));
+        const int64_t i0 = lo + (int64_t)c;
+        if (i0 >= hi) break;
+        if (i0 + lane < hi) f(i0 + lane);
+    }
+}
+
 // Persistent kernel: each workgroup copies the LDS image (element records,
 // parameters, culling lists and selector tables of the stages marked in
 // stage_i/sel_i) once, then walks photons with a grid stride.
@@ -2248,12 +2273,12 @@ __global__ __launch_bounds__(BS) void mx_trace_kernel(MxTables T_arg, MxPhotons 
         const int nvec = (lds_bytes + 15) >> 4;
         for (int v = threadIdx.x; v < nvec; v += BS) dst[v] = src[v];
     }
+    __shared__ unsigned s_next;
+    if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
     const Counters cnt = {s_count, (flags & MX_FLAG_COUNT_TESTS) != 0};
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
-    const int64_t stride = (int64_t)gridDim.x * BS;
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
-        trace_one<PREC, EXT, BZ>(T, P, i, use_grid, cnt, lds);
+    photon_chunks(P.n, &s_next, [&](int64_t i) { trace_one<PREC, EXT, BZ>(T, P, i, use_grid, cnt, lds); });
     __syncthreads();
     if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
@@ -2380,9 +2405,10 @@ __global__ __launch_bounds__(BS) void mx_trace_split_a_kernel(MxTables T_arg, Mx
     MX_SPLIT_PROLOGUE(BS);
     SplitIO io;
     io.Q = split_queue<PREC>(ws, P.n);
-    const int64_t stride = (int64_t)gridDim.x * BS;
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < P.n; i += stride)
-        trace_one_split_a<PREC>(T, P, i, use_grid, cnt, lds, split, io);
+    __shared__ unsigned s_next;
+    if (threadIdx.x == 0) s_next = 0;
+    __syncthreads();
+    photon_chunks(P.n, &s_next, [&](int64_t i) { trace_one_split_a<PREC>(T, P, i, use_grid, cnt, lds, split, io); });
     __syncthreads();
     if (status && threadIdx.x < MX_NCOUNT && s_count[threadIdx.x])
         atomicAdd(status + threadIdx.x, (unsigned long long)s_count[threadIdx.x]);
@@ -2489,6 +2515,7 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ long long s_tile;
     __shared__ int s_var;
+    __shared__ unsigned s_next;   // next chunk of 64 photons of the tile
     __shared__ unsigned s_count[MX_NCOUNT];
     const bool use_grid = !(flags & MX_FLAG_NO_GRID);
     const Counters cnt = {s_count, (flags & MX_FLAG_COUNT_TESTS) != 0};
@@ -2508,6 +2535,7 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
             }
             s_tile = t;
             s_var = v;
+            s_next = 0;
         }
         __syncthreads();
         const long long t = s_tile;
@@ -2534,7 +2562,15 @@ __global__ __launch_bounds__(BS) void mx_trace_batched_kernel(
         int64_t e = b + tp;
         if (e > P.n) e = P.n;
         if (b < 0 || tp <= 0) e = b;
-        for (int64_t i = b + threadIdx.x; i < e; i += BS) {
+        // the tile's photons in chunks of 64 that the waves take in turn
+        const int lane = (int)(threadIdx.x & 63);
+        for (;;) {
+            unsigned c = 0;
+            if (lane == 0) c = atomicAdd(&s_next, 64u);
+            c = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl((int)c, 0));
+            const int64_t i = b + (int64_t)c + lane;
+            if (b + (int64_t)c >= e) break;
+            if (i >= e) continue;
             // the struct pointers pass through an empty asm for every photon,
             // so that the compiler reloads struct fields where they are used
             // (scalar loads) instead of keeping them live across the loop

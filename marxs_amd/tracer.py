@@ -169,19 +169,19 @@ class CompiledInstrument:
         that waves are all interacting or all passing (identical results;
         off by default: slower on the workloads measured, because of the
         workgroup barriers it needs).
-    split : bool or 'auto'
+    split : bool
         Trace instruments with one array of stack elements (e.g.
         ``CATL1L2Stack`` facets) as a split trace (``mx_trace_split``: the
         plain kernel up to each photon's first stack hit, then the layer
         programs over the queued photons only; FP64 results identical, FP32
         within float rounding) where the tables allow it
-        (``tables.split_stage``).  ``'auto'`` (the default) splits FP64
-        traces only: on the 600-facet ``CATL1L2Stack`` workload the split
-        trace is 4-5 % faster in FP64 (2.76 against 2.88 ms per 1e7 photons)
-        and slower in FP32 (2.48 against 2.00 ms).
+        (``tables.split_stage``).  Off by default: on the 600-facet
+        ``CATL1L2Stack`` workload the one-kernel trace is faster in both
+        precisions (1e7 photons: FP32 1.93 against 2.50 ms, FP64 2.42
+        against 2.73 ms).
     """
 
-    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, split='auto', **grid_kw):
+    def __init__(self, tree, n_draws=2, grid=True, device=None, regroup=False, split=False, **grid_kw):
         self.tree = tree
         self.tables = compile_tree(tree, n_draws=n_draws, grid=grid, **grid_kw)
         self.device = _dev(device)
@@ -234,8 +234,6 @@ class CompiledInstrument:
         if self.regroup if regroup is None else regroup:
             return False
         sp = self.split if split is None else split
-        if sp == 'auto':
-            sp = PREC[precision] == 64
         return bool(sp) and self.tables.split_stage is not None
 
     def launch(self, dp, uniforms=None, precision=64, status=None, stream=None,

@@ -1,6 +1,6 @@
 """The split trace (mx_trace_split: the plain kernel up to each photon's
 first hit of the stack stage, the layer programs and later stages over the
-queued photons; the default for FP64 traces of stack arrays) against the
+queued photons; opt-in, CompiledInstrument(split=True)) against the
 one-kernel trace of the same tables (needs an MI355X).
 
 Photons are independent and the split changes only which kernel runs which
@@ -173,9 +173,9 @@ def test_split_not_used_where_tables_do_not_allow(gpu):
                                                  facet_kwargs={'l1_transmission': SI})
     assert not CompiledInstrument(inst, regroup=True, split=True).uses_split()
     assert not CompiledInstrument(inst, split=False).uses_split()
-    # the default splits FP64 traces only
+    # the default is the one-kernel trace
     ci = CompiledInstrument(inst)
-    assert ci.uses_split(precision=64) and not ci.uses_split(precision=32)
+    assert not ci.uses_split(precision=64) and not ci.uses_split(precision=32)
 
 
 def test_split_oracle_parity_fp64(gpu):
