@@ -2235,7 +2235,8 @@ __device__ __forceinline__ void photon_chunks(const int64_t n, unsigned *next, F
 
 // Persistent kernel: each workgroup copies the LDS image (element records,
 // parameters, culling lists and selector tables of the stages marked in
-// stage_i/sel_i) once, then walks photons with a grid stride.
+// stage_i/sel_i) once, then its waves take chunks of its share of the
+// photons (photon_chunks).
 // EXT: the tables hold stack or tube stages (a separate instantiation, so that the
 // layer code does not weigh on the register allocation of the others).
 // BZ: an order selector depends on the blaze angle (interpolated tables), so
