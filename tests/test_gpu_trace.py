@@ -536,3 +536,40 @@ def test_plane_test_counter(gpu):
             assert sa[7] == n * (1 + 200 + 16)
         else:
             assert n < sa[7] < 0.05 * n * (1 + 200 + 16)
+
+
+@pytest.mark.parametrize('prec', [64, 32])
+def test_every_photon_traced_once_any_grid(gpu, prec):
+    """The waves of a workgroup take 64-photon chunks of its contiguous share
+    of the photons: for photon counts around the chunk and workgroup sizes
+    and grids of 1 to 8 workgroups per CU (and more workgroups than photon
+    chunks), every photon is traced once (output columns filled with a
+    sentinel first are all overwritten) and the results equal those of one
+    workgroup per CU bit for bit (FP64) or exactly (FP32: the same kernel
+    code runs for every photon, only the grid changes)."""
+    import torch
+    from marxs_amd.tracer import CompiledInstrument, DevicePhotons
+    from marxs_amd.batch import outputs_like
+    ci = CompiledInstrument(configs.cat_rowland_instrument(200)[0])
+    for n in (1, 63, 64, 65, 1023, 1025, 64 * 256 + 1, 200_003):
+        ph, u = make_inputs(n, 50 + n % 97, ci.n_uniform_slots, 'lines')
+        src = DevicePhotons(ph['dir'].T, ph['pol'].T, ph['energy'], ph['prob'], device=gpu)
+        uu = torch.as_tensor(u, device=gpu)
+        ref = None
+        for wpc in (1, 2, 3, 8):
+            out = outputs_like(src)
+            for t in (out.pos, out.dir, out.pol, out.prob):
+                t.fill_(-7.5)
+            st = ci.launch(out, uu, prec, src=src, wg_per_cu=wpc).cpu().numpy()
+            cols = {'pos': out.pos, 'dir': out.dir, 'pol': out.pol, 'prob': out.prob}
+            cols.update(out.fcols)
+            cols.update(out.icols)
+            cols = {k: v.cpu().numpy() for k, v in cols.items()}
+            for k in ('pos', 'dir', 'pol', 'prob'):
+                assert not np.any(cols[k] == -7.5), (n, wpc, k)
+            if ref is None:
+                ref = (cols, st)
+                continue
+            for k in cols:
+                assert np.array_equal(cols[k], ref[0][k], equal_nan=cols[k].dtype.kind == 'f'), (n, wpc, k)
+            assert np.array_equal(st, ref[1]), (n, wpc)
