@@ -394,8 +394,12 @@ int mx_order_stats(const MxOrderStats *args, void *stream);
  * device array of MX_OS_STATIC_BYTES (64 MiB) that every GPU the library runs
  * on holds until the process ends; args->workspace is ignored.  The call
  * allocates nothing (no hipMalloc, no allocating library routine).  No
- * counterpart in the reference: it is for callers that cannot allocate (code
- * under HIP graph capture, tight serving loops).  Limit: at most
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (tight serving loops, harnesses that forbid hipMalloc).  Like
+ * mx_order_stats it sorts with hipCUB's DeviceRadixSort, which clears its
+ * per-pass state with hipMemsetAsync; a graph captured around either call
+ * holds those memset nodes, so the calls are not offered for graph capture
+ * (the trace entry points clear with kernels and are).  Limit: at most
  * MX_OS_STATIC_BYTES of workspace as mx_order_stats_workspace counts it
  * (about 2.4e6 photons with a value column, 15 orders); larger calls return
  * 1 with an error.  Calls on one GPU share the array: calls that overlap on
