@@ -643,7 +643,8 @@ extern "C" int mx_order_stats(const MxOrderStats *a, void *stream) {
 // ------------------------------------------------ allocation-free variant
 // mx_order_stats_static: mx_order_stats with its workspace in `scratch`, a
 // file-scope device array of MX_OS_STATIC_BYTES, for callers that cannot
-// allocate (code under graph capture, tight serving loops).  Every GPU the
+// allocate (tight serving loops, harnesses that forbid hipMalloc; not graph
+// capture: the radix sorts clear their pass state with memsets).  Every GPU the
 // library runs on holds its own copy until the process ends.  Host code
 // never passes the array's name to a kernel or a copy (in host code it is a
 // host-side stand-in): the device address of this GPU's copy comes from
