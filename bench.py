@@ -202,6 +202,18 @@ def main():
     status = torch.zeros(8, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
 
+    # the persistent grid (workgroups per CU) measured on this GPU before the
+    # timed steps (marxs_amd.autotune.tune_trace: 1 to 4 per CU, median of 5
+    # launches each); every later launch of this instrument uses it
+    from marxs_amd.autotune import tune_trace
+    grid_tuned = {}
+    for prec, out in ((32, out32), (64, out64)):
+        if args.only and args.only != f'fp{prec}':
+            continue
+        best, table = tune_trace(ci, out, uni, prec, src=src)
+        grid_tuned[f'fp{prec}'] = {'wg_per_cu': best, 'ms': {str(k): round(v, 4) for k, v in table.items()}}
+    log('grid tuned', grid_tuned)
+
     def step(out, prec):
         ci.launch(out, uni, precision=prec, src=src, status=status)
 
@@ -416,6 +428,7 @@ def main():
         'fp32_error_vs_fp64': err,
         'kernel_ms_fp32': k32 * 1e3,
         'kernel_ms_fp64': k64 * 1e3,
+        'grid_tuned': grid_tuned,
         'with_regrouping': regrouped,
         # plane tests the kernels made (status counter), per second of step
         'photon_element_tests': {
